@@ -1,0 +1,876 @@
+// cfdmx_ctx.hip -- context, grid generation and the C-ABI of include/cfdmx.h.
+#include "cfdmx.h"
+#include "cfdmx_internal.hpp"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <utility>
+#include <vector>
+
+using namespace cfdmx;
+
+struct cfdmx_ctx {
+    cfdmx_config cfg;
+    Geo g;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // host metrics (grid::X ... Dzs)
+    std::vector<double> X, Y, Z, Dx, Dy, Dz, Dxs, Dys, Dzs;
+    // device memory
+    std::vector<void *> allocs;
+    double *metric = nullptr;
+    double *Xn = nullptr, *Yn = nullptr;
+    double *t0[3] = {}, *t1[3] = {}, *t3[3] = {};
+    double *p = nullptr, *eta = nullptr, *f = nullptr;
+    double *x = nullptr, *b = nullptr, *px1 = nullptr, *px2 = nullptr;
+    BicgBuffers B = {};
+    double *part = nullptr, *cfl_part = nullptr, *flux_part = nullptr;
+    double *scal = nullptr; // [0] cfl, [1] cd, [2] cl
+    double *inflow = nullptr;
+    KState *kst = nullptr;
+    SorState *sst = nullptr;
+    int *count = nullptr;
+    // pinned host mirrors
+    KState *kst_h = nullptr;
+    SorState *sst_h = nullptr;
+    double *scal_h = nullptr;
+    int *count_h = nullptr;
+    BCParams bc = {};
+    int loop = 1;
+    double time = 0.0;
+    int resets = 0;
+    int started = 0;
+};
+
+static thread_local std::string g_create_error;
+
+#define FAIL(code, ...)                                                                        \
+    do {                                                                                       \
+        char buf_[512];                                                                        \
+        std::snprintf(buf_, sizeof buf_, __VA_ARGS__);                                         \
+        ctx->err = buf_;                                                                       \
+        return code;                                                                           \
+    } while (0)
+
+#define HIPCHK(expr)                                                                           \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            FAIL(CFDMX_ERR_HIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__,       \
+                 __LINE__);                                                                    \
+    } while (0)
+
+#define LAUNCHCHK() HIPCHK(hipGetLastError())
+
+// ---------------------------------------------------------------------------
+// grid generation: grid::init_uniform (structureGrid.hpp:336-367),
+// unit_SML / init_nonuniform_SML (:250-329), init_gC_dx (:433-486)
+// ---------------------------------------------------------------------------
+static void gc_dx(std::vector<double> &X, std::vector<double> &D, std::vector<double> &Ds, int n)
+{
+    X[1] = 2 * X[2] - X[3];
+    X[0] = 2 * X[1] - X[2];
+    X[n - 1] = 2 * X[n - 2] - X[n - 3];
+    X[n] = 2 * X[n - 1] - X[n - 2];
+    for (int i = kGC; i < n - kGC - 1; ++i) Ds[i] = (X[i + 2] - X[i]) / 2.0;
+    for (int i = kGC; i < n - kGC - 1; ++i) D[i] = (X[i + 1] - X[i]);
+    D[0] = D[1] = D[2];
+    Ds[0] = Ds[1] = Ds[2];
+    D[n - 1] = D[n - 2] = D[n - 3] = X[n - 2] - X[n - 3];
+    Ds[n - 1] = Ds[n - 2] = Ds[n - 3] = Ds[n - 4];
+}
+
+static bool unit_sml(std::vector<double> &a, double lsml, int ns, int off_ns, double l, int n)
+{
+    const int ncal = n - 2 * kGC;
+    const double dS = lsml / ns, d = (l - lsml) / (ncal - ns);
+    a[kGC] = 0.0;
+    if (ns != 0) {
+        if ((n - ns) % 2 != 0) return false; // structureGrid.hpp:271
+        int nlEnd0 = (ncal - ns) / 2;
+        int nsEnd0 = nlEnd0 + ns;
+        const int nlEnd1 = nsEnd0 + nlEnd0;
+        nlEnd0 += off_ns;
+        nsEnd0 += off_ns;
+        if (nlEnd0 <= 0) return false; // structureGrid.hpp:287
+        for (int i = 0; i < nlEnd0; i++) a[i + kGC + 1] = a[i + kGC] + d;
+        for (int i = nlEnd0; i < nsEnd0; ++i) a[i + kGC + 1] = a[i + kGC] + dS;
+        for (int i = nsEnd0; i < nlEnd1; i++) a[i + kGC + 1] = a[i + kGC] + d;
+    } else {
+        for (int i = kGC; i < n - kGC; i++) a[i + 1] = a[i] + l / ncal;
+    }
+    return true;
+}
+
+static int build_grid(cfdmx_ctx *ctx)
+{
+    const cfdmx_config &c = ctx->cfg;
+    const int nx = c.nx + 4, ny = c.ny + 4, nz = c.nz + 4;
+    ctx->X.assign(nx + 1, 0.0);
+    ctx->Y.assign(ny + 1, 0.0);
+    ctx->Z.assign(nz + 1, 0.0);
+    ctx->Dx.assign(nx, 0.0);
+    ctx->Dxs.assign(nx, 0.0);
+    ctx->Dy.assign(ny, 0.0);
+    ctx->Dys.assign(ny, 0.0);
+    ctx->Dz.assign(nz, 0.0);
+    ctx->Dzs.assign(nz, 0.0);
+    if (c.grid_kind == CFDMX_GRID_UNIFORM) {
+        const double dx = c.lx / (nx - 2.0 * kGC), dy = c.ly / (ny - 2.0 * kGC),
+                     dz = c.lz / (nz - 2.0 * kGC);
+        ctx->X[0] = 0.0 - 2.0 * dx;
+        for (int i = 1; i < nx + 1; ++i) ctx->X[i] = ctx->X[i - 1] + dx;
+        ctx->Y[0] = 0.0 - 2.0 * dy;
+        for (int j = 1; j < ny + 1; ++j) ctx->Y[j] = ctx->Y[j - 1] + dy;
+        ctx->Z[0] = 0.0 - 2.0 * dz;
+        for (int k = 1; k < nz + 1; ++k) ctx->Z[k] = ctx->Z[k - 1] + dz;
+        for (int i = 0; i < nx; ++i) ctx->Dx[i] = ctx->Dxs[i] = dx;
+        for (int j = 0; j < ny; ++j) ctx->Dy[j] = ctx->Dys[j] = dy;
+        for (int k = 0; k < nz; ++k) ctx->Dz[k] = ctx->Dzs[k] = dz;
+        return CFDMX_OK;
+    }
+    if (c.grid_kind == CFDMX_GRID_SML) {
+        if (!unit_sml(ctx->X, 0.3 * c.lx, 100, -4, c.lx, nx) ||
+            !unit_sml(ctx->Y, 0.3 * c.ly, 100, 0, c.ly, ny) ||
+            !unit_sml(ctx->Z, 0.5 * c.lz, 0, 0, c.lz, nz))
+            FAIL(CFDMX_ERR_ARG, "SML grid: nx must be even and >= 110, ny even and >= 102 "
+                                "(structureGrid.hpp:271,287)");
+        gc_dx(ctx->X, ctx->Dx, ctx->Dxs, nx);
+        gc_dx(ctx->Y, ctx->Dy, ctx->Dys, ny);
+        gc_dx(ctx->Z, ctx->Dz, ctx->Dzs, nz);
+        return CFDMX_OK;
+    }
+    FAIL(CFDMX_ERR_ARG, "unknown grid_kind %d", c.grid_kind);
+}
+
+template <class T> static int dalloc(cfdmx_ctx *ctx, T **ptr, size_t count)
+{
+    void *p = nullptr;
+    HIPCHK(hipMalloc(&p, count * sizeof(T) > 0 ? count * sizeof(T) : 8));
+    ctx->allocs.push_back(p);
+    *ptr = static_cast<T *>(p);
+    return CFDMX_OK;
+}
+
+#define ALLOC(ptr, count)                                                                      \
+    do {                                                                                       \
+        int rc_ = dalloc(ctx, &(ptr), (size_t)(count));                                        \
+        if (rc_) return rc_;                                                                   \
+    } while (0)
+
+// metric tables of one axis (cfdmx_device.hpp:Axis) appended to buf
+static void axis_tables(const std::vector<double> &D, const std::vector<double> &Ds,
+                        std::vector<double> &buf, size_t off[13])
+{
+    const int n = (int)D.size();
+    std::vector<double> t(n);
+    auto push = [&](int slot, auto fn) {
+        off[slot] = buf.size();
+        for (int I = 0; I < n; ++I) buf.push_back(fn(I));
+    };
+    push(0, [&](int I) { return D[I]; });
+    push(1, [&](int I) { return Ds[I]; });
+    push(2, [&](int I) { return 1.0 / D[I]; });
+    push(3, [&](int I) { return 1.0 / Ds[I]; });
+    push(4, [&](int I) { return (Ds[I] * 0.5) / D[I]; });
+    auto ok = [&](int I) { return I >= 1 && I + 1 < n; };
+    push(5, [&](int I) { return ok(I) ? 0.125 * D[I + 1] * D[I + 1] / Ds[I] : 0.0; });
+    push(6, [&](int I) { return ok(I) ? 0.125 * D[I + 1] * D[I + 1] / Ds[I + 1] : 0.0; });
+    push(7, [&](int I) { return ok(I) ? 0.125 * D[I] * D[I] / Ds[I - 1] : 0.0; });
+    push(8, [&](int I) { return ok(I) ? 0.125 * D[I] * D[I] / Ds[I] : 0.0; });
+    push(9, [&](int I) { return ok(I) ? 0.125 * Ds[I] * Ds[I] / D[I] : 0.0; });
+    push(10, [&](int I) { return ok(I) ? 0.125 * Ds[I] * Ds[I] / D[I + 1] : 0.0; });
+    push(11, [&](int I) { return ok(I) ? 0.125 * Ds[I - 1] * Ds[I - 1] / D[I - 1] : 0.0; });
+    push(12, [&](int I) { return ok(I) ? 0.125 * Ds[I - 1] * Ds[I - 1] / D[I] : 0.0; });
+}
+
+static int setup_device(cfdmx_ctx *ctx)
+{
+    cfdmx_config &c = ctx->cfg;
+    Geo &g = ctx->g;
+    g.nxc = c.nx;
+    g.nyc = c.ny;
+    g.nzc = c.nz;
+    g.nx = c.nx + 4;
+    g.ny = c.ny + 4;
+    g.nz = c.nz + 4;
+    g.N = c.nx * c.ny * c.nz;
+    g.G = g.nx * g.ny * g.nz;
+    g.sx = g.ny * g.nz;
+    g.sy = g.nz;
+    g.cyz = c.ny * c.nz;
+    g.zper = c.z_periodic ? 1 : 0;
+    g.nblocks = (g.N + kPerBlock - 1) / kPerBlock;
+    g.xcd_remap = c.xcd_remap ? 1 : 0;
+    g.dz = make_fastdiv((unsigned)c.nz);
+    g.dy = make_fastdiv((unsigned)c.ny);
+
+    // metric tables in one buffer
+    std::vector<double> buf;
+    size_t off[3][13];
+    axis_tables(ctx->Dx, ctx->Dxs, buf, off[0]);
+    axis_tables(ctx->Dy, ctx->Dys, buf, off[1]);
+    axis_tables(ctx->Dz, ctx->Dzs, buf, off[2]);
+    // pressure coefficients: xm(i) = 1/Dx[i+2]/Dxs[i+1], xp(i) = 1/Dx[i+2]/Dxs[i+2]
+    size_t poff[3][2];
+    const std::vector<double> *DD[3] = {&ctx->Dx, &ctx->Dy, &ctx->Dz};
+    const std::vector<double> *DS[3] = {&ctx->Dxs, &ctx->Dys, &ctx->Dzs};
+    const int nc[3] = {c.nx, c.ny, c.nz};
+    for (int a = 0; a < 3; ++a) {
+        poff[a][0] = buf.size();
+        for (int i = 0; i < nc[a]; ++i)
+            buf.push_back(1.0 / (*DD[a])[i + kGC] / (*DS[a])[i + kGC - 1]);
+        poff[a][1] = buf.size();
+        for (int i = 0; i < nc[a]; ++i)
+            buf.push_back(1.0 / (*DD[a])[i + kGC] / (*DS[a])[i + kGC]);
+    }
+    const size_t xoff = buf.size();
+    buf.insert(buf.end(), ctx->X.begin(), ctx->X.end());
+    const size_t yoff = buf.size();
+    buf.insert(buf.end(), ctx->Y.begin(), ctx->Y.end());
+    ALLOC(ctx->metric, buf.size());
+    HIPCHK(hipMemcpy(ctx->metric, buf.data(), buf.size() * sizeof(double),
+                     hipMemcpyHostToDevice));
+    const double *m = ctx->metric;
+    for (int a = 0; a < 3; ++a) {
+        Axis &A = g.ax[a];
+        A.D = m + off[a][0];
+        A.Ds = m + off[a][1];
+        A.rD = m + off[a][2];
+        A.rDs = m + off[a][3];
+        A.IT = m + off[a][4];
+        for (int q = 0; q < 4; ++q) {
+            A.qa[q] = m + off[a][5 + q];
+            A.qb[q] = m + off[a][9 + q];
+        }
+        g.pa[a].m = m + poff[a][0];
+        g.pa[a].p = m + poff[a][1];
+    }
+    ctx->Xn = ctx->metric + xoff;
+    ctx->Yn = ctx->metric + yoff;
+
+    // fields
+    const size_t G = g.G, N = g.N;
+    for (int q = 0; q < 3; ++q) {
+        ALLOC(ctx->t0[q], G);
+        ALLOC(ctx->t1[q], G);
+        ALLOC(ctx->t3[q], G);
+    }
+    ALLOC(ctx->p, G);
+    ALLOC(ctx->eta, G);
+    ALLOC(ctx->f, 3 * G);
+    ALLOC(ctx->x, N);
+    ALLOC(ctx->b, N);
+    ALLOC(ctx->px1, N);
+    ALLOC(ctx->px2, N);
+    ALLOC(ctx->B.r0, N);
+    ALLOC(ctx->B.r, N);
+    ALLOC(ctx->B.as, N);
+    ALLOC(ctx->B.p[0], N);
+    ALLOC(ctx->B.p[1], N);
+    ALLOC(ctx->B.ap[0], N);
+    ALLOC(ctx->B.ap[1], N);
+    ALLOC(ctx->part, 4 * (size_t)g.nblocks + 64);
+    ALLOC(ctx->cfl_part, (size_t)g.nblocks + 8);
+    ALLOC(ctx->flux_part, 3 * ((size_t)g.cyz / kPerBlock + 2));
+    ALLOC(ctx->scal, 16);
+    ALLOC(ctx->inflow, (size_t)g.ny);
+    ALLOC(ctx->kst, 1);
+    ALLOC(ctx->sst, 1);
+    ALLOC(ctx->count, 1);
+    HIPCHK(hipHostMalloc((void **)&ctx->kst_h, sizeof(KState), hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void **)&ctx->sst_h, sizeof(SorState), hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void **)&ctx->scal_h, 16 * sizeof(double), hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void **)&ctx->count_h, sizeof(int), hipHostMallocDefault));
+
+    // reference initial state: u = 1, v = w = 0 in T0/T1/T3 (RUN_CPU.cpp:66-78),
+    // p = 0, eta = 0, f = 0 (resize_variable), X_result = 0 (RUN_CPU.cpp:120)
+    hipStream_t s = ctx->stream;
+    for (int q = 0; q < 3; ++q) {
+        const double val = q == 0 ? 1.0 : 0.0;
+        launch_fill(s, (int)G, val, ctx->t0[q]);
+        launch_fill(s, (int)G, val, ctx->t1[q]);
+        launch_fill(s, (int)G, val, ctx->t3[q]);
+    }
+    launch_fill(s, (int)G, 0.0, ctx->p);
+    launch_fill(s, (int)G, 0.0, ctx->eta);
+    launch_fill(s, (int)(3 * G), 0.0, ctx->f);
+    double *zs[] = {ctx->x, ctx->b, ctx->px1, ctx->px2, ctx->B.r0, ctx->B.r, ctx->B.as,
+                    ctx->B.p[0], ctx->B.p[1], ctx->B.ap[0], ctx->B.ap[1]};
+    for (double *z : zs) launch_fill(s, (int)N, 0.0, z);
+    launch_fill(s, 16, 0.0, ctx->scal);
+    LAUNCHCHK();
+
+    KState ks = {};
+    ks.tol = c.tol;
+    ks.restart_factor = 1e-7; // pRestartFactor_ (bicgstabRe2.hpp:52)
+    ks.maxit = c.maxit;
+    ks.done = 1;
+    HIPCHK(hipMemcpyAsync(ctx->kst, &ks, sizeof ks, hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+
+    // boundary table
+    BCParams &bc = ctx->bc;
+    std::memcpy(bc.num, c.bc_num, sizeof bc.num);
+    std::memcpy(bc.dir, c.bc_dir, sizeof bc.dir);
+    bc.inflow = nullptr;
+    bc.convective = c.outflow_convective;
+    bc.conv_coef = c.convective_speed * c.dt / ctx->Dx[g.nx - 3];
+    bc.flux_correction = c.flux_correction;
+    return CFDMX_OK;
+}
+
+static int validate(cfdmx_ctx *ctx)
+{
+    const cfdmx_config &c = ctx->cfg;
+    if (c.nx < 4 || c.ny < 4 || c.nz < 4)
+        FAIL(CFDMX_ERR_ARG, "interior sizes must be >= 4 (got %d %d %d)", c.nx, c.ny, c.nz);
+    const double G = (double)(c.nx + 4) * (c.ny + 4) * (c.nz + 4);
+    if (3.0 * G >= 2147483647.0)
+        FAIL(CFDMX_ERR_ARG, "grid too large for 32-bit indexing (3*G = %.0f)", 3.0 * G);
+    const double N = (double)c.nx * c.ny * c.nz;
+    if (N * c.nz >= 1099511627776.0 || (N / c.nz) * c.ny >= 1099511627776.0)
+        FAIL(CFDMX_ERR_ARG, "grid too large for the index decoder");
+    if (!(c.dt > 0.0) || !(c.re > 0.0)) FAIL(CFDMX_ERR_ARG, "dt and re must be positive");
+    if (c.solver != CFDMX_SOLVER_BICGSTAB && c.solver != CFDMX_SOLVER_RBSOR)
+        FAIL(CFDMX_ERR_ARG, "unknown solver %d", c.solver);
+    if (c.precond != CFDMX_PRECOND_NONE && c.precond != CFDMX_PRECOND_JACOBI)
+        FAIL(CFDMX_ERR_ARG, "unknown precond %d", c.precond);
+    if (c.maxit < 0) FAIL(CFDMX_ERR_ARG, "maxit must be >= 0");
+    if (c.precision != CFDMX_PRECISION_FP64)
+        FAIL(CFDMX_ERR_ARG, "precision %d is not available", c.precision);
+    if (c.z_periodic && c.solver == CFDMX_SOLVER_RBSOR && (c.nz % 2) != 0)
+        FAIL(CFDMX_ERR_ARG, "red-black SOR with periodic z needs an even nz");
+    if (c.flux_correction) {
+        const bool walls = c.bc_num[2][1] == 0.0 && c.bc_dir[2][1] == 0.0 &&
+                           c.bc_num[3][1] == 0.0 && c.bc_dir[3][1] == 0.0;
+        const bool zclosed = c.z_periodic || (c.bc_num[4][2] == 0.0 && c.bc_dir[4][2] == 0.0 &&
+                                              c.bc_num[5][2] == 0.0 && c.bc_dir[5][2] == 0.0);
+        if (!walls || !zclosed)
+            FAIL(CFDMX_ERR_ARG, "flux_correction needs zero normal velocity on the y faces "
+                                "and periodic or zero-normal z faces");
+    }
+    return CFDMX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// C-ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int cfdmx_config_default(cfdmx_config *cfg)
+{
+    if (!cfg) return CFDMX_ERR_ARG;
+    std::memset(cfg, 0, sizeof *cfg);
+    cfg->nx = 128;
+    cfg->ny = 64;
+    cfg->nz = 4;
+    cfg->lx = 20.0;
+    cfg->ly = 10.0;
+    cfg->lz = 0.5;
+    cfg->grid_kind = CFDMX_GRID_UNIFORM;
+    cfg->re = 100.0;
+    cfg->dt = 0.002;
+    // FLOWpassing.hpp:6-40
+    const double num[6][3] = {{0, 0, 0}, {1, 1, 1}, {1, 1, 1}, {1, 1, 1}, {1, 1, 1}, {1, 1, 1}};
+    const double dir[6][3] = {{1, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    std::memcpy(cfg->bc_num, num, sizeof num);
+    std::memcpy(cfg->bc_dir, dir, sizeof dir);
+    cfg->convective_speed = 1.0;
+    cfg->solver = CFDMX_SOLVER_BICGSTAB;
+    cfg->precond = CFDMX_PRECOND_NONE;
+    cfg->tol = 1e-3;   // simpulationVariable::p_criteria (0_1_SimulationVariable.hpp:61)
+    cfg->maxit = 5000; // bicgstabRe2::iters_max_ (bicgstabRe2.hpp:47)
+    cfg->sor_omega = 1.8; // p_sor_omega (0_1_SimulationVariable.hpp:65)
+    cfg->skip_first_pressure = 1;
+    cfg->precision = CFDMX_PRECISION_FP64;
+    cfg->inner_tol = 1e-5;
+    cfg->xcd_remap = 1;
+    return CFDMX_OK;
+}
+
+const char *cfdmx_last_error(const cfdmx_ctx *ctx)
+{
+    return ctx ? ctx->err.c_str() : g_create_error.c_str();
+}
+
+int cfdmx_destroy(cfdmx_ctx *ctx)
+{
+    if (!ctx) return CFDMX_OK;
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (void *p : ctx->allocs) (void)hipFree(p);
+    if (ctx->kst_h) (void)hipHostFree(ctx->kst_h);
+    if (ctx->sst_h) (void)hipHostFree(ctx->sst_h);
+    if (ctx->scal_h) (void)hipHostFree(ctx->scal_h);
+    if (ctx->count_h) (void)hipHostFree(ctx->count_h);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return CFDMX_OK;
+}
+
+int cfdmx_create(const cfdmx_config *cfg, cfdmx_ctx **out)
+{
+    if (!cfg || !out) {
+        g_create_error = "cfdmx_create: null argument";
+        return CFDMX_ERR_ARG;
+    }
+    *out = nullptr;
+    cfdmx_ctx *ctx = new cfdmx_ctx();
+    ctx->cfg = *cfg;
+    int rc = validate(ctx);
+    if (!rc) rc = build_grid(ctx);
+    if (!rc) {
+        hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            ctx->err = std::string("hipStreamCreate: ") + hipGetErrorString(e);
+            rc = CFDMX_ERR_HIP;
+        }
+    }
+    if (!rc) rc = setup_device(ctx);
+    if (rc) {
+        g_create_error = ctx->err;
+        cfdmx_destroy(ctx);
+        return rc;
+    }
+    *out = ctx;
+    return CFDMX_OK;
+}
+
+int cfdmx_sizes(const cfdmx_ctx *ctx, long long *G, long long *N)
+{
+    if (!ctx) return CFDMX_ERR_ARG;
+    if (G) *G = ctx->g.G;
+    if (N) *N = ctx->g.N;
+    return CFDMX_OK;
+}
+
+int cfdmx_grid_metrics(const cfdmx_ctx *ctx, double *out)
+{
+    if (!ctx || !out) return CFDMX_ERR_ARG;
+    size_t o = 0;
+    for (const std::vector<double> *v : {&ctx->X, &ctx->Y, &ctx->Z, &ctx->Dx, &ctx->Dy, &ctx->Dz,
+                                         &ctx->Dxs, &ctx->Dys, &ctx->Dzs})
+        for (double d : *v) out[o++] = d;
+    return CFDMX_OK;
+}
+
+static double **vel_set(cfdmx_ctx *ctx, int which)
+{
+    return which == 0 ? ctx->t0 : which == 1 ? ctx->t1 : which == 2 ? ctx->t3 : nullptr;
+}
+
+static int copy_h2d(cfdmx_ctx *ctx, double *dst, const double *src, size_t n)
+{
+    if (!src) FAIL(CFDMX_ERR_ARG, "null host pointer");
+    HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return CFDMX_OK;
+}
+
+static int copy_d2h(cfdmx_ctx *ctx, double *dst, const double *src, size_t n)
+{
+    if (!dst) FAIL(CFDMX_ERR_ARG, "null host pointer");
+    HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return CFDMX_OK;
+}
+
+int cfdmx_upload_velocity(cfdmx_ctx *ctx, int which, const double *u, const double *v,
+                          const double *w)
+{
+    if (!ctx) return CFDMX_ERR_ARG;
+    double **t = vel_set(ctx, which);
+    if (!t) FAIL(CFDMX_ERR_ARG, "which must be 0 (T0), 1 (T1) or 2 (T3)");
+    const double *h[3] = {u, v, w};
+    for (int q = 0; q < 3; ++q) {
+        int rc = copy_h2d(ctx, t[q], h[q], ctx->g.G);
+        if (rc) return rc;
+    }
+    return CFDMX_OK;
+}
+
+int cfdmx_download_velocity(cfdmx_ctx *ctx, int which, double *u, double *v, double *w)
+{
+    if (!ctx) return CFDMX_ERR_ARG;
+    double **t = vel_set(ctx, which);
+    if (!t) FAIL(CFDMX_ERR_ARG, "which must be 0 (T0), 1 (T1) or 2 (T3)");
+    double *h[3] = {u, v, w};
+    for (int q = 0; q < 3; ++q) {
+        int rc = copy_d2h(ctx, h[q], t[q], ctx->g.G);
+        if (rc) return rc;
+    }
+    return CFDMX_OK;
+}
+
+int cfdmx_upload_pressure(cfdmx_ctx *ctx, const double *p)
+{
+    return ctx ? copy_h2d(ctx, ctx->p, p, ctx->g.G) : CFDMX_ERR_ARG;
+}
+int cfdmx_download_pressure(cfdmx_ctx *ctx, double *p)
+{
+    return ctx ? copy_d2h(ctx, p, ctx->p, ctx->g.G) : CFDMX_ERR_ARG;
+}
+int cfdmx_upload_xresult(cfdmx_ctx *ctx, const double *x)
+{
+    return ctx ? copy_h2d(ctx, ctx->x, x, ctx->g.N) : CFDMX_ERR_ARG;
+}
+int cfdmx_download_xresult(cfdmx_ctx *ctx, double *x)
+{
+    return ctx ? copy_d2h(ctx, x, ctx->x, ctx->g.N) : CFDMX_ERR_ARG;
+}
+int cfdmx_download_rhs(cfdmx_ctx *ctx, double *b)
+{
+    return ctx ? copy_d2h(ctx, b, ctx->b, ctx->g.N) : CFDMX_ERR_ARG;
+}
+int cfdmx_upload_eta(cfdmx_ctx *ctx, const double *eta)
+{
+    return ctx ? copy_h2d(ctx, ctx->eta, eta, ctx->g.G) : CFDMX_ERR_ARG;
+}
+int cfdmx_download_eta(cfdmx_ctx *ctx, double *eta)
+{
+    return ctx ? copy_d2h(ctx, eta, ctx->eta, ctx->g.G) : CFDMX_ERR_ARG;
+}
+int cfdmx_download_force(cfdmx_ctx *ctx, double *f)
+{
+    return ctx ? copy_d2h(ctx, f, ctx->f, 3 * (size_t)ctx->g.G) : CFDMX_ERR_ARG;
+}
+
+int cfdmx_set_inflow(cfdmx_ctx *ctx, const double *profile)
+{
+    if (!ctx) return CFDMX_ERR_ARG;
+    if (!profile) {
+        ctx->bc.inflow = nullptr;
+        return CFDMX_OK;
+    }
+    int rc = copy_h2d(ctx, ctx->inflow, profile, ctx->g.ny);
+    if (rc) return rc;
+    ctx->bc.inflow = ctx->inflow;
+    return CFDMX_OK;
+}
+
+static int start_impl(cfdmx_ctx *ctx)
+{
+    hipStream_t s = ctx->stream;
+    const Geo &g = ctx->g;
+    for (int q = 0; q < 3; ++q) {
+        HIPCHK(hipMemcpyAsync(ctx->t1[q], ctx->t0[q], g.G * sizeof(double),
+                              hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(ctx->t3[q], ctx->t0[q], g.G * sizeof(double),
+                              hipMemcpyDeviceToDevice, s));
+    }
+    launch_bc(g, s, ctx->bc, 0, ctx->t0[0], ctx->t0[1], ctx->t0[2], ctx->p, ctx->t1[0],
+              ctx->t1[1], ctx->t1[2], ctx->flux_part);
+    LAUNCHCHK();
+    HIPCHK(hipStreamSynchronize(s));
+    ctx->loop = 1;
+    ctx->time = 0.0;
+    ctx->started = 1;
+    return CFDMX_OK;
+}
+
+int cfdmx_start(cfdmx_ctx *ctx) { return ctx ? start_impl(ctx) : CFDMX_ERR_ARG; }
+
+int cfdmx_init_flow(cfdmx_ctx *ctx, double u0, double amp, unsigned long long seed)
+{
+    if (!ctx) return CFDMX_ERR_ARG;
+    launch_init_flow(ctx->g, ctx->stream, u0, amp, seed, ctx->t0[0], ctx->t0[1], ctx->t0[2],
+                     ctx->p);
+    LAUNCHCHK();
+    return start_impl(ctx);
+}
+
+int cfdmx_set_eta_cylinder_z(cfdmx_ctx *ctx, double cx, double cy, double radius, int nsub)
+{
+    if (!ctx) return CFDMX_ERR_ARG;
+    if (nsub <= 0) FAIL(CFDMX_ERR_ARG, "nsub must be positive");
+    launch_fill(ctx->stream, ctx->g.G, 0.0, ctx->eta); // RUN_CPU.cpp:228
+    launch_eta_cylz(ctx->g, ctx->stream, ctx->Xn, ctx->Yn, cx, cy, radius, nsub,
+                    std::pow(nsub, -2), ctx->eta);
+    LAUNCHCHK();
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return CFDMX_OK;
+}
+
+int cfdmx_set_eta_smoothed(cfdmx_ctx *ctx, double cx, double cy, double R, double eps)
+{
+    // smoothed Heaviside of R - distance on the k = 0 plane, copied along z like
+    // DFIB_CylinderZ (setEta.hpp:200-203); evaluated on the host with libm
+    if (!ctx) return CFDMX_ERR_ARG;
+    if (!(eps > 0.0)) FAIL(CFDMX_ERR_ARG, "eps must be positive");
+    const Geo &g = ctx->g;
+    std::vector<double> eta((size_t)g.G, 0.0);
+    const double pi = 3.14159265358979323846;
+    for (int i = kGC; i < g.nx - kGC; ++i)
+        for (int j = kGC; j < g.ny - kGC; ++j) {
+            const double xc = (ctx->X[i] + ctx->X[i + 1]) * 0.5 - cx;
+            const double yc = (ctx->Y[j] + ctx->Y[j + 1]) * 0.5 - cy;
+            const double phi = R - std::sqrt(xc * xc + yc * yc);
+            double e;
+            if (phi < -eps) e = 0.0;
+            else if (phi > eps) e = 1.0;
+            else e = 0.5 * (1.0 + phi / eps + std::sin(pi * phi / eps) / pi);
+            eta[((size_t)i * g.ny + j) * g.nz] = e;
+        }
+    for (int i = 0; i < g.nx - 1; ++i)
+        for (int j = 0; j < g.ny - 1; ++j)
+            for (int k = 1; k < g.nz - 1; ++k)
+                eta[((size_t)i * g.ny + j) * g.nz + k] = eta[((size_t)i * g.ny + j) * g.nz];
+    return copy_h2d(ctx, ctx->eta, eta.data(), g.G);
+}
+
+int cfdmx_convdiff(cfdmx_ctx *ctx)
+{
+    if (!ctx) return CFDMX_ERR_ARG;
+    launch_convdiff(ctx->g, ctx->stream, ctx->t0[0], ctx->t0[1], ctx->t0[2], ctx->t1[0],
+                    ctx->t1[1], ctx->t1[2], ctx->cfg.dt, 1.0 / ctx->cfg.re, ctx->cfl_part);
+    launch_max_fin(ctx->g, ctx->stream, ctx->cfl_part, ctx->scal + 0);
+    LAUNCHCHK();
+    return CFDMX_OK;
+}
+
+int cfdmx_bc_main(cfdmx_ctx *ctx)
+{
+    if (!ctx) return CFDMX_ERR_ARG;
+    launch_bc(ctx->g, ctx->stream, ctx->bc, 1, ctx->t0[0], ctx->t0[1], ctx->t0[2], ctx->p,
+              nullptr, nullptr, nullptr, ctx->flux_part);
+    LAUNCHCHK();
+    return CFDMX_OK;
+}
+
+int cfdmx_bc_copy(cfdmx_ctx *ctx)
+{
+    if (!ctx) return CFDMX_ERR_ARG;
+    launch_bc_copy(ctx->g, ctx->stream, ctx->t0[0], ctx->t0[1], ctx->t0[2], ctx->t1[0],
+                   ctx->t1[1], ctx->t1[2]);
+    LAUNCHCHK();
+    return CFDMX_OK;
+}
+
+int cfdmx_pressure_rhs(cfdmx_ctx *ctx)
+{
+    if (!ctx) return CFDMX_ERR_ARG;
+    launch_rhs(ctx->g, ctx->stream, ctx->cfg.precond, ctx->t1[0], ctx->t1[1], ctx->t1[2],
+               ctx->cfg.dt, ctx->b);
+    LAUNCHCHK();
+    return CFDMX_OK;
+}
+
+static int solve_bicgstab_impl(cfdmx_ctx *ctx, int *iters, double *res)
+{
+    hipStream_t s = ctx->stream;
+    const Geo &g = ctx->g;
+    const int pre = ctx->cfg.precond;
+    std::swap(ctx->px1, ctx->px2); // px2 <- px1, then px1 <- x (bicgstabRe2.hpp:94-95)
+    launch_bicg_init(g, s, pre, ctx->b, ctx->x, ctx->B.r0, ctx->B.r, ctx->px1, ctx->part,
+                     ctx->kst);
+    LAUNCHCHK();
+    int batch = 4;
+    for (;;) {
+        for (int q = 0; q < batch; ++q)
+            launch_bicg_iter(g, s, pre, ctx->B, ctx->x, ctx->part, ctx->kst);
+        LAUNCHCHK();
+        HIPCHK(hipMemcpyAsync(ctx->kst_h, ctx->kst, sizeof(KState), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if (ctx->kst_h->done) break;
+        batch = batch < 64 ? batch * 2 : 64;
+    }
+    if (ctx->kst_h->done == 2) {
+        // NaN/Inf reset at the iteration limit (bicgstabRe2.hpp:133-153)
+        launch_set_int(s, ctx->count, 0);
+        launch_nanfix(s, g.N, ctx->x, ctx->px1, ctx->count);
+        LAUNCHCHK();
+        HIPCHK(hipMemcpyAsync(ctx->count_h, ctx->count, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        const int check = *ctx->count_h;
+        if (check > g.N * 0.1) {
+            launch_copy(s, g.N, ctx->px2, ctx->x);
+            LAUNCHCHK();
+            ++ctx->resets;
+        } else if (check != 0) {
+            ++ctx->resets;
+        }
+    }
+    if (iters) *iters = ctx->kst_h->iter;
+    if (res) *res = ctx->kst_h->res;
+    return CFDMX_OK;
+}
+
+int cfdmx_solve_bicgstab(cfdmx_ctx *ctx, int *iters, double *residual)
+{
+    if (!ctx) return CFDMX_ERR_ARG;
+    return solve_bicgstab_impl(ctx, iters, residual);
+}
+
+static int solve_rbsor_impl(cfdmx_ctx *ctx, int *iters, double *err)
+{
+    hipStream_t s = ctx->stream;
+    const Geo &g = ctx->g;
+    SorState ss = {};
+    ss.tol = ctx->cfg.tol;
+    ss.itmax = ctx->cfg.maxit;
+    *ctx->sst_h = ss;
+    HIPCHK(hipMemcpyAsync(ctx->sst, ctx->sst_h, sizeof ss, hipMemcpyHostToDevice, s));
+    launch_sor_rhs(g, s, ctx->t1[0], ctx->t1[1], ctx->t1[2], ctx->cfg.dt, ctx->b);
+    LAUNCHCHK();
+    int batch = 4;
+    for (;;) {
+        for (int q = 0; q < batch; ++q)
+            launch_sor_sweep(g, s, ctx->cfg.sor_omega, ctx->b, ctx->p, ctx->part, ctx->sst);
+        LAUNCHCHK();
+        HIPCHK(hipMemcpyAsync(ctx->sst_h, ctx->sst, sizeof ss, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if (ctx->sst_h->done) break;
+        batch = batch < 64 ? batch * 2 : 64;
+    }
+    if (iters) *iters = ctx->sst_h->iters - 1; // SorPipeLine.seq.hpp:123
+    if (err) *err = ctx->sst_h->residual;
+    return CFDMX_OK;
+}
+
+int cfdmx_solve_rbsor(cfdmx_ctx *ctx, int *iters, double *err)
+{
+    if (!ctx) return CFDMX_ERR_ARG;
+    return solve_rbsor_impl(ctx, iters, err);
+}
+
+int cfdmx_transform(cfdmx_ctx *ctx)
+{
+    if (!ctx) return CFDMX_ERR_ARG;
+    launch_transform(ctx->g, ctx->stream, ctx->x, ctx->p);
+    LAUNCHCHK();
+    return CFDMX_OK;
+}
+
+int cfdmx_project_ib(cfdmx_ctx *ctx)
+{
+    if (!ctx) return CFDMX_ERR_ARG;
+    launch_project(ctx->g, ctx->stream, 0, ctx->cfg.dt, ctx->cfg.outflow_convective, ctx->x,
+                   ctx->p, ctx->eta, ctx->t1[0], ctx->t1[1], ctx->t1[2], ctx->t3[0],
+                   ctx->t3[1], ctx->t3[2], ctx->f, ctx->part);
+    launch_cdcl_fin(ctx->g, ctx->stream, ctx->part, ctx->cfg.lz, ctx->scal + 1);
+    LAUNCHCHK();
+    return CFDMX_OK;
+}
+
+int cfdmx_swap_t0_t3(cfdmx_ctx *ctx)
+{
+    if (!ctx) return CFDMX_ERR_ARG;
+    for (int q = 0; q < 3; ++q) std::swap(ctx->t0[q], ctx->t3[q]);
+    return CFDMX_OK;
+}
+
+static int read_scalars(cfdmx_ctx *ctx)
+{
+    HIPCHK(hipMemcpyAsync(ctx->scal_h, ctx->scal, 16 * sizeof(double), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return CFDMX_OK;
+}
+
+int cfdmx_cd_cl(cfdmx_ctx *ctx, double *cd, double *cl)
+{
+    if (!ctx) return CFDMX_ERR_ARG;
+    launch_force_partial(ctx->g, ctx->stream, ctx->f, ctx->part);
+    launch_cdcl_fin(ctx->g, ctx->stream, ctx->part, ctx->cfg.lz, ctx->scal + 1);
+    LAUNCHCHK();
+    int rc = read_scalars(ctx);
+    if (rc) return rc;
+    if (cd) *cd = ctx->scal_h[1];
+    if (cl) *cl = ctx->scal_h[2];
+    return CFDMX_OK;
+}
+
+int cfdmx_cfl(cfdmx_ctx *ctx, double *cfl)
+{
+    if (!ctx) return CFDMX_ERR_ARG;
+    int rc = read_scalars(ctx);
+    if (rc) return rc;
+    if (cfl) *cfl = ctx->scal_h[0];
+    return CFDMX_OK;
+}
+
+int cfdmx_apply_operator(cfdmx_ctx *ctx, const double *x, double *r)
+{
+    if (!ctx) return CFDMX_ERR_ARG;
+    int rc = copy_h2d(ctx, ctx->B.as, x, ctx->g.N);
+    if (rc) return rc;
+    launch_apply(ctx->g, ctx->stream, ctx->cfg.precond, ctx->B.as, ctx->B.r);
+    LAUNCHCHK();
+    return copy_d2h(ctx, r, ctx->B.r, ctx->g.N);
+}
+
+// one iteration of the runSeqOmp loop body (RUN_CPU.cpp:257-436)
+static int step_once(cfdmx_ctx *ctx, int *iters, double *err, int *solved)
+{
+    hipStream_t s = ctx->stream;
+    const Geo &g = ctx->g;
+    const cfdmx_config &c = ctx->cfg;
+    int rc = cfdmx_convdiff(ctx);
+    if (rc) return rc;
+    *iters = 0;
+    *err = 0.0;
+    *solved = 0;
+    if (ctx->loop > 1 || !c.skip_first_pressure) {
+        *solved = 1;
+        if (c.solver == CFDMX_SOLVER_BICGSTAB) {
+            rc = cfdmx_pressure_rhs(ctx);
+            if (!rc) rc = solve_bicgstab_impl(ctx, iters, err);
+        } else {
+            rc = solve_rbsor_impl(ctx, iters, err);
+        }
+        if (rc) return rc;
+    }
+    const int from_x = (c.solver == CFDMX_SOLVER_BICGSTAB && *solved) ? 1 : 0;
+    launch_project(g, s, from_x, c.dt, c.outflow_convective, ctx->x, ctx->p, ctx->eta,
+                   ctx->t1[0], ctx->t1[1], ctx->t1[2], ctx->t3[0], ctx->t3[1], ctx->t3[2],
+                   ctx->f, ctx->part);
+    launch_cdcl_fin(g, s, ctx->part, c.lz, ctx->scal + 1);
+    for (int q = 0; q < 3; ++q) std::swap(ctx->t0[q], ctx->t3[q]);
+    launch_bc(g, s, ctx->bc, 1, ctx->t0[0], ctx->t0[1], ctx->t0[2], ctx->p, ctx->t1[0],
+              ctx->t1[1], ctx->t1[2], ctx->flux_part);
+    LAUNCHCHK();
+    ctx->time += c.dt;
+    ctx->loop += 1;
+    return CFDMX_OK;
+}
+
+int cfdmx_step(cfdmx_ctx *ctx, int nsteps, cfdmx_step_info *info)
+{
+    if (!ctx) return CFDMX_ERR_ARG;
+    if (!ctx->started) FAIL(CFDMX_ERR_STATE, "call cfdmx_init_flow or cfdmx_start first");
+    int iters = 0, solved = 0;
+    double err = 0.0, t_at = ctx->time;
+    for (int q = 0; q < nsteps; ++q) {
+        t_at = ctx->time;
+        int rc = step_once(ctx, &iters, &err, &solved);
+        if (rc) return rc;
+    }
+    if (info) {
+        int rc = read_scalars(ctx);
+        if (rc) return rc;
+        info->cfl = ctx->scal_h[0];
+        info->cd = ctx->scal_h[1];
+        info->cl = ctx->scal_h[2];
+        info->time = t_at;
+        info->iters = iters;
+        info->err = err;
+        info->solved = solved;
+        HIPCHK(hipMemcpyAsync(ctx->kst_h, ctx->kst, sizeof(KState), hipMemcpyDeviceToHost,
+                              ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        info->restarts = ctx->kst_h->restarts;
+        info->resets = ctx->resets;
+    }
+    return CFDMX_OK;
+}
+
+int cfdmx_synchronize(cfdmx_ctx *ctx)
+{
+    if (!ctx) return CFDMX_ERR_ARG;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return CFDMX_OK;
+}
+
+} // extern "C"

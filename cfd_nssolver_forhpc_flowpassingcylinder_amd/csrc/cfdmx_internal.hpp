@@ -1,0 +1,87 @@
+// cfdmx_internal.hpp -- structures shared by the kernels and the context code.
+#pragma once
+#include "cfdmx_device.hpp"
+
+namespace cfdmx {
+
+// Num/Dir boundary table (FLOWpassing.hpp:6-40) and the extra outflow options
+struct BCParams {
+    double num[6][3];
+    double dir[6][3];
+    const double *inflow; // device, ghosted j, or nullptr
+    int convective;
+    double conv_coef; // convective_speed*dt/Dx[nx-3]
+    int flux_correction;
+};
+
+// BiCGSTAB scalars kept on the device (bicgstabRe2.hpp:89-176)
+struct KState {
+    double rho;   // r1r0
+    double alpha; // a
+    double omega; // w
+    double beta;  // b
+    double norm0;
+    double res;
+    double tol;
+    double restart_factor; // pRestartFactor_
+    int iter;              // pIter
+    int maxit;
+    int done; // 0 running, 1 converged, 2 iteration limit
+    int mode; // next phase A: 0 update p, 1 restart, 2 first iteration
+    int timestep;
+    int restarts;
+};
+
+struct SorState {
+    double residual;
+    double tol;
+    int iters;
+    int itmax;
+    int done;
+    int pad;
+};
+
+struct BicgBuffers {
+    double *r0, *r, *as;
+    double *p[2], *ap[2];
+    int cur;
+};
+
+void launch_convdiff(const Geo &g, hipStream_t s, const double *u, const double *v,
+                     const double *w, double *us, double *vs, double *ws, double dt,
+                     double nu, double *cfl_part);
+void launch_bc(const Geo &g, hipStream_t s, const BCParams &bc, int advance, double *u,
+               double *v, double *w, double *p, double *u1, double *v1, double *w1,
+               double *flux_part);
+void launch_bc_copy(const Geo &g, hipStream_t s, const double *u0, const double *v0,
+                    const double *w0, double *u1, double *v1, double *w1);
+void launch_rhs(const Geo &g, hipStream_t s, int pre, const double *us, const double *vs,
+                const double *ws, double dt, double *b);
+void launch_apply(const Geo &g, hipStream_t s, int pre, const double *x, double *r);
+void launch_bicg_init(const Geo &g, hipStream_t s, int pre, const double *b, const double *x,
+                      double *r0, double *r, double *px1, double *part, KState *st);
+void launch_bicg_iter(const Geo &g, hipStream_t s, int pre, BicgBuffers &B, double *x,
+                      double *part, KState *st);
+void launch_nanfix(hipStream_t s, int n, double *x, const double *px1, int *count);
+void launch_copy(hipStream_t s, int n, const double *a, double *b);
+void launch_fill(hipStream_t s, int n, double val, double *a);
+void launch_sor_rhs(const Geo &g, hipStream_t s, const double *us, const double *vs,
+                    const double *ws, double dt, double *cf6);
+void launch_sor_sweep(const Geo &g, hipStream_t s, double omega, const double *cf6, double *p,
+                      double *part, SorState *st);
+void launch_project(const Geo &g, hipStream_t s, int from_x, double dt, int convective,
+                    const double *xres, double *p, const double *eta, const double *us,
+                    const double *vs, const double *ws, double *u, double *v, double *w,
+                    double *f, double *part);
+void launch_cdcl_fin(const Geo &g, hipStream_t s, const double *part, double lz, double *out);
+void launch_force_partial(const Geo &g, hipStream_t s, const double *f, double *part);
+void launch_max_fin(const Geo &g, hipStream_t s, const double *part, double *out);
+void launch_eta_cylz(const Geo &g, hipStream_t s, const double *Xn, const double *Yn, double cx,
+                     double cy, double R, int nsub, double inv_nsub2, double *eta);
+void launch_init_flow(const Geo &g, hipStream_t s, double u0, double amp,
+                      unsigned long long seed, double *u, double *v, double *w, double *p);
+
+void launch_transform(const Geo &g, hipStream_t s, const double *x, double *p);
+void launch_set_int(hipStream_t s, int *a, int val);
+
+} // namespace cfdmx

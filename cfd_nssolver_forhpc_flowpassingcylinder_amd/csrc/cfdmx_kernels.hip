@@ -1,0 +1,1266 @@
+// cfdmx_kernels.hip -- gfx950 kernels of the time step.
+//
+// Every kernel restates the reference expression by expression (same operands,
+// same association; the library is compiled with -ffp-contract=off and divisions
+// by grid metrics use the exactly rounded divr), so results equal the CPU
+// restatement bit for bit; sums use the canonical order of cfdmx_device.hpp.
+#include "cfdmx_internal.hpp"
+
+namespace cfdmx {
+
+// ---------------------------------------------------------------------------
+// QUICK face values, QuickUnitA / QuickUnitB (Convection_Difussion.hpp:20-90).
+// f[0..4] = phi at I-2, I-1, I, I+1, I+2.  The differences over D (A) or Ds (B)
+// are formed once and shared with the diffusion term, which uses the same
+// quotients (Convection_Difussion.hpp:257-263,339-345,420-426).
+// ---------------------------------------------------------------------------
+struct QuickOut {
+    double P, N;
+    double dm1, d0, d1; // A: g0 - g-1, g1 - g0, g2 - g1   B: h-1 - h-2, h0 - h-1, h1 - h0
+};
+
+__device__ __forceinline__ void quick_grads_A(const Axis &a, int I, const double f[5],
+                                              double &gm1, double &g0, double &g1, double &g2)
+{
+    gm1 = divr(f[1] - f[0], a.D[I - 1], a.rD[I - 1]);
+    g0 = divr(f[2] - f[1], a.D[I], a.rD[I]);
+    g1 = divr(f[3] - f[2], a.D[I + 1], a.rD[I + 1]);
+    g2 = divr(f[4] - f[3], a.D[I + 2], a.rD[I + 2]);
+}
+
+__device__ __forceinline__ void quickA(const Axis &a, int I, double Up, double Un,
+                                       const double f[5], double &P, double &N, double &g0,
+                                       double &g1)
+{
+    double gm1, g2;
+    quick_grads_A(a, I, f, gm1, g0, g1, g2);
+    const double cp = Up > 0.0 ? a.qa[0][I] : a.qa[1][I];
+    const double dp = Up > 0.0 ? (g1 - g0) : (g2 - g1);
+    P = 0.5 * (f[2] + f[3]) - cp * dp;
+    const double cn = Un > 0.0 ? a.qa[2][I] : a.qa[3][I];
+    const double dn = Un > 0.0 ? (g0 - gm1) : (g1 - g0);
+    N = 0.5 * (f[1] + f[2]) - cn * dn;
+}
+
+__device__ __forceinline__ void quickB(const Axis &a, int I, double Up, double Un,
+                                       const double f[5], double &P, double &N, double &hm1,
+                                       double &h0)
+{
+    const double hm2 = divr(f[1] - f[0], a.Ds[I - 2], a.rDs[I - 2]);
+    hm1 = divr(f[2] - f[1], a.Ds[I - 1], a.rDs[I - 1]);
+    h0 = divr(f[3] - f[2], a.Ds[I], a.rDs[I]);
+    const double h1 = divr(f[4] - f[3], a.Ds[I + 1], a.rDs[I + 1]);
+    const double cp = Up > 0.0 ? a.qb[0][I] : a.qb[1][I];
+    const double dp = Up > 0.0 ? (h0 - hm1) : (h1 - h0);
+    P = 0.5 * (f[2] + f[3]) - cp * dp;
+    const double cn = Un > 0.0 ? a.qb[2][I] : a.qb[3][I];
+    const double dn = Un > 0.0 ? (hm1 - hm2) : (h0 - hm1);
+    N = 0.5 * (f[1] + f[2]) - cn * dn;
+}
+
+__device__ __forceinline__ void load5(const double *phi, int c, int s, double f[5])
+{
+    f[0] = phi[c - 2 * s];
+    f[1] = phi[c - s];
+    f[2] = phi[c];
+    f[3] = phi[c + s];
+    f[4] = phi[c + 2 * s];
+}
+
+// ---------------------------------------------------------------------------
+// ConvectionDifussion (Convection_Difussion.hpp:174-450): T0 -> T1 for every
+// interior cell's u, v and w faces, skipping the last face in each component's own
+// direction (:198-202,278-284,361-367) unless z is periodic.  Also the per-cell
+// CFL sum of get_Cfl (cfl.hpp:24-63) as a block maximum.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kThreads)
+k_convdiff(Geo g, const double *__restrict__ u, const double *__restrict__ v,
+           const double *__restrict__ w, double *__restrict__ us, double *__restrict__ vs,
+           double *__restrict__ ws, double mdt, double nudt, double dt,
+           double *__restrict__ cfl_part)
+{
+    const int L = logical_block(g);
+    if (L >= g.nblocks) return;
+    const int sx = g.sx, sy = g.sy;
+    const Axis &X = g.ax[0], &Y = g.ax[1], &Z = g.ax[2];
+    double cmax = -1.0e300;
+    for (int c = 0; c < kCells; ++c) {
+        const int n = L * kPerBlock + c * kThreads + threadIdx.x;
+        if (n >= g.N) break;
+        int i, j, k;
+        decode(g, n, i, j, k);
+        const int I = i + kGC, J = j + kGC, K = k + kGC;
+        const int ic = gidx(g, I, J, K);
+        double cfl = 0.0;
+        // ---------------- u (:205-273) ----------------
+        if (I < g.nx - 3) {
+            const double IT = X.IT[I];
+            const double Up = 0.5 * (u[ic + sx] + u[ic]);
+            const double Un = 0.5 * (u[ic - sx] + u[ic]);
+            const double Vp = v[ic] + (v[ic + sx] - v[ic]) * IT;
+            const double Vn = v[ic - sy] + (v[ic - sy + sx] - v[ic - sy]) * IT;
+            const double Wp = w[ic] + (w[ic + sx] - w[ic]) * IT;
+            const double Wn = w[ic - 1] + (w[ic - 1 + sx] - w[ic - 1]) * IT;
+            double f[5], uPx, uNx, uPy, uNy, uPz, uNz, ax0, ax1, ay0, ay1, az0, az1;
+            load5(u, ic, sx, f);
+            quickA(X, I, Up, Un, f, uPx, uNx, ax0, ax1);
+            load5(u, ic, sy, f);
+            quickB(Y, J, Vp, Vn, f, uPy, uNy, ay0, ay1);
+            load5(u, ic, 1, f);
+            quickB(Z, K, Wp, Wn, f, uPz, uNz, az0, az1);
+            const double convection =
+                mdt * (divr(uPx * uPx - uNx * uNx, X.Ds[I], X.rDs[I]) +
+                       divr(uPy * Vp - uNy * Vn, Y.D[J], Y.rD[J]) +
+                       divr(uPz * Wp - uNz * Wn, Z.D[K], Z.rD[K]));
+            // ((u[xp]-u)/Dx[i+1] - (u-u[xm])/Dx[i]) = ax1 - ax0, etc.
+            const double difussion = nudt * (divr(ax1 - ax0, X.Ds[I], X.rDs[I]) +
+                                             divr(ay1 - ay0, Y.D[J], Y.rD[J]) +
+                                             divr(az1 - az0, Z.D[K], Z.rD[K]));
+            const double r = u[ic] + convection + difussion;
+            us[ic] = r;
+            cfl += divr(r, X.Ds[I], X.rDs[I]);
+        }
+        // ---------------- v (:286-354) ----------------
+        if (J < g.ny - 3) {
+            const double IT = Y.IT[J];
+            const double Vp = 0.5 * (v[ic + sy] + v[ic]);
+            const double Vn = 0.5 * (v[ic - sy] + v[ic]);
+            const double Up = u[ic] + (u[ic + sy] - u[ic]) * IT;
+            const double Un = u[ic - sx] + (u[ic - sx + sy] - u[ic - sx]) * IT;
+            const double Wp = w[ic] + (w[ic + sy] - w[ic]) * IT;
+            const double Wn = w[ic - 1] + (w[ic - 1 + sy] - w[ic - 1]) * IT;
+            double f[5], vPx, vNx, vPy, vNy, vPz, vNz, ax0, ax1, ay0, ay1, az0, az1;
+            load5(v, ic, sx, f);
+            quickB(X, I, Up, Un, f, vPx, vNx, ax0, ax1);
+            load5(v, ic, sy, f);
+            quickA(Y, J, Vp, Vn, f, vPy, vNy, ay0, ay1);
+            load5(v, ic, 1, f);
+            quickB(Z, K, Wp, Wn, f, vPz, vNz, az0, az1);
+            const double convection =
+                mdt * (divr(Up * vPx - Un * vNx, X.D[I], X.rD[I]) +
+                       divr(vPy * vPy - vNy * vNy, Y.Ds[J], Y.rDs[J]) +
+                       divr(Wp * vPz - Wn * vNz, Z.D[K], Z.rD[K]));
+            const double difussion = nudt * (divr(ax1 - ax0, X.D[I], X.rD[I]) +
+                                             divr(ay1 - ay0, Y.Ds[J], Y.rDs[J]) +
+                                             divr(az1 - az0, Z.D[K], Z.rD[K]));
+            const double r = v[ic] + convection + difussion;
+            vs[ic] = r;
+            cfl += divr(r, Y.Ds[J], Y.rDs[J]);
+        }
+        // ---------------- w (:368-437) ----------------
+        if (K < g.nz - 3 || g.zper) {
+            const double IT = Z.IT[K];
+            const double Wp = 0.5 * (w[ic + 1] + w[ic]);
+            const double Wn = 0.5 * (w[ic - 1] + w[ic]);
+            const double Up = u[ic] + (u[ic + 1] - u[ic]) * IT;
+            const double Un = u[ic - sx] + (u[ic - sx + 1] - u[ic - sx]) * IT;
+            const double Vp = v[ic] + (v[ic + 1] - v[ic]) * IT;
+            const double Vn = v[ic - sy] + (v[ic - sy + 1] - v[ic - sy]) * IT;
+            double f[5], wPx, wNx, wPy, wNy, wPz, wNz, ax0, ax1, ay0, ay1, az0, az1;
+            load5(w, ic, sx, f);
+            quickB(X, I, Up, Un, f, wPx, wNx, ax0, ax1);
+            load5(w, ic, sy, f);
+            quickB(Y, J, Vp, Vn, f, wPy, wNy, ay0, ay1);
+            load5(w, ic, 1, f);
+            quickA(Z, K, Wp, Wn, f, wPz, wNz, az0, az1);
+            const double convection =
+                mdt * (divr(Up * wPx - Un * wNx, X.D[I], X.rD[I]) +
+                       divr(Vp * wPy - Vn * wNy, Y.D[J], Y.rD[J]) +
+                       divr(wPz * wPz - wNz * wNz, Z.Ds[K], Z.rDs[K]));
+            const double difussion = nudt * (divr(ax1 - ax0, X.D[I], X.rD[I]) +
+                                             divr(ay1 - ay0, Y.D[J], Y.rD[J]) +
+                                             divr(az1 - az0, Z.Ds[K], Z.rDs[K]));
+            const double r = w[ic] + convection + difussion;
+            ws[ic] = r;
+            cfl += divr(r, Z.Ds[K], Z.rDs[K]);
+        }
+        cmax = cfl > cmax ? cfl : cmax;
+        if (g.zper) {
+            // u* z-ghosts are images of the interior (face k = 1 == nz-3)
+            const int nzc = g.nzc;
+            int img = -1;
+            if (K == 2) img = ic + nzc;
+            else if (K == 3) img = ic + nzc;
+            else if (K == g.nz - 3) img = ic - nzc;
+            else if (K == g.nz - 4) img = ic - nzc;
+            if (img >= 0) {
+                us[img] = us[ic];
+                vs[img] = vs[ic];
+                ws[img] = ws[ic];
+            }
+        }
+    }
+    __shared__ double red[4];
+    const double wm = wave_max(cmax);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = wm;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = red[0];
+        for (int q = 1; q < 4; ++q) m = red[q] > m ? red[q] : m;
+        cfl_part[L] = m * dt;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Boundary conditions, BC_staggered_main (Boundary_Condition.hpp:37-176) with the
+// Num/Dir table, applied per face group in the reference's order (x, y, z faces),
+// plus pressure ghosts.  With t1 != nullptr the same ghost values are written into
+// T1, which is BC_staggered_copy (updateBC.hpp:6-110).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void put2(double *a, double *b, int c0, int c1, double val)
+{
+    a[c0] = val;
+    a[c1] = val;
+    if (b) {
+        b[c0] = val;
+        b[c1] = val;
+    }
+}
+
+__global__ void k_bc_x(Geo g, BCParams bc, int advance, double *u, double *v, double *w,
+                       double *p, double *u1, double *v1, double *w1)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= g.cyz) return;
+    const int J = m / g.nzc + kGC, K = m % g.nzc + kGC;
+    const int nx = g.nx;
+    const int c0 = gidx(g, 0, J, K), c1 = gidx(g, 1, J, K), c2 = gidx(g, 2, J, K);
+    // face 0 (:72-80)
+    const double d0 = bc.inflow ? bc.inflow[J] : bc.dir[0][0];
+    put2(u, u1, c1, c0, d0 + bc.num[0][0] * u[c2]);
+    put2(v, v1, c1, c0, bc.dir[0][1] * 2. + (2.0 * bc.num[0][1] - 1.0) * v[c2]);
+    put2(w, w1, c1, c0, bc.dir[0][2] * 2. + (2.0 * bc.num[0][2] - 1.0) * w[c2]);
+    // face 1 (:84-94); convective outflow advances u on the outlet face
+    const int e4 = gidx(g, nx - 4, J, K), e3 = gidx(g, nx - 3, J, K),
+              e2 = gidx(g, nx - 2, J, K), e1 = gidx(g, nx - 1, J, K);
+    double ub;
+    if (bc.convective && advance) {
+        const double uo = u[e3];
+        ub = uo - bc.conv_coef * (uo - u[e4]);
+    } else {
+        ub = bc.dir[1][0] + bc.num[1][0] * u[e4];
+    }
+    u[e1] = ub;
+    u[e2] = ub;
+    u[e3] = ub;
+    if (u1) {
+        u1[e1] = ub;
+        u1[e2] = ub;
+        u1[e3] = ub;
+    }
+    put2(v, v1, e2, e1, bc.dir[1][1] * 2. + (2.0 * bc.num[1][1] - 1.0) * v[e3]);
+    put2(w, w1, e2, e1, bc.dir[1][2] * 2. + (2.0 * bc.num[1][2] - 1.0) * w[e3]);
+    // pressure (:149-155)
+    const double pw = p[c2];
+    p[c1] = pw;
+    p[c0] = pw;
+    const double pe = p[e3];
+    p[e2] = pe;
+    p[e1] = pe;
+}
+
+// outlet flux correction (no reference counterpart): canonical sums over the
+// interior (j,k) of u_in*Dy*Dz, u_out*Dy*Dz and Dy*Dz
+__global__ void __launch_bounds__(kThreads)
+k_flux_partial(Geo g, const double *u, double *part, int nb)
+{
+    const int L = blockIdx.x;
+    double v[3] = {0.0, 0.0, 0.0};
+    for (int c = 0; c < kCells; ++c) {
+        const int m = L * kPerBlock + c * kThreads + threadIdx.x;
+        if (m >= g.cyz) break;
+        const int J = m / g.nzc + kGC, K = m % g.nzc + kGC;
+        const double a = g.ax[1].D[J] * g.ax[2].D[K];
+        v[0] += u[gidx(g, 1, J, K)] * a;
+        v[1] += u[gidx(g, g.nx - 3, J, K)] * a;
+        v[2] += a;
+    }
+    block_sums<3>(v, part, nb, L);
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_flux_apply(Geo g, const double *part, int nb, double *u, double *u1)
+{
+    double t[3];
+    fold_partials<3>(part, nb, nb, t);
+    const double corr = (t[0] - t[1]) / t[2];
+    for (int m = threadIdx.x; m < g.cyz; m += kThreads) {
+        const int J = m / g.nzc + kGC, K = m % g.nzc + kGC;
+        const int e3 = gidx(g, g.nx - 3, J, K);
+        const double ub = u[e3] + corr;
+        u[e3] = ub;
+        u[e3 + g.sx] = ub;
+        u[e3 + 2 * g.sx] = ub;
+        if (u1) {
+            u1[e3] = ub;
+            u1[e3 + g.sx] = ub;
+            u1[e3 + 2 * g.sx] = ub;
+        }
+    }
+}
+
+__global__ void k_bc_y(Geo g, BCParams bc, double *u, double *v, double *w, double *p,
+                       double *u1, double *v1, double *w1)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= g.nxc * g.nzc) return;
+    const int I = m / g.nzc + kGC, K = m % g.nzc + kGC;
+    const int ny = g.ny;
+    const int c0 = gidx(g, I, 0, K), c1 = gidx(g, I, 1, K), c2 = gidx(g, I, 2, K);
+    // face 2 (:97-105)
+    put2(u, u1, c1, c0, bc.dir[2][0] * 2. + (2.0 * bc.num[2][0] - 1.0) * u[c2]);
+    put2(v, v1, c1, c0, bc.dir[2][1] + bc.num[2][1] * v[c2]);
+    put2(w, w1, c1, c0, bc.dir[2][2] * 2. + (2.0 * bc.num[2][2] - 1.0) * w[c2]);
+    // face 3 (:107-117)
+    const int e4 = gidx(g, I, ny - 4, K), e3 = gidx(g, I, ny - 3, K),
+              e2 = gidx(g, I, ny - 2, K), e1 = gidx(g, I, ny - 1, K);
+    const double vb = bc.dir[3][1] + bc.num[3][1] * v[e4];
+    v[e1] = vb;
+    v[e2] = vb;
+    v[e3] = vb;
+    if (v1) {
+        v1[e1] = vb;
+        v1[e2] = vb;
+        v1[e3] = vb;
+    }
+    put2(u, u1, e2, e1, bc.dir[3][0] * 2. + (2.0 * bc.num[3][0] - 1.0) * u[e3]);
+    put2(w, w1, e2, e1, bc.dir[3][2] * 2. + (2.0 * bc.num[3][2] - 1.0) * w[e3]);
+    // pressure (:158-164)
+    const double ps = p[c2];
+    p[c1] = ps;
+    p[c0] = ps;
+    const double pn = p[e3];
+    p[e2] = pn;
+    p[e1] = pn;
+}
+
+__global__ void k_bc_z(Geo g, BCParams bc, double *u, double *v, double *w, double *p,
+                       double *u1, double *v1, double *w1)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= g.nxc * g.nyc) return;
+    const int I = m / g.nyc + kGC, J = m % g.nyc + kGC;
+    const int nz = g.nz, nzc = g.nzc;
+    const int b = gidx(g, I, J, 0);
+    if (g.zper) {
+        // periodic: ghost k <- k + nzc (k < 2) or k - nzc (k >= nz-2), u, v, w and p
+        double *f[4] = {u, v, w, p};
+        double *f1[4] = {u1, v1, w1, nullptr};
+        for (int q = 0; q < 4; ++q) {
+            const double a1 = f[q][b + 1 + nzc], a0 = f[q][b + nzc];
+            const double a2 = f[q][b + nz - 2 - nzc], a3 = f[q][b + nz - 1 - nzc];
+            f[q][b + 1] = a1;
+            f[q][b + 0] = a0;
+            f[q][b + nz - 2] = a2;
+            f[q][b + nz - 1] = a3;
+            if (f1[q]) {
+                f1[q][b + 1] = a1;
+                f1[q][b + 0] = a0;
+                f1[q][b + nz - 2] = a2;
+                f1[q][b + nz - 1] = a3;
+                if (q == 2) f1[q][b + nz - 3] = f[q][b + nz - 3];
+            }
+        }
+        return;
+    }
+    // face 4 (:121-129)
+    put2(u, u1, b + 1, b, bc.dir[4][0] * 2. + (2.0 * bc.num[4][0] - 1.0) * u[b + 2]);
+    put2(v, v1, b + 1, b, bc.dir[4][1] * 2. + (2.0 * bc.num[4][1] - 1.0) * v[b + 2]);
+    put2(w, w1, b + 1, b, bc.dir[4][2] + bc.num[4][2] * w[b + 2]);
+    // face 5 (:133-143)
+    const double wb = bc.dir[5][2] + bc.num[5][2] * w[b + nz - 4];
+    w[b + nz - 1] = wb;
+    w[b + nz - 2] = wb;
+    w[b + nz - 3] = wb;
+    if (w1) {
+        w1[b + nz - 1] = wb;
+        w1[b + nz - 2] = wb;
+        w1[b + nz - 3] = wb;
+    }
+    put2(u, u1, b + nz - 2, b + nz - 1,
+         bc.dir[5][0] * 2. + (2.0 * bc.num[5][0] - 1.0) * u[b + nz - 3]);
+    put2(v, v1, b + nz - 2, b + nz - 1,
+         bc.dir[5][1] * 2. + (2.0 * bc.num[5][1] - 1.0) * v[b + nz - 3]);
+    // pressure (:166-172)
+    const double pb = p[b + 2];
+    p[b + 1] = pb;
+    p[b + 0] = pb;
+    const double pt = p[b + nz - 3];
+    p[b + nz - 2] = pt;
+    p[b + nz - 1] = pt;
+}
+
+// BC_staggered_copy (updateBC.hpp:6-110) on its own: ghost entries T0 -> T1
+__global__ void k_bc_copy(Geo g, const double *u0, const double *v0, const double *w0,
+                          double *u1, double *v1, double *w1)
+{
+    int m = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    if (m < g.cyz) {
+        const int J = m / g.nzc + kGC, K = m % g.nzc + kGC;
+        const int cs[5] = {0, 1, nx - 3, nx - 2, nx - 1};
+        for (int q = 0; q < 5; ++q) {
+            const int c = gidx(g, cs[q], J, K);
+            u1[c] = u0[c];
+            if (q != 2) {
+                v1[c] = v0[c];
+                w1[c] = w0[c];
+            }
+        }
+        return;
+    }
+    m -= g.cyz;
+    if (m < g.nxc * g.nzc) {
+        const int I = m / g.nzc + kGC, K = m % g.nzc + kGC;
+        const int cs[5] = {0, 1, ny - 3, ny - 2, ny - 1};
+        for (int q = 0; q < 5; ++q) {
+            const int c = gidx(g, I, cs[q], K);
+            v1[c] = v0[c];
+            if (q != 2) {
+                u1[c] = u0[c];
+                w1[c] = w0[c];
+            }
+        }
+        return;
+    }
+    m -= g.nxc * g.nzc;
+    if (m < g.nxc * g.nyc) {
+        const int I = m / g.nyc + kGC, J = m % g.nyc + kGC;
+        const int cs[5] = {0, 1, nz - 3, nz - 2, nz - 1};
+        for (int q = 0; q < 5; ++q) {
+            const int c = gidx(g, I, J, cs[q]);
+            w1[c] = w0[c];
+            if (q != 2) {
+                u1[c] = u0[c];
+                v1[c] = v0[c];
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Pressure operator rows.  SPE (SPE_sparseMatrix.hpp:42-106): slots -x, -y, -z,
+// diag, +z, +y, +x with -y holding -yp(j) and +y holding -ym(j) as in the
+// reference, boundary faces folded into the diagonal.  JACOBI: the row-scaled CSR
+// matrix of createPressureMatrixCSR (structureGrid.hpp:557-654), entries
+// (-coef)*jp with jp = 1/diag.  Zero boundary slots are skipped, which equals
+// adding 0*x for finite x.
+// ---------------------------------------------------------------------------
+struct RowCoef {
+    double v[7];
+    bool has[7];
+    int off[7];
+};
+
+template <int PRE>
+__device__ __forceinline__ void row_coef(const Geo &g, int i, int j, int k, RowCoef &rc)
+{
+    const double xm = g.pa[0].m[i], xp = g.pa[0].p[i];
+    const double ym = g.pa[1].m[j], yp = g.pa[1].p[j];
+    const double zm = g.pa[2].m[k], zp = g.pa[2].p[k];
+    const bool kf = (k == 0), kl = (k == g.nzc - 1);
+    rc.has[0] = i != 0;
+    rc.has[1] = j != 0;
+    rc.has[2] = !kf || g.zper;
+    rc.has[3] = true;
+    rc.has[4] = !kl || g.zper;
+    rc.has[5] = j != g.nyc - 1;
+    rc.has[6] = i != g.nxc - 1;
+    rc.off[0] = -g.cyz;
+    rc.off[1] = -g.nzc;
+    rc.off[2] = kf ? g.nzc - 1 : -1;
+    rc.off[3] = 0;
+    rc.off[4] = kl ? -(g.nzc - 1) : 1;
+    rc.off[5] = g.nzc;
+    rc.off[6] = g.cyz;
+    if (PRE == 0) {
+        double ijk = xm + ym + zm + xp + yp + zp;
+        if (i == 0) ijk -= xm;
+        if (i == g.nxc - 1) ijk -= xp;
+        if (j == 0) ijk -= ym;
+        if (j == g.nyc - 1) ijk -= yp;
+        if (!g.zper) {
+            if (kf) ijk -= zm;
+            if (kl) ijk -= zp;
+        }
+        rc.v[0] = -xm;
+        rc.v[1] = -yp;
+        rc.v[2] = -zm;
+        rc.v[3] = ijk;
+        rc.v[4] = -zp;
+        rc.v[5] = -ym;
+        rc.v[6] = -xp;
+    } else {
+        double aa = 0.0, bb = 0.0, cc = 0.0;
+        if (i == 0) aa = -xm;
+        if (i == g.nxc - 1) aa = -xp;
+        if (j == 0) bb = -ym;
+        if (j == g.nyc - 1) bb = -yp;
+        if (!g.zper) {
+            if (kf) cc = -zm;
+            if (kl) cc = -zp;
+        }
+        const double ijk = xm + ym + zm + xp + yp + zp + aa + bb + cc;
+        const double jp = 1.0 / ijk;
+        rc.v[0] = -xm * jp;
+        rc.v[1] = -ym * jp;
+        rc.v[2] = -zm * jp;
+        rc.v[3] = ijk * jp;
+        rc.v[4] = -zp * jp;
+        rc.v[5] = -yp * jp;
+        rc.v[6] = -xp * jp;
+    }
+}
+
+template <class F>
+__device__ __forceinline__ double row_apply(const RowCoef &rc, int n, F &&val)
+{
+    double s = 0.0;
+#pragma unroll
+    for (int q = 0; q < 7; ++q)
+        if (rc.has[q]) s += rc.v[q] * val(n + rc.off[q]);
+    return s;
+}
+
+template <int PRE>
+__global__ void __launch_bounds__(kThreads)
+k_apply(Geo g, const double *__restrict__ x, double *__restrict__ r)
+{
+    const int L = logical_block(g);
+    if (L >= g.nblocks) return;
+    for (int c = 0; c < kCells; ++c) {
+        const int n = L * kPerBlock + c * kThreads + threadIdx.x;
+        if (n >= g.N) break;
+        int i, j, k;
+        decode(g, n, i, j, k);
+        RowCoef rc;
+        row_coef<PRE>(g, i, j, k, rc);
+        r[n] = row_apply(rc, n, [&](int m) { return x[m]; });
+    }
+}
+
+// createBMatrix_omp (b_matrix.hpp:47-93): b = -(1/dt) div u*, times jp on the CSR path
+template <int PRE>
+__global__ void __launch_bounds__(kThreads)
+k_rhs(Geo g, const double *__restrict__ us, const double *__restrict__ vs,
+      const double *__restrict__ ws, double ddt, double *__restrict__ b)
+{
+    const int L = logical_block(g);
+    if (L >= g.nblocks) return;
+    const Axis &X = g.ax[0], &Y = g.ax[1], &Z = g.ax[2];
+    for (int c = 0; c < kCells; ++c) {
+        const int n = L * kPerBlock + c * kThreads + threadIdx.x;
+        if (n >= g.N) break;
+        int i, j, k;
+        decode(g, n, i, j, k);
+        const int I = i + kGC, J = j + kGC, K = k + kGC;
+        const int ic = gidx(g, I, J, K);
+        const double div = divr(us[ic] - us[ic - g.sx], X.D[I], X.rD[I]) +
+                           divr(vs[ic] - vs[ic - g.sy], Y.D[J], Y.rD[J]) +
+                           divr(ws[ic] - ws[ic - 1], Z.D[K], Z.rD[K]);
+        if (PRE == 1) {
+            // jp = 1/diag of createPressureMatrixCSR (structureGrid.hpp:590-603)
+            const double xm = g.pa[0].m[i], xp = g.pa[0].p[i];
+            const double ym = g.pa[1].m[j], yp = g.pa[1].p[j];
+            const double zm = g.pa[2].m[k], zp = g.pa[2].p[k];
+            double aa = 0.0, bb = 0.0, cc = 0.0;
+            if (i == 0) aa = -xm;
+            if (i == g.nxc - 1) aa = -xp;
+            if (j == 0) bb = -ym;
+            if (j == g.nyc - 1) bb = -yp;
+            if (!g.zper) {
+                if (k == 0) cc = -zm;
+                if (k == g.nzc - 1) cc = -zp;
+            }
+            const double jp = 1.0 / (xm + ym + zm + xp + yp + zp + aa + bb + cc);
+            b[n] = -jp * ddt * div;
+        } else {
+            b[n] = -ddt * div;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// BiCGSTAB, bicgstabRe2::solve (bicgstabRe2.hpp:82-180), as three fused phases per
+// iteration with the scalars kept on the device:
+//   A: p = r + beta*(p - omega*Ap) (or p = r, and r0 = r on a restart),
+//      Ap = A p, partial (Ap, r0)                                   -> alpha
+//   B: s = r - alpha*Ap, As = A s, partials (As, s), (As, As)       -> omega
+//   C: x += alpha*p + omega*s, r = s - omega*As, partials (r,r), (r,r0)
+//      -> convergence test, beta, restart decision
+// p and s are recomputed at the stencil neighbours from r, p, Ap (no extra pass).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kThreads)
+k_bicg_init(Geo g, int pre, const double *__restrict__ b, const double *__restrict__ x,
+            double *__restrict__ r0, double *__restrict__ r, double *__restrict__ px1,
+            double *__restrict__ part, KState *st)
+{
+    const int L = logical_block(g);
+    if (L >= g.nblocks) return;
+    double acc[2] = {0.0, 0.0};
+    for (int c = 0; c < kCells; ++c) {
+        const int n = L * kPerBlock + c * kThreads + threadIdx.x;
+        if (n >= g.N) break;
+        int i, j, k;
+        decode(g, n, i, j, k);
+        RowCoef rc;
+        if (pre) row_coef<1>(g, i, j, k, rc);
+        else row_coef<0>(g, i, j, k, rc);
+        const double ax = row_apply(rc, n, [&](int m) { return x[m]; });
+        const double bn = b[n];
+        const double rr = bn - ax;
+        r0[n] = rr;
+        r[n] = rr;
+        px1[n] = x[n];
+        acc[0] += bn * bn;
+        acc[1] += rr * rr;
+    }
+    block_sums<2>(acc, part, g.nblocks, L);
+}
+
+template <int PRE>
+__global__ void __launch_bounds__(kThreads)
+k_bicg_A(Geo g, const double *__restrict__ r, const double *__restrict__ p_in,
+         const double *__restrict__ ap_in, double *__restrict__ r0,
+         double *__restrict__ p_out, double *__restrict__ ap_out, double *__restrict__ part,
+         const KState *st)
+{
+    if (st->done) return;
+    const int L = logical_block(g);
+    if (L >= g.nblocks) return;
+    const int mode = st->mode;
+    const double beta = st->beta, omega = st->omega;
+    double acc[1] = {0.0};
+    for (int c = 0; c < kCells; ++c) {
+        const int n = L * kPerBlock + c * kThreads + threadIdx.x;
+        if (n >= g.N) break;
+        int i, j, k;
+        decode(g, n, i, j, k);
+        RowCoef rc;
+        row_coef<PRE>(g, i, j, k, rc);
+        auto pnew = [&](int m) {
+            return mode == 0 ? r[m] + beta * (p_in[m] - omega * ap_in[m]) : r[m];
+        };
+        const double apn = row_apply(rc, n, pnew);
+        const double pn = pnew(n);
+        p_out[n] = pn;
+        ap_out[n] = apn;
+        double r0n;
+        if (mode == 1) {
+            r0n = r[n];
+            r0[n] = r0n;
+        } else {
+            r0n = r0[n];
+        }
+        acc[0] += apn * r0n;
+    }
+    block_sums<1>(acc, part, g.nblocks, L);
+}
+
+template <int PRE>
+__global__ void __launch_bounds__(kThreads)
+k_bicg_B(Geo g, const double *__restrict__ r, const double *__restrict__ ap,
+         double *__restrict__ as, double *__restrict__ part, const KState *st)
+{
+    if (st->done) return;
+    const int L = logical_block(g);
+    if (L >= g.nblocks) return;
+    const double alpha = st->alpha;
+    double acc[2] = {0.0, 0.0};
+    for (int c = 0; c < kCells; ++c) {
+        const int n = L * kPerBlock + c * kThreads + threadIdx.x;
+        if (n >= g.N) break;
+        int i, j, k;
+        decode(g, n, i, j, k);
+        RowCoef rc;
+        row_coef<PRE>(g, i, j, k, rc);
+        auto s = [&](int m) { return r[m] - alpha * ap[m]; };
+        const double asn = row_apply(rc, n, s);
+        const double sn = s(n);
+        as[n] = asn;
+        acc[0] += asn * sn;
+        acc[1] += asn * asn;
+    }
+    block_sums<2>(acc, part, g.nblocks, L);
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_bicg_C(Geo g, double *__restrict__ r, const double *__restrict__ ap,
+         const double *__restrict__ p, const double *__restrict__ as,
+         const double *__restrict__ r0, double *__restrict__ x, double *__restrict__ part,
+         const KState *st)
+{
+    if (st->done) return;
+    const int L = logical_block(g);
+    if (L >= g.nblocks) return;
+    const double alpha = st->alpha, omega = st->omega;
+    double acc[2] = {0.0, 0.0};
+    for (int c = 0; c < kCells; ++c) {
+        const int n = L * kPerBlock + c * kThreads + threadIdx.x;
+        if (n >= g.N) break;
+        const double sn = r[n] - alpha * ap[n];
+        x[n] += alpha * p[n] + omega * sn;
+        const double rn = sn - omega * as[n];
+        r[n] = rn;
+        acc[0] += rn * rn;
+        acc[1] += rn * r0[n];
+    }
+    block_sums<2>(acc, part, g.nblocks, L);
+}
+
+// scalar updates between the phases (one block)
+__global__ void __launch_bounds__(kThreads)
+k_bicg_fin(const double *part, int nb, KState *st, int phase)
+{
+    if (phase != 0 && st->done) return;
+    if (phase == 0) {
+        double t[2];
+        fold_partials<2>(part, nb, nb, t);
+        if (threadIdx.x == 0) {
+            st->timestep += 1;
+            st->norm0 = sqrt(t[0]);
+            st->rho = t[1];
+            st->iter = 0;
+            st->done = 0;
+            st->mode = 2;
+        }
+    } else if (phase == 1) {
+        double t[1];
+        fold_partials<1>(part, nb, nb, t);
+        if (threadIdx.x == 0) st->alpha = st->rho / t[0];
+    } else if (phase == 2) {
+        double t[2];
+        fold_partials<2>(part, nb, nb, t);
+        if (threadIdx.x == 0) st->omega = t[0] / t[1];
+    } else {
+        double t[2];
+        fold_partials<2>(part, nb, nb, t);
+        if (threadIdx.x == 0) {
+            const double res = sqrt(t[0]) / st->norm0;
+            st->res = res;
+            if (res < st->tol) {
+                st->done = 1;
+            } else if (st->iter >= st->maxit) {
+                st->done = 2;
+            } else {
+                const double rho_l = st->rho;
+                st->rho = t[1];
+                st->beta = (st->alpha / st->omega) * (st->rho / rho_l);
+                if (st->timestep < 10 || fabs(rho_l) > st->tol * st->restart_factor) {
+                    st->mode = 0;
+                } else {
+                    st->mode = 1;
+                    st->restarts += 1;
+                }
+                st->iter += 1;
+            }
+        }
+    }
+}
+
+// NaN/Inf reset after the iteration limit (bicgstabRe2.hpp:133-153)
+__global__ void k_bicg_nanfix(int n, double *x, const double *px1, int *count)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n) return;
+    const double v = x[m];
+    if (isnan(v) || isinf(v)) {
+        x[m] = px1[m];
+        atomicAdd(count, 1);
+    }
+}
+
+__global__ void k_copy(int n, const double *a, double *b)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m < n) b[m] = a[m];
+}
+
+// ---------------------------------------------------------------------------
+// Red-black SOR with the coefficients and update of SorPipeLine_seq
+// (SorPipeLine.seq.hpp:40-123); ghosts of p hold the last boundary values.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kThreads)
+k_sor_rhs(Geo g, const double *us, const double *vs, const double *ws, double rdt, double *cf6)
+{
+    const int L = logical_block(g);
+    if (L >= g.nblocks) return;
+    const Axis &X = g.ax[0], &Y = g.ax[1], &Z = g.ax[2];
+    for (int c = 0; c < kCells; ++c) {
+        const int n = L * kPerBlock + c * kThreads + threadIdx.x;
+        if (n >= g.N) break;
+        int i, j, k;
+        decode(g, n, i, j, k);
+        const int I = i + kGC, J = j + kGC, K = k + kGC;
+        const int ic = gidx(g, I, J, K);
+        const double mChange = (us[ic] - us[ic - g.sx]) * Y.D[J] * Z.D[K] +
+                               (vs[ic] - vs[ic - g.sy]) * X.D[I] * Z.D[K] +
+                               (ws[ic] - ws[ic - 1]) * X.D[I] * Y.D[J];
+        cf6[n] = mChange * rdt;
+    }
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_sor_half(Geo g, int colour, double omega, const double *__restrict__ cf6,
+           double *__restrict__ p, double *__restrict__ part, const SorState *st)
+{
+    if (st->done) return;
+    const int L = logical_block(g);
+    if (L >= g.nblocks) return;
+    const Axis &X = g.ax[0], &Y = g.ax[1], &Z = g.ax[2];
+    double mx = 0.0;
+    for (int c = 0; c < kCells; ++c) {
+        const int n = L * kPerBlock + c * kThreads + threadIdx.x;
+        if (n >= g.N) break;
+        int i, j, k;
+        decode(g, n, i, j, k);
+        const int I = i + kGC, J = j + kGC, K = k + kGC;
+        if (((I + J + K) & 1) != colour) continue;
+        const int ic = gidx(g, I, J, K);
+        const double cf0 = divr(Y.D[J] * Z.D[K], X.Ds[I], X.rDs[I]);
+        const double cf1 = divr(Y.D[J] * Z.D[K], X.Ds[I - 1], X.rDs[I - 1]);
+        const double cf2 = divr(X.D[I] * Z.D[K], Y.Ds[J], Y.rDs[J]);
+        const double cf3 = divr(X.D[I] * Z.D[K], Y.Ds[J - 1], Y.rDs[J - 1]);
+        const double cf4 = divr(X.D[I] * Y.D[J], Z.Ds[K], Z.rDs[K]);
+        const double cf5 = divr(X.D[I] * Y.D[J], Z.Ds[K - 1], Z.rDs[K - 1]);
+        const double cf7 = -1.0 / (cf0 + cf1 + cf2 + cf3 + cf4 + cf5);
+        const int kp = (g.zper && K == g.nz - 3) ? ic + 1 - g.nzc : ic + 1;
+        const int km = (g.zper && K == kGC) ? ic - 1 + g.nzc : ic - 1;
+        const double pc = p[ic];
+        const double pNEW = (-p[ic + g.sx] * cf0 - p[ic - g.sx] * cf1 - p[ic + g.sy] * cf2 -
+                             p[ic - g.sy] * cf3 - p[kp] * cf4 - p[km] * cf5 + cf6[n]) *
+                            cf7;
+        const double pChange = fabs(pNEW - pc);
+        p[ic] = pc + omega * (pNEW - pc);
+        mx = pChange > mx ? pChange : mx;
+    }
+    __shared__ double red[4];
+    const double wm = wave_max(mx);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = wm;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = red[0];
+        for (int q = 1; q < 4; ++q) m = red[q] > m ? red[q] : m;
+        part[colour * g.nblocks + L] = m;
+    }
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_sor_fin(const double *part, int nb, SorState *st)
+{
+    if (st->done) return;
+    __shared__ double red[4];
+    double mx = 0.0;
+    for (int m = threadIdx.x; m < 2 * nb; m += kThreads) mx = part[m] > mx ? part[m] : mx;
+    const double wm = wave_max(mx);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = wm;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = red[0];
+        for (int q = 1; q < 4; ++q) m = red[q] > m ? red[q] : m;
+        st->residual = m;
+        st->iters += 1;
+        if (!(m > st->tol && st->iters < st->itmax)) st->done = 1;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// update_UandF_omp (updateUandF.hpp:90-180) fused with Pressure_transform_X_result
+// (Transform.hpp:8-25) and the drag/lift partial sums of virtualF_Int
+// (virtualForceIntergrator.hpp:79-86).  FROM_X: pressure read from X_result.
+// ---------------------------------------------------------------------------
+template <int FROM_X>
+__global__ void __launch_bounds__(kThreads)
+k_project(Geo g, double dt, double rdt, int convective, const double *__restrict__ xres,
+          double *__restrict__ p, const double *__restrict__ eta,
+          const double *__restrict__ us, const double *__restrict__ vs,
+          const double *__restrict__ ws, double *__restrict__ u, double *__restrict__ v,
+          double *__restrict__ w, double *__restrict__ f, double *__restrict__ part)
+{
+    const int L = logical_block(g);
+    if (L >= g.nblocks) return;
+    const Axis &X = g.ax[0], &Y = g.ax[1], &Z = g.ax[2];
+    const double uS = 0.0, vS = 0.0, wS = 0.0;
+    double acc[2] = {0.0, 0.0};
+    for (int c = 0; c < kCells; ++c) {
+        const int n = L * kPerBlock + c * kThreads + threadIdx.x;
+        if (n >= g.N) break;
+        int i, j, k;
+        decode(g, n, i, j, k);
+        const int I = i + kGC, J = j + kGC, K = k + kGC;
+        const int ic = gidx(g, I, J, K);
+        double pc;
+        if (FROM_X) {
+            pc = xres[n];
+            p[ic] = pc;
+        } else {
+            pc = p[ic];
+        }
+        const double ec = eta[ic];
+        double fx = 0.0, fy = 0.0;
+        if (I < g.nx - 3) {
+            const double pe = FROM_X ? xres[n + g.cyz] : p[ic + g.sx];
+            const double T2u = us[ic] - divr(dt * (pe - pc), X.Ds[I], X.rDs[I]);
+            const double e = ec + (eta[ic + g.sx] - ec) * X.IT[I];
+            const double un = ec * uS + (1.0 - e) * T2u;
+            u[ic] = un;
+            fx = divr(un - T2u, dt, rdt);
+            f[ic] = fx;
+        } else if (convective) {
+            u[ic] = us[ic];
+        }
+        if (J < g.ny - 3) {
+            const double pn = FROM_X ? xres[n + g.nzc] : p[ic + g.sy];
+            const double T2v = vs[ic] - divr(dt * (pn - pc), Y.Ds[J], Y.rDs[J]);
+            const double e = ec + (eta[ic + g.sy] - ec) * Y.IT[J];
+            const double vn = ec * vS + (1.0 - e) * T2v;
+            v[ic] = vn;
+            fy = divr(vn - T2v, dt, rdt);
+            f[ic + g.G] = fy;
+        }
+        if (K < g.nz - 3 || g.zper) {
+            const bool wrap = (K == g.nz - 3);
+            const int icP = wrap ? ic - (g.nzc - 1) : ic + 1;
+            const double pt = FROM_X ? xres[wrap ? n - (g.nzc - 1) : n + 1] : p[icP];
+            const double T2w = ws[ic] - divr(dt * (pt - pc), Z.Ds[K], Z.rDs[K]);
+            const double e = ec + (eta[icP] - ec) * Z.IT[K];
+            const double wn = ec * wS + (1.0 - e) * T2w;
+            w[ic] = wn;
+            f[ic + 2 * g.G] = divr(wn - T2w, dt, rdt);
+        }
+        const double vol = X.D[I] * Y.D[J] * Z.D[K];
+        acc[0] += fx * vol;
+        acc[1] += fy * vol;
+    }
+    block_sums<2>(acc, part, g.nblocks, L);
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_cdcl_fin(const double *part, int nb, double lz, double *out)
+{
+    double t[2];
+    fold_partials<2>(part, nb, nb, t);
+    if (threadIdx.x == 0) {
+        out[0] = -2.0 * t[0] / lz;
+        out[1] = -2.0 * t[1] / lz;
+    }
+}
+
+// drag/lift from a stored force field (CD_CL on its own)
+__global__ void __launch_bounds__(kThreads)
+k_force_partial(Geo g, const double *f, double *part)
+{
+    const int L = logical_block(g);
+    if (L >= g.nblocks) return;
+    double acc[2] = {0.0, 0.0};
+    for (int c = 0; c < kCells; ++c) {
+        const int n = L * kPerBlock + c * kThreads + threadIdx.x;
+        if (n >= g.N) break;
+        int i, j, k;
+        decode(g, n, i, j, k);
+        const int I = i + kGC, J = j + kGC, K = k + kGC;
+        const int ic = gidx(g, I, J, K);
+        const double vol = g.ax[0].D[I] * g.ax[1].D[J] * g.ax[2].D[K];
+        acc[0] += f[ic] * vol;
+        acc[1] += f[ic + g.G] * vol;
+    }
+    block_sums<2>(acc, part, g.nblocks, L);
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_max_fin(const double *part, int nb, double *out)
+{
+    __shared__ double red[4];
+    double mx = -1.0e300;
+    for (int m = threadIdx.x; m < nb; m += kThreads) mx = part[m] > mx ? part[m] : mx;
+    const double wm = wave_max(mx);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = wm;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = red[0];
+        for (int q = 1; q < 4; ++q) m = red[q] > m ? red[q] : m;
+        *out = m;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// DFIB_CylinderZ (setEta.hpp:136-204): area fraction of each interior column from
+// nsub x nsub sub-cell midpoints, then the k = 0 plane copied to k = 1..nz-2.
+// ---------------------------------------------------------------------------
+__global__ void k_eta_cylz(Geo g, const double *Xn, const double *Yn, double cx, double cy,
+                           double R, int nsub, double inv_nsub2, double *eta)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= g.nxc * g.nyc) return;
+    const int I = m / g.nyc + kGC, J = m % g.nyc + kGC;
+    const double Dx = g.ax[0].D[I], Dy = g.ax[1].D[J];
+    const double Diagonal = sqrt(Dx * Dx + Dy * Dy) * 0.5;
+    const double ax = ((Xn[I] + Xn[I + 1]) * 0.5) - cx;
+    const double ay = ((Yn[J] + Yn[J + 1]) * 0.5) - cy;
+    const double Distance = sqrt(ax * ax + ay * ay);
+    double e;
+    if (fabs(Distance - R) < Diagonal) {
+        const double dyg = Dy / (double)nsub;
+        const double dxg = Dx / (double)nsub;
+        long xi = 0;
+        for (int ii = 0; ii < nsub; ++ii) {
+            const double sx0 = Xn[I] + (double)ii * dxg, sx1 = Xn[I] + (double)(ii + 1) * dxg;
+            const double dx = (sx0 + sx1) * 0.5 - cx;
+            for (int jj = 0; jj < nsub; ++jj) {
+                const double sy0 = Yn[J] + (double)jj * dyg,
+                             sy1 = Yn[J] + (double)(jj + 1) * dyg;
+                const double dy = (sy0 + sy1) * 0.5 - cy;
+                if (sqrt(dx * dx + dy * dy) <= R) ++xi;
+            }
+        }
+        e = (double)xi * inv_nsub2;
+    } else if (Distance > R) {
+        e = 0.;
+    } else {
+        e = 1.;
+    }
+    eta[(I * g.ny + J) * g.nz] = e;
+}
+
+__global__ void k_eta_copyk(Geo g, double *eta)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= (g.nx - 1) * (g.ny - 1)) return;
+    const int I = m / (g.ny - 1), J = m % (g.ny - 1);
+    const int b = (I * g.ny + J) * g.nz;
+    const double e = eta[b];
+    for (int k = 1; k < g.nz - 1; ++k) eta[b + k] = e;
+}
+
+// seeded initial field (no reference counterpart)
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x)
+{
+    uint64_t z = x + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__global__ void k_init_flow(Geo g, double u0, double amp, unsigned long long seed, double *u,
+                            double *v, double *w, double *p)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= g.G) return;
+    const int K = c % g.nz, t = c / g.nz, J = t % g.ny, I = t / g.ny;
+    double uu = u0;
+    if (I >= kGC && I < g.nx - kGC && J >= kGC && J < g.ny - kGC && K >= kGC && K < g.nz - kGC) {
+        const uint64_t n = (uint64_t)(((I - kGC) * g.nyc + (J - kGC)) * g.nzc + (K - kGC));
+        const uint64_t h = splitmix64((uint64_t)seed ^ n);
+        const double U = (double)(h >> 11) * 0x1.0p-53;
+        uu = u0 + amp * (2.0 * U - 1.0);
+    }
+    u[c] = uu;
+    v[c] = 0.0;
+    w[c] = 0.0;
+    p[c] = 0.0;
+}
+
+__global__ void k_fill(int n, double val, double *a)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m < n) a[m] = val;
+}
+
+// Pressure_transform_X_result (Transform.hpp:8-25)
+__global__ void __launch_bounds__(kThreads)
+k_transform(Geo g, const double *__restrict__ x, double *__restrict__ p)
+{
+    const int L = logical_block(g);
+    if (L >= g.nblocks) return;
+    for (int c = 0; c < kCells; ++c) {
+        const int n = L * kPerBlock + c * kThreads + threadIdx.x;
+        if (n >= g.N) break;
+        int i, j, k;
+        decode(g, n, i, j, k);
+        p[gidx(g, i + kGC, j + kGC, k + kGC)] = x[n];
+    }
+}
+
+__global__ void k_set_int(int *a, int val) { *a = val; }
+
+// ---------------------------------------------------------------------------
+// host-side launchers
+// ---------------------------------------------------------------------------
+static inline int grid_blocks(const Geo &g)
+{
+    return g.xcd_remap ? ((g.nblocks + 7) / 8) * 8 : g.nblocks;
+}
+static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
+
+void launch_convdiff(const Geo &g, hipStream_t s, const double *u, const double *v,
+                     const double *w, double *us, double *vs, double *ws, double dt,
+                     double nu, double *cfl_part)
+{
+    hipLaunchKernelGGL(k_convdiff, dim3(grid_blocks(g)), dim3(kThreads), 0, s, g, u, v, w, us,
+                       vs, ws, -dt, nu * dt, dt, cfl_part);
+}
+
+void launch_bc(const Geo &g, hipStream_t s, const BCParams &bc, int advance, double *u,
+               double *v, double *w, double *p, double *u1, double *v1, double *w1,
+               double *flux_part)
+{
+    hipLaunchKernelGGL(k_bc_x, dim3(cdiv(g.cyz, 256)), dim3(256), 0, s, g, bc, advance, u, v, w,
+                       p, u1, v1, w1);
+    if (bc.flux_correction) {
+        const int nb = cdiv(g.cyz, kPerBlock);
+        hipLaunchKernelGGL(k_flux_partial, dim3(nb), dim3(kThreads), 0, s, g, u, flux_part, nb);
+        hipLaunchKernelGGL(k_flux_apply, dim3(1), dim3(kThreads), 0, s, g, flux_part, nb, u,
+                           u1);
+    }
+    hipLaunchKernelGGL(k_bc_y, dim3(cdiv(g.nxc * g.nzc, 256)), dim3(256), 0, s, g, bc, u, v, w,
+                       p, u1, v1, w1);
+    hipLaunchKernelGGL(k_bc_z, dim3(cdiv(g.nxc * g.nyc, 256)), dim3(256), 0, s, g, bc, u, v, w,
+                       p, u1, v1, w1);
+}
+
+void launch_bc_copy(const Geo &g, hipStream_t s, const double *u0, const double *v0,
+                    const double *w0, double *u1, double *v1, double *w1)
+{
+    const int tot = g.cyz + g.nxc * g.nzc + g.nxc * g.nyc;
+    hipLaunchKernelGGL(k_bc_copy, dim3(cdiv(tot, 256)), dim3(256), 0, s, g, u0, v0, w0, u1, v1,
+                       w1);
+}
+
+void launch_rhs(const Geo &g, hipStream_t s, int pre, const double *us, const double *vs,
+                const double *ws, double dt, double *b)
+{
+    const double ddt = 1.0 / dt;
+    if (pre)
+        hipLaunchKernelGGL(k_rhs<1>, dim3(grid_blocks(g)), dim3(kThreads), 0, s, g, us, vs, ws,
+                           ddt, b);
+    else
+        hipLaunchKernelGGL(k_rhs<0>, dim3(grid_blocks(g)), dim3(kThreads), 0, s, g, us, vs, ws,
+                           ddt, b);
+}
+
+void launch_apply(const Geo &g, hipStream_t s, int pre, const double *x, double *r)
+{
+    if (pre)
+        hipLaunchKernelGGL(k_apply<1>, dim3(grid_blocks(g)), dim3(kThreads), 0, s, g, x, r);
+    else
+        hipLaunchKernelGGL(k_apply<0>, dim3(grid_blocks(g)), dim3(kThreads), 0, s, g, x, r);
+}
+
+void launch_bicg_init(const Geo &g, hipStream_t s, int pre, const double *b, const double *x,
+                      double *r0, double *r, double *px1, double *part, KState *st)
+{
+    hipLaunchKernelGGL(k_bicg_init, dim3(grid_blocks(g)), dim3(kThreads), 0, s, g, pre, b, x,
+                       r0, r, px1, part, st);
+    hipLaunchKernelGGL(k_bicg_fin, dim3(1), dim3(kThreads), 0, s, part, g.nblocks, st, 0);
+}
+
+void launch_bicg_iter(const Geo &g, hipStream_t s, int pre, BicgBuffers &B, double *x,
+                      double *part, KState *st)
+{
+    const dim3 gb(grid_blocks(g)), tb(kThreads);
+    if (pre)
+        hipLaunchKernelGGL(k_bicg_A<1>, gb, tb, 0, s, g, B.r, B.p[B.cur], B.ap[B.cur], B.r0,
+                           B.p[1 - B.cur], B.ap[1 - B.cur], part, st);
+    else
+        hipLaunchKernelGGL(k_bicg_A<0>, gb, tb, 0, s, g, B.r, B.p[B.cur], B.ap[B.cur], B.r0,
+                           B.p[1 - B.cur], B.ap[1 - B.cur], part, st);
+    B.cur = 1 - B.cur;
+    hipLaunchKernelGGL(k_bicg_fin, dim3(1), tb, 0, s, part, g.nblocks, st, 1);
+    if (pre)
+        hipLaunchKernelGGL(k_bicg_B<1>, gb, tb, 0, s, g, B.r, B.ap[B.cur], B.as, part, st);
+    else
+        hipLaunchKernelGGL(k_bicg_B<0>, gb, tb, 0, s, g, B.r, B.ap[B.cur], B.as, part, st);
+    hipLaunchKernelGGL(k_bicg_fin, dim3(1), tb, 0, s, part, g.nblocks, st, 2);
+    hipLaunchKernelGGL(k_bicg_C, gb, tb, 0, s, g, B.r, B.ap[B.cur], B.p[B.cur], B.as, B.r0, x,
+                       part, st);
+    hipLaunchKernelGGL(k_bicg_fin, dim3(1), tb, 0, s, part, g.nblocks, st, 3);
+}
+
+void launch_nanfix(hipStream_t s, int n, double *x, const double *px1, int *count)
+{
+    hipLaunchKernelGGL(k_bicg_nanfix, dim3(cdiv(n, 256)), dim3(256), 0, s, n, x, px1, count);
+}
+
+void launch_copy(hipStream_t s, int n, const double *a, double *b)
+{
+    hipLaunchKernelGGL(k_copy, dim3(cdiv(n, 256)), dim3(256), 0, s, n, a, b);
+}
+
+void launch_fill(hipStream_t s, int n, double val, double *a)
+{
+    hipLaunchKernelGGL(k_fill, dim3(cdiv(n, 256)), dim3(256), 0, s, n, val, a);
+}
+
+void launch_sor_rhs(const Geo &g, hipStream_t s, const double *us, const double *vs,
+                    const double *ws, double dt, double *cf6)
+{
+    hipLaunchKernelGGL(k_sor_rhs, dim3(grid_blocks(g)), dim3(kThreads), 0, s, g, us, vs, ws,
+                       1.0 / dt, cf6);
+}
+
+void launch_sor_sweep(const Geo &g, hipStream_t s, double omega, const double *cf6, double *p,
+                      double *part, SorState *st)
+{
+    hipLaunchKernelGGL(k_sor_half, dim3(grid_blocks(g)), dim3(kThreads), 0, s, g, 0, omega, cf6,
+                       p, part, st);
+    hipLaunchKernelGGL(k_sor_half, dim3(grid_blocks(g)), dim3(kThreads), 0, s, g, 1, omega, cf6,
+                       p, part, st);
+    hipLaunchKernelGGL(k_sor_fin, dim3(1), dim3(kThreads), 0, s, part, g.nblocks, st);
+}
+
+void launch_project(const Geo &g, hipStream_t s, int from_x, double dt, int convective,
+                    const double *xres, double *p, const double *eta, const double *us,
+                    const double *vs, const double *ws, double *u, double *v, double *w,
+                    double *f, double *part)
+{
+    const double rdt = 1.0 / dt;
+    if (from_x)
+        hipLaunchKernelGGL(k_project<1>, dim3(grid_blocks(g)), dim3(kThreads), 0, s, g, dt, rdt,
+                           convective, xres, p, eta, us, vs, ws, u, v, w, f, part);
+    else
+        hipLaunchKernelGGL(k_project<0>, dim3(grid_blocks(g)), dim3(kThreads), 0, s, g, dt, rdt,
+                           convective, xres, p, eta, us, vs, ws, u, v, w, f, part);
+}
+
+void launch_cdcl_fin(const Geo &g, hipStream_t s, const double *part, double lz, double *out)
+{
+    hipLaunchKernelGGL(k_cdcl_fin, dim3(1), dim3(kThreads), 0, s, part, g.nblocks, lz, out);
+}
+
+void launch_force_partial(const Geo &g, hipStream_t s, const double *f, double *part)
+{
+    hipLaunchKernelGGL(k_force_partial, dim3(grid_blocks(g)), dim3(kThreads), 0, s, g, f, part);
+}
+
+void launch_max_fin(const Geo &g, hipStream_t s, const double *part, double *out)
+{
+    hipLaunchKernelGGL(k_max_fin, dim3(1), dim3(kThreads), 0, s, part, g.nblocks, out);
+}
+
+void launch_eta_cylz(const Geo &g, hipStream_t s, const double *Xn, const double *Yn, double cx,
+                     double cy, double R, int nsub, double inv_nsub2, double *eta)
+{
+    hipLaunchKernelGGL(k_eta_cylz, dim3(cdiv(g.nxc * g.nyc, 64)), dim3(64), 0, s, g, Xn, Yn, cx,
+                       cy, R, nsub, inv_nsub2, eta);
+    hipLaunchKernelGGL(k_eta_copyk, dim3(cdiv((g.nx - 1) * (g.ny - 1), 256)), dim3(256), 0, s, g,
+                       eta);
+}
+
+void launch_init_flow(const Geo &g, hipStream_t s, double u0, double amp,
+                      unsigned long long seed, double *u, double *v, double *w, double *p)
+{
+    hipLaunchKernelGGL(k_init_flow, dim3(cdiv(g.G, 256)), dim3(256), 0, s, g, u0, amp, seed, u,
+                       v, w, p);
+}
+
+void launch_transform(const Geo &g, hipStream_t s, const double *x, double *p)
+{
+    hipLaunchKernelGGL(k_transform, dim3(grid_blocks(g)), dim3(kThreads), 0, s, g, x, p);
+}
+
+void launch_set_int(hipStream_t s, int *a, int val)
+{
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, s, a, val);
+}
+
+} // namespace cfdmx

@@ -1,0 +1,172 @@
+/*
+ * cfdmx.h -- C-ABI of the MI355X (gfx950) time-step library for incompressible
+ * flow past a cylinder.
+ *
+ * The reference (jimmy3769tw/CFD_NSsolver_forHPC_flowPassingCylinder) has no
+ * library or FFI layer: its hot path is a set of header-defined C++ functions that
+ * runSeqOmp (src/RUN_CPU.cpp:9-440) calls on host std::vectors.  Each entry point
+ * below replaces one of those functions (cited per declaration) and operates on
+ * device-resident state owned by a context; the C++ adapter
+ * cfdmx_reference_api.hpp re-exposes them under the reference's own names and
+ * signatures.
+ *
+ * Conventions
+ *   - Plain C types only; every function returns an int status (CFDMX_OK or a
+ *     negative CFDMX_ERR_*) and never throws.  cfdmx_last_error() gives the text.
+ *   - Host arrays use the reference layouts: ghosted fields of G = nx*ny*nz doubles,
+ *     nx = nxc + 4 (two ghost layers, structureGrid.hpp:34,86-90), index
+ *     (i*ny + j)*nz + k (structureGrid.hpp:153-156); pressure unknowns of
+ *     N = nxc*nyc*nzc doubles indexed ((i-2)*nyc + (j-2))*nzc + (k-2)
+ *     (structureGrid.hpp:176-181).
+ *   - Calls on one context are ordered on that context's HIP stream; a context is
+ *     not re-entrant.  Functions returning host values synchronise the stream.
+ */
+#ifndef CFDMX_H
+#define CFDMX_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CFDMX_OK 0
+#define CFDMX_ERR_ARG (-1)    /* invalid argument or configuration */
+#define CFDMX_ERR_HIP (-2)    /* HIP runtime failure (device, memory, launch) */
+#define CFDMX_ERR_STATE (-3)  /* call not valid in the current context state */
+
+#define CFDMX_GRID_UNIFORM 0 /* grid::init_uniform, structureGrid.hpp:336-367 */
+#define CFDMX_GRID_SML 1     /* grid::init_nonuniform_SML, structureGrid.hpp:250-329 */
+
+#define CFDMX_SOLVER_BICGSTAB 0 /* solver::bicgstabRe2, bicgstabRe2.hpp:82-180 */
+#define CFDMX_SOLVER_RBSOR 1    /* red-black ordering of SorPipeLine_seq's update */
+
+#define CFDMX_PRECOND_NONE 0   /* SPE 7-slot operator, SPE_sparseMatrix.hpp:42-106 */
+#define CFDMX_PRECOND_JACOBI 1 /* row-scaled CSR operator, structureGrid.hpp:557-654 */
+
+#define CFDMX_PRECISION_FP64 0
+#define CFDMX_PRECISION_MIXED 1 /* FP32 inner Krylov, FP64 outer refinement */
+
+typedef struct cfdmx_config {
+    /* grid (class grid, structureGrid.hpp:20-40, made runtime) */
+    int nx, ny, nz; /* interior cells: grid::nxCal, nyCal, nzCal */
+    double lx, ly, lz;
+    int grid_kind;
+    /* physics and time (simpulationVariable, 0_1_SimulationVariable.hpp:26-31,95-106) */
+    double re, dt;
+    /* boundary conditions: the Num/Dir table of FLOWpassing.hpp:6-40, face x component */
+    double bc_num[6][3];
+    double bc_dir[6][3];
+    int z_periodic;          /* faces 4/5 periodic (no reference counterpart) */
+    int outflow_convective;  /* face 1 u convective (no reference counterpart) */
+    double convective_speed; /* advection speed of the convective outflow */
+    int flux_correction;     /* shift the outlet face so its flux equals the inlet's */
+    /* pressure (bicgstabRe2.hpp:47-52, 0_1_SimulationVariable.hpp:51-67) */
+    int solver;
+    int precond;
+    double tol;
+    int maxit;
+    double sor_omega;
+    int skip_first_pressure; /* RUN_CPU.cpp:303: no pressure solve on step 1 */
+    int precision;
+    double inner_tol; /* mixed precision: FP32 inner tolerance */
+    /* launch shaping */
+    int xcd_remap; /* map consecutive logical blocks to one XCD (0 = hardware order) */
+} cfdmx_config;
+
+typedef struct cfdmx_ctx cfdmx_ctx;
+
+/* per-step report (the stdout of RUN_CPU.cpp:346-435 and Time_cDcL.dat) */
+typedef struct cfdmx_step_info {
+    double cd, cl;    /* CD_CL, virtualForceIntergrator.hpp:92-111 */
+    double time;      /* simulation time at the call (simpulationVariable::time) */
+    int iters;        /* pressure iterations (simu.iters) */
+    double err;       /* pressure residual (simu.error) */
+    double cfl;       /* get_Cfl, cfl.hpp:3-79 */
+    int solved;       /* 1 if a pressure solve ran this step */
+    int restarts;     /* cumulative BiCGSTAB rho restarts */
+    int resets;       /* cumulative BiCGSTAB NaN resets */
+} cfdmx_step_info;
+
+/* Defaults: FLOWpassing table, reference solver defaults (tol 1e-3, 5000 iterations,
+ * omega 1.8), uniform 20 x 10 x 0.5 domain. */
+int cfdmx_config_default(cfdmx_config *cfg);
+
+/* Context lifecycle; replaces the allocations of runSeqOmp (RUN_CPU.cpp:18-28,64,120,
+ * 158-163) and generateGride (Generator.hpp:14-46).  Builds the grid metrics. */
+int cfdmx_create(const cfdmx_config *cfg, cfdmx_ctx **out);
+int cfdmx_destroy(cfdmx_ctx *ctx);
+const char *cfdmx_last_error(const cfdmx_ctx *ctx); /* ctx may be NULL */
+int cfdmx_sizes(const cfdmx_ctx *ctx, long long *G, long long *N);
+
+/* Grid metrics as built by the reference generators: X,Y,Z (n+1 each), then
+ * Dx,Dy,Dz,Dxs,Dys,Dzs (n each).  structureGrid.hpp:93,336-367,433-486. */
+int cfdmx_grid_metrics(const cfdmx_ctx *ctx, double *out);
+
+/* Field movement.  which: 0 = T0 (u^n), 1 = T1 (u*), 2 = T3 (u^{n+1}).
+ * Pressure p is ghosted (pressure::p), x is X_result (MxClass::X_result). */
+int cfdmx_upload_velocity(cfdmx_ctx *ctx, int which, const double *u, const double *v,
+                          const double *w);
+int cfdmx_download_velocity(cfdmx_ctx *ctx, int which, double *u, double *v, double *w);
+int cfdmx_upload_pressure(cfdmx_ctx *ctx, const double *p);
+int cfdmx_download_pressure(cfdmx_ctx *ctx, double *p);
+int cfdmx_upload_xresult(cfdmx_ctx *ctx, const double *x);
+int cfdmx_download_xresult(cfdmx_ctx *ctx, double *x);
+int cfdmx_download_rhs(cfdmx_ctx *ctx, double *b);
+int cfdmx_upload_eta(cfdmx_ctx *ctx, const double *eta);
+int cfdmx_download_eta(cfdmx_ctx *ctx, double *eta);
+int cfdmx_download_force(cfdmx_ctx *ctx, double *f); /* DfibArray::f, 3*G */
+int cfdmx_set_inflow(cfdmx_ctx *ctx, const double *profile); /* u inflow per ghosted j, ny */
+
+/* Seeded initial field on the device: u = u0 + amp*(2U-1) on interior cells with U
+ * from splitmix64(seed ^ interior index), v = w = 0, p = 0; then the initial
+ * BC_staggered_main + BC_staggered_copy (RUN_CPU.cpp:66-109). */
+int cfdmx_init_flow(cfdmx_ctx *ctx, double u0, double amp, unsigned long long seed);
+/* Initial T1 = T3 = T0 copy + boundary conditions (RUN_CPU.cpp:104-109) for fields
+ * uploaded by the caller. */
+int cfdmx_start(cfdmx_ctx *ctx);
+
+/* Immersed-boundary mask.  DFIB_CylinderZ setEta.hpp:136-204 (centre, radius and
+ * sub-cell count parameters; the reference fixes 6.5, ly/2, 0.5, 100). */
+int cfdmx_set_eta_cylinder_z(cfdmx_ctx *ctx, double cx, double cy, double radius, int nsub);
+/* Smoothed mask of half-width eps for differentiation (no reference counterpart). */
+int cfdmx_set_eta_smoothed(cfdmx_ctx *ctx, double cx, double cy, double radius, double eps);
+
+/* ---- stages of one time step (RUN_CPU.cpp:257-436) ---- */
+/* ConvectionDifussion, Convection_Difussion.hpp:174-450: T0 -> T1, QUICK, explicit Euler */
+int cfdmx_convdiff(cfdmx_ctx *ctx);
+/* BC_staggered_main, Boundary_Condition.hpp:37-176, on T0 and p */
+int cfdmx_bc_main(cfdmx_ctx *ctx);
+/* BC_staggered_copy, updateBC.hpp:6-110: ghosts T0 -> T1 */
+int cfdmx_bc_copy(cfdmx_ctx *ctx);
+/* createBMatrix_omp, b_matrix.hpp:35-98: b from T1 (kept on the device) */
+int cfdmx_pressure_rhs(cfdmx_ctx *ctx);
+/* bicgstabRe2::solve, bicgstabRe2.hpp:82-180, on the device b and X_result; returns
+ * iterations and relative residual like std::pair<int,double> */
+int cfdmx_solve_bicgstab(cfdmx_ctx *ctx, int *iters, double *residual);
+/* Red-black SOR with SorPipeLine_seq's coefficients and update
+ * (SorPipeLine.seq.hpp:31-123) on p, RHS from T1; stop on max |dp| < tol */
+int cfdmx_solve_rbsor(cfdmx_ctx *ctx, int *iters, double *err);
+/* Pressure_transform_X_result, Transform.hpp:8-25 */
+int cfdmx_transform(cfdmx_ctx *ctx);
+/* update_UandF_omp, updateUandF.hpp:90-180: T1, p, eta -> T3, f */
+int cfdmx_project_ib(cfdmx_ctx *ctx);
+/* copyVel(T3, T0), copyVel.hpp:4-12 (a device pointer swap) */
+int cfdmx_swap_t0_t3(cfdmx_ctx *ctx);
+/* CD_CL, virtualForceIntergrator.hpp:92-111 */
+int cfdmx_cd_cl(cfdmx_ctx *ctx, double *cd, double *cl);
+/* get_Cfl, cfl.hpp:3-79, on T1 */
+int cfdmx_cfl(cfdmx_ctx *ctx, double *cfl);
+
+/* Pressure operator on host vectors of N doubles: SPE_matrix::multiply_omp
+ * (SPE_sparseMatrix.hpp:92-106) or CSR_matrix::multiply_omp (CSR_sparseMatrix.hpp:231-245). */
+int cfdmx_apply_operator(cfdmx_ctx *ctx, const double *x, double *r);
+
+/* n whole time steps (the loop body of runSeqOmp, RUN_CPU.cpp:257-436, without file
+ * output).  info (may be NULL) receives the last step's report. */
+int cfdmx_step(cfdmx_ctx *ctx, int nsteps, cfdmx_step_info *info);
+
+int cfdmx_synchronize(cfdmx_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CFDMX_H */

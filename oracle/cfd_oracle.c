@@ -401,16 +401,22 @@ void orc_bc_main(const orc_grid *g, const orc_bc *bc, int advance_outflow, doubl
                 Dir[b][2] * 2. + (2.0 * Num[b][2] - 1.0) * w[IC(g, nx - 3, j, k)];
         }
     if (bc->flux_correction) {
-        /* outlet flux rescaled to the inlet flux: sum over the interior (j,k) of
-         * u*Dy*Dz in index order; no reference counterpart */
-        double qin = 0.0, qout = 0.0, area = 0.0;
+        /* outlet flux shifted to the inlet flux (no reference counterpart): sums of
+         * u*Dy*Dz over the interior (j,k), m = (j-2)*nzc + (k-2), in the device's
+         * canonical blocked order */
+        const long m2 = (long)g->nyc * g->nzc;
+        double *t = (double *)calloc((size_t)(3 * m2), sizeof(double));
         for (int j = GC; j < ny - GC; ++j)
             for (int k = GC; k < nz - GC; ++k) {
+                const long m = (long)(j - GC) * g->nzc + (k - GC);
                 const double a = g->Dy[j] * g->Dz[k];
-                qin += u[IC(g, 1, j, k)] * a;
-                qout += u[IC(g, nx - 3, j, k)] * a;
-                area += a;
+                t[m] = u[IC(g, 1, j, k)] * a;
+                t[m2 + m] = u[IC(g, nx - 3, j, k)] * a;
+                t[2 * m2 + m] = a;
             }
+        const double qin = orc_blocked_sum(t, m2), qout = orc_blocked_sum(t + m2, m2),
+                     area = orc_blocked_sum(t + 2 * m2, m2);
+        free(t);
         const double corr = (qin - qout) / area;
         for (int j = GC; j < ny - GC; ++j)
             for (int k = GC; k < nz - GC; ++k) {

@@ -1,0 +1,197 @@
+"""GPU parity: the HIP library against the CPU oracle on identical inputs.
+
+Bar: bit-for-bit equality (np.array_equal, so +0 == -0) for every field after each
+stage and after whole time steps, with the oracle run in the device's canonical
+reduction order.  The oracle itself is pinned to the reference in
+test_oracle_reference.py.
+"""
+import numpy as np
+import pytest
+
+import cfd_nssolver_forhpc_flowpassingcylinder_amd as pkg
+import oracle_py as op
+from gpu_helpers import fields_equal, make_pair
+
+pytestmark = pytest.mark.gpu
+
+
+def test_library_exports_and_metrics():
+    for kind, n, L in ((0, (24, 12, 4), (20.0, 10.0, 0.5)), (1, (112, 104, 4), (17.0, 12.0, 3.0))):
+        gpu = pkg.FlowSolver(nx=n[0], ny=n[1], nz=n[2], lx=L[0], ly=L[1], lz=L[2],
+                             grid_kind=kind)
+        g = op.Oracle().grid(*n, *L, kind=kind)
+        m = gpu.metrics()
+        for name, arr in m.items():
+            assert np.array_equal(arr, getattr(g, name)), name
+        gpu.close()
+
+
+def _random_velocity(g, seed):
+    rng = np.random.default_rng(seed)
+    shape = g.shape()
+    x = np.linspace(0, 1, shape[0])[:, None, None]
+    y = np.linspace(0, 1, shape[1])[None, :, None]
+    z = np.linspace(0, 1, shape[2])[None, None, :]
+    u = 1.0 + 0.3 * np.sin(6 * x + 2 * y) * np.cos(4 * z) + 0.05 * rng.standard_normal(shape)
+    v = 0.4 * np.cos(5 * x - 3 * y + z) + 0.05 * rng.standard_normal(shape)
+    w = 0.2 * np.sin(3 * x + 7 * z) * np.cos(y) + 0.05 * rng.standard_normal(shape)
+    return (np.ascontiguousarray(a.ravel()) for a in (u, v, w))
+
+
+@pytest.mark.parametrize("kind,n,L", [(0, (40, 24, 8), (20.0, 10.0, 0.5)),
+                                      (1, (112, 104, 6), (17.0, 12.0, 3.0))])
+@pytest.mark.parametrize("zper", [0, 1])
+def test_convdiff_bitwise(kind, n, L, zper):
+    cfg = pkg.default_config(nx=n[0], ny=n[1], nz=n[2], lx=L[0], ly=L[1], lz=L[2],
+                             grid_kind=kind, re=150.0, dt=0.003, z_periodic=zper)
+    gpu = pkg.FlowSolver(cfg)
+    o = op.Oracle()
+    g = o.grid(*n, *L, kind=kind)
+    u, v, w = _random_velocity(g, 3)
+    gpu.upload_velocity(0, u, v, w)
+    zeros = np.zeros(g.G)
+    gpu.upload_velocity(1, zeros, zeros, zeros)
+    gpu.convdiff()
+    us, vs, ws = np.zeros(g.G), np.zeros(g.G), np.zeros(g.G)
+    o.convdiff(g, 0.003, 1.0 / 150.0, u, v, w, us, vs, ws, z_periodic=zper)
+    a, b, c = gpu.download_velocity(1)
+    assert np.array_equal(a, us) and np.array_equal(b, vs) and np.array_equal(c, ws)
+    assert gpu.cfl() == o.cfl(g, 0.003, us, vs, ws, z_periodic=zper)
+    gpu.close()
+
+
+@pytest.mark.parametrize("bc", ["flowpassing", "slip", "cylinder"])
+def test_boundary_conditions_bitwise(bc):
+    gpu, sim, o, g = make_pair(32, 16, 6, bc=bc)
+    assert fields_equal(gpu, sim) == []
+    # BC_staggered_main on a perturbed field, then BC_staggered_copy
+    u, v, w = _random_velocity(g, 5)
+    p = np.random.default_rng(1).standard_normal(g.G)
+    gpu.upload_velocity(0, u, v, w)
+    gpu.set_pressure(p)
+    gpu.bc_main()
+    obc = sim.bc
+    uo, vo, wo, po = u.copy(), v.copy(), w.copy(), p.copy()
+    o.bc_main(g, obc, uo, vo, wo, po, advance=1, dt=0.002)
+    gu, gv, gw = gpu.download_velocity(0)
+    assert np.array_equal(gu, uo) and np.array_equal(gv, vo) and np.array_equal(gw, wo)
+    assert np.array_equal(gpu.pressure(), po)
+    z = np.zeros(g.G)
+    gpu.upload_velocity(1, z, z, z)
+    gpu.bc_copy()
+    u1, v1, w1 = z.copy(), z.copy(), z.copy()
+    o.bc_copy(g, uo, vo, wo, u1, v1, w1)
+    a, b, c = gpu.download_velocity(1)
+    assert np.array_equal(a, u1) and np.array_equal(b, v1) and np.array_equal(c, w1)
+    gpu.close()
+
+
+@pytest.mark.parametrize("precond", [0, 1])
+@pytest.mark.parametrize("zper", [0, 1])
+def test_operator_rhs_bicgstab_bitwise(precond, zper):
+    n = (24, 20, 6)
+    cfg = pkg.default_config(nx=n[0], ny=n[1], nz=n[2], precond=precond, z_periodic=zper,
+                             tol=1e-9, dt=0.002)
+    gpu = pkg.FlowSolver(cfg)
+    o = op.Oracle()
+    g = o.grid(*n)
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal(g.N)
+    assert np.array_equal(gpu.apply_operator(x), o.apply_A(g, x, precond, zper))
+    u, v, w = _random_velocity(g, 9)
+    gpu.upload_velocity(1, u, v, w)
+    gpu.pressure_rhs()
+    b = o.rhs(g, 0.002, u, v, w, precond, zper)
+    assert np.array_equal(gpu.rhs(), b)
+    gpu.close()
+
+
+@pytest.mark.parametrize("precond", [0, 1])
+def test_bicgstab_solve_bitwise(precond):
+    gpu, sim, o, g = make_pair(24, 16, 6, precond=precond, tol=1e-10)
+    # a pressure solve on a divergent u*: run conv-diff, then the RHS and BiCGSTAB
+    gpu.convdiff()
+    o.convdiff(g, 0.002, 1.0 / 100.0, sim.field("u0"), sim.field("v0"), sim.field("w0"),
+               sim.field("u1"), sim.field("v1"), sim.field("w1"))
+    gpu.pressure_rhs()
+    b = o.rhs(g, 0.002, sim.field("u1"), sim.field("v1"), sim.field("w1"), precond)
+    assert np.array_equal(gpu.rhs(), b)
+    bic = op.BiCG(o, g)
+    for rep in range(2):  # two solves: warm start and the history vectors
+        it_g, res_g = gpu.solve_bicgstab()
+        xo = sim.field("x")
+        it_o, res_o = bic.solve(b, xo, 1e-10, precond, 0, op.RED_BLOCKED)
+        assert it_g == it_o and res_g == res_o, (rep, it_g, it_o, res_g, res_o)
+        assert np.array_equal(gpu.xresult(), xo)
+    gpu.close()
+
+
+def test_projection_and_drag_bitwise():
+    gpu, sim, o, g = make_pair(32, 16, 4)
+    u, v, w = _random_velocity(g, 11)
+    p = np.random.default_rng(2).standard_normal(g.G)
+    gpu.upload_velocity(1, u, v, w)
+    gpu.set_pressure(p)
+    gpu.project_ib()
+    eta = sim.field("eta")
+    u3, v3, w3 = np.zeros(g.G), np.zeros(g.G), np.zeros(g.G)
+    u3[:], v3[:], w3[:] = gpu.download_velocity(2)  # untouched entries kept
+    f = np.zeros(3 * g.G)
+    o.update_uf(g, 0.002, p, eta, u, v, w, u3, v3, w3, f)
+    a, b, c = gpu.download_velocity(2)
+    assert np.array_equal(a, u3) and np.array_equal(b, v3) and np.array_equal(c, w3)
+    assert np.array_equal(gpu.force(), f)
+    cd, cl = gpu.cd_cl()
+    assert (cd, cl) == o.cd_cl(g, f, op.RED_BLOCKED)
+    gpu.close()
+
+
+@pytest.mark.parametrize("n,cyl", [((64, 32, 4), (5.0, 5.0, 0.5)), ((96, 48, 4), (6.5, 5.0, 0.5))])
+def test_eta_cylinder_bitwise(n, cyl):
+    gpu = pkg.FlowSolver(nx=n[0], ny=n[1], nz=n[2])
+    gpu.set_eta_cylinder_z(*cyl, 100)
+    o = op.Oracle()
+    g = o.grid(*n)
+    assert np.array_equal(gpu.eta(), o.eta_cylinder_z(g, *cyl, 100))
+    gpu.set_eta_smoothed(*cyl, 0.3)
+    assert np.array_equal(gpu.eta(), o.eta_smoothed(g, *cyl, 0.3))
+    gpu.close()
+
+
+CASES = [
+    dict(n=(48, 24, 4), bc="flowpassing", solver=0, precond=0, tol=1e-6, steps=4),
+    dict(n=(48, 24, 4), bc="flowpassing", solver=0, precond=1, tol=1e-8, steps=4),
+    dict(n=(40, 20, 6), bc="cylinder", solver=0, precond=1, tol=1e-8, steps=4),
+    dict(n=(40, 20, 6), bc="cylinder", solver=1, precond=0, tol=1e-7, steps=3),
+    dict(n=(40, 24, 4), bc="slip", solver=1, precond=0, tol=1e-6, steps=3, maxit=300),
+    dict(n=(112, 104, 4), bc="flowpassing", solver=0, precond=0, tol=1e-3, steps=3, kind=1,
+         lengths=(17.0, 12.0, 3.0), cyl=(6.5, 6.0, 0.5)),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: f"{c['bc']}-s{c['solver']}p{c['precond']}")
+def test_time_steps_bitwise(case):
+    c = dict(case)
+    n, steps = c.pop("n"), c.pop("steps")
+    gpu, sim, o, g = make_pair(*n, **c)
+    for s in range(steps):
+        a = gpu.step(1)
+        b = sim.step()
+        assert a["iters"] == b["iters"], (s, a, b)
+        assert a["err"] == b["err"] and a["cd"] == b["cd"] and a["cl"] == b["cl"], (s, a, b)
+        assert a["cfl"] == b["cfl"] and a["time"] == b["time"], (s, a, b)
+        assert fields_equal(gpu, sim) == [], s
+    gpu.close()
+
+
+def test_xcd_remap_does_not_change_results():
+    a, sa, _, _ = make_pair(40, 20, 6, bc="cylinder", precond=1, tol=1e-8, xcd_remap=1)
+    b, sb, _, _ = make_pair(40, 20, 6, bc="cylinder", precond=1, tol=1e-8, xcd_remap=0)
+    for _ in range(3):
+        ra, rb = a.step(1), b.step(1)
+        assert ra == rb
+    for which in range(3):
+        for x, y in zip(a.download_velocity(which), b.download_velocity(which)):
+            assert np.array_equal(x, y)
+    a.close()
+    b.close()
