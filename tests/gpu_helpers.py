@@ -58,14 +58,11 @@ def fields_equal(gpu, sim):
                          (2, ("u3", "v3", "w3"))):
         got = gpu.download_velocity(which)
         for a, n in zip(got, names):
-            if not np.array_equal(a, sim.field(n)):
+            if not np.array_equal(a, sim.field(n), equal_nan=True):
                 bad.append(n)
-    if not np.array_equal(gpu.pressure(), sim.field("p")):
-        bad.append("p")
-    if not np.array_equal(gpu.force(), sim.field("f")):
-        bad.append("f")
-    if not np.array_equal(gpu.xresult(), sim.field("x")):
-        bad.append("x")
+    for name, got in (("p", gpu.pressure()), ("f", gpu.force()), ("x", gpu.xresult())):
+        if not np.array_equal(got, sim.field(name), equal_nan=True):
+            bad.append(name)
     return bad
 
 

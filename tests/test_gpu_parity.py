@@ -106,23 +106,31 @@ def test_operator_rhs_bicgstab_bitwise(precond, zper):
     gpu.close()
 
 
+def _same(a, b):
+    return a == b or (np.isnan(a) and np.isnan(b))
+
+
 @pytest.mark.parametrize("precond", [0, 1])
-def test_bicgstab_solve_bitwise(precond):
-    gpu, sim, o, g = make_pair(24, 16, 6, precond=precond, tol=1e-10)
-    # a pressure solve on a divergent u*: run conv-diff, then the RHS and BiCGSTAB
+@pytest.mark.parametrize("bc,tol,maxit", [("cylinder", 1e-10, 5000), ("flowpassing", 1e-10, 40)])
+def test_bicgstab_solve_bitwise(precond, bc, tol, maxit):
+    # "cylinder": consistent singular system (flux-corrected outlet, periodic z);
+    # "flowpassing" at tol 1e-10 is inconsistent and ends at the iteration limit,
+    # which exercises the NaN/Inf reset path
+    gpu, sim, o, g = make_pair(24, 16, 6, precond=precond, tol=tol, maxit=maxit, bc=bc)
+    zp = 1 if bc == "cylinder" else 0
     gpu.convdiff()
     o.convdiff(g, 0.002, 1.0 / 100.0, sim.field("u0"), sim.field("v0"), sim.field("w0"),
-               sim.field("u1"), sim.field("v1"), sim.field("w1"))
+               sim.field("u1"), sim.field("v1"), sim.field("w1"), z_periodic=zp)
     gpu.pressure_rhs()
-    b = o.rhs(g, 0.002, sim.field("u1"), sim.field("v1"), sim.field("w1"), precond)
+    b = o.rhs(g, 0.002, sim.field("u1"), sim.field("v1"), sim.field("w1"), precond, zp)
     assert np.array_equal(gpu.rhs(), b)
     bic = op.BiCG(o, g)
     for rep in range(2):  # two solves: warm start and the history vectors
         it_g, res_g = gpu.solve_bicgstab()
         xo = sim.field("x")
-        it_o, res_o = bic.solve(b, xo, 1e-10, precond, 0, op.RED_BLOCKED)
-        assert it_g == it_o and res_g == res_o, (rep, it_g, it_o, res_g, res_o)
-        assert np.array_equal(gpu.xresult(), xo)
+        it_o, res_o = bic.solve(b, xo, tol, precond, zp, op.RED_BLOCKED, maxit=maxit)
+        assert it_g == it_o and _same(res_g, res_o), (rep, it_g, it_o, res_g, res_o)
+        assert np.array_equal(gpu.xresult(), xo, equal_nan=True)
     gpu.close()
 
 
@@ -159,8 +167,10 @@ def test_eta_cylinder_bitwise(n, cyl):
 
 
 CASES = [
-    dict(n=(48, 24, 4), bc="flowpassing", solver=0, precond=0, tol=1e-6, steps=4),
-    dict(n=(48, 24, 4), bc="flowpassing", solver=0, precond=1, tol=1e-8, steps=4),
+    dict(n=(48, 24, 4), bc="flowpassing", solver=0, precond=0, tol=1e-3, steps=4),
+    dict(n=(48, 24, 4), bc="flowpassing", solver=0, precond=1, tol=1e-3, steps=4),
+    # inconsistent Neumann system at a tight tolerance: iteration limit + NaN reset
+    dict(n=(24, 12, 4), bc="flowpassing", solver=0, precond=1, tol=1e-9, maxit=60, steps=3),
     dict(n=(40, 20, 6), bc="cylinder", solver=0, precond=1, tol=1e-8, steps=4),
     dict(n=(40, 20, 6), bc="cylinder", solver=1, precond=0, tol=1e-7, steps=3),
     dict(n=(40, 24, 4), bc="slip", solver=1, precond=0, tol=1e-6, steps=3, maxit=300),
@@ -169,7 +179,7 @@ CASES = [
 ]
 
 
-@pytest.mark.parametrize("case", CASES, ids=lambda c: f"{c['bc']}-s{c['solver']}p{c['precond']}")
+@pytest.mark.parametrize("case", CASES, ids=lambda c: f"{c['bc']}-s{c['solver']}p{c['precond']}-{c['n'][0]}")
 def test_time_steps_bitwise(case):
     c = dict(case)
     n, steps = c.pop("n"), c.pop("steps")
@@ -177,9 +187,8 @@ def test_time_steps_bitwise(case):
     for s in range(steps):
         a = gpu.step(1)
         b = sim.step()
-        assert a["iters"] == b["iters"], (s, a, b)
-        assert a["err"] == b["err"] and a["cd"] == b["cd"] and a["cl"] == b["cl"], (s, a, b)
-        assert a["cfl"] == b["cfl"] and a["time"] == b["time"], (s, a, b)
+        for key in ("iters", "err", "cd", "cl", "cfl", "time"):
+            assert _same(a[key], b[key]), (s, key, a, b)
         assert fields_equal(gpu, sim) == [], s
     gpu.close()
 
