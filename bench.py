@@ -1,0 +1,229 @@
+#!/usr/bin/env python3
+"""Throughput of the flow-past-a-cylinder time step on one MI355X.
+
+Workload (the metric's configuration, one GPU): uniform 512 x 256 x 64 cells over
+20 x 10 x 0.5 (D = 1), cylinder at (5, 5) with R = 0.5 (sharp area-fraction mask),
+Re = 200, dt = 0.001, inflow U = 1, convective outflow with flux correction, slip
+top/bottom walls, periodic z, FP64, pressure by BiCGSTAB on the SPE operator at
+tol 1e-10; initial field u = 1 +- 1e-3 seeded noise.  W warm-up steps (from the
+initial field; step 1 has no pressure solve, as in the reference driver), then K
+timed steps.  value = Mcell-steps/s = N*K / wall / 1e6 with all fields resident on
+the device.
+
+The JSON line also carries:
+  roofline      -- the dominant kernel's algorithmic bytes per launch over its mean
+                   launch duration, from HIP events on the library's stream during the
+                   timed steps, against the 8 TB/s HBM peak;
+  cpu_baseline  -- the reference's own CPU code (oracle/_ref, built from the reference
+                   headers) timed on this host on a bounded sample of the same grid.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+# algorithmic HBM bytes per interior cell and launch (FP64; each array counted once,
+# ghosts and 1-D metric tables excluded)
+BYTES_PER_CELL = {
+    "bicg_A": 48,    # r r, p, Ap, r0;  w p', Ap'
+    "bicg_B": 24,    # r r, Ap;         w As
+    "bicg_C": 64,    # r r, Ap, p, As, r0, x;  w x, r
+    "bicg_init": 48, # r b, x;  w r0, r, px1
+    "convdiff": 48,  # r u, v, w;  w u*, v*, w*
+    "rhs": 32,       # r u*, v*, w*;  w b
+    "project": 96,   # r u*, v*, w*, X, eta;  w u, v, w, p, f(3)
+    "sor_half": 24,  # r p (all), rhs (half);  w p (half)
+}
+
+WORKLOADS = {
+    "cylinder512": dict(nx=512, ny=256, nz=64, re=200.0, dt=0.001, tol=1e-10),
+    "cylinder256": dict(nx=256, ny=128, nz=32, re=100.0, dt=0.002, tol=1e-10),
+    "cylinder128": dict(nx=128, ny=64, nz=4, re=100.0, dt=0.002, tol=1e-8),
+}
+
+
+def make_solver(wl, precond, solver, seed):
+    import cfd_nssolver_forhpc_flowpassingcylinder_amd as pkg
+    from cfd_nssolver_forhpc_flowpassingcylinder_amd.api import (PRECOND_JACOBI, PRECOND_NONE,
+                                                                  SOLVER_BICGSTAB, SOLVER_RBSOR)
+    slip_num = [[0, 0, 0], [1, 1, 1], [1, 0, 1], [1, 0, 1], [1, 1, 1], [1, 1, 1]]
+    slip_dir = [[1, 0, 0], [0, 0, 0], [0, 0, 0], [0, 0, 0], [0, 0, 0], [0, 0, 0]]
+    cfg = pkg.default_config(
+        nx=wl["nx"], ny=wl["ny"], nz=wl["nz"], lx=20.0, ly=10.0, lz=0.5, re=wl["re"],
+        dt=wl["dt"], tol=wl["tol"], maxit=5000,
+        precond=PRECOND_JACOBI if precond == "jacobi" else PRECOND_NONE,
+        solver=SOLVER_RBSOR if solver == "rbsor" else SOLVER_BICGSTAB, sor_omega=1.8,
+        bc_num=slip_num, bc_dir=slip_dir, z_periodic=1, outflow_convective=1,
+        convective_speed=1.0, flux_correction=1, skip_first_pressure=1)
+    s = pkg.FlowSolver(cfg)
+    s.init_flow(1.0, 1e-3, seed)
+    s.set_eta_cylinder_z(5.0, 5.0, 0.5, 100)
+    return s
+
+
+def cpu_baseline(wl, iters_per_step, budget_s):
+    """The reference's CPU path (oracle/_ref) on this host: one pass of every explicit
+    stage at the full grid, plus a BiCGSTAB solve capped at a few iterations; the
+    per-step cost is t_explicit + iters_per_step * t_iteration."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import numpy as np
+    import oracle_py as op
+
+    if not op.ref_available():
+        return None
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = max(1, min(cores, 16))
+    nx, ny, nz = wl["nx"], wl["ny"], wl["nz"]
+    r = op.RefHarness(nx, ny, nz, 20.0, 10.0, 0.5, sml=0, re=wl["re"], dt=wl["dt"],
+                      precond=0, tol=1e-300, threads=cores)
+    u0 = r.field("u0")
+    rng = np.random.default_rng(1)
+    u0[:] = 1.0 + 1e-3 * (2.0 * rng.random(u0.size) - 1.0)
+    r.call("eta")
+    r.call("start")
+    t = time.perf_counter()
+    r.call("convdiff")
+    r.call("rhs")
+    t_rhs = time.perf_counter()
+    r.call("transform")
+    r.call("update")
+    r.call("copyvel")
+    r.call("bc_main")
+    r.call("bc_copy")
+    r.cdcl()
+    t_explicit = time.perf_counter() - t
+    # capped solves: k+1 iterations each (the solver breaks after the iteration limit)
+    timings = {}
+    for k in (1, 4):
+        r.set_maxit(k)
+        t = time.perf_counter()
+        r.solve()
+        timings[k] = time.perf_counter() - t
+        if time.perf_counter() - t_rhs > budget_s:
+            break
+    if 4 in timings:
+        t_iter = (timings[4] - timings[1]) / 3.0
+    else:
+        t_iter = timings[1] / 2.0
+    step_s = t_explicit + iters_per_step * t_iter
+    n = nx * ny * nz
+    return {
+        "value": n / step_s / 1e6,
+        "unit": "Mcell-steps/s",
+        "cores": cores,
+        "kind": "reference",
+        "sample": (f"reference CPU path (oracle/_ref, OpenMP {cores} threads) at "
+                   f"{nx}x{ny}x{nz}: one pass of conv-diff, RHS, transform, projection, "
+                   f"copy, BCs and Cd/Cl ({t_explicit:.2f} s) plus BiCGSTAB solves capped at "
+                   f"2 and 5 iterations ({t_iter:.3f} s per iteration); per-step cost = "
+                   f"explicit + {iters_per_step:.1f} iterations (the GPU's mean per timed step)"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="cylinder512", choices=sorted(WORKLOADS))
+    ap.add_argument("--precond", default="none", choices=["none", "jacobi"])
+    ap.add_argument("--solver", default="bicgstab", choices=["bicgstab", "rbsor"])
+    ap.add_argument("--seed", type=int, default=20261202)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=25.0)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not run "
+              "(multi-GPU x-slab sharding is design-only)", file=sys.stderr)
+        sys.exit(2)
+
+    wl = WORKLOADS[args.workload]
+    s = make_solver(wl, args.precond, args.solver, args.seed)
+    n_cells = wl["nx"] * wl["ny"] * wl["nz"]
+    warm = []
+    for q in range(args.warmup):
+        warm.append(s.step(1))
+        print(f"warmup step {q + 1}: {warm[-1]['iters']} iterations", file=sys.stderr,
+              flush=True)
+    s.synchronize()
+    s.profile(True)
+    infos = []
+    t0 = time.perf_counter()
+    for q in range(args.steps):
+        infos.append(s.step(1))
+        print(f"timed step {q + 1}: {infos[-1]['iters']} iterations", file=sys.stderr,
+              flush=True)
+    s.synchronize()
+    t1 = time.perf_counter()
+    prof = s.profile_read()
+    s.profile(False)
+    wall = t1 - t0
+    value = n_cells * args.steps / wall / 1e6
+    iters = [i["iters"] for i in infos]
+    mean_iters = sum(iters) / max(1, len(iters))
+
+    # dominant kernel by device time during the timed steps
+    cands = {k: v for k, v in prof.items() if k in BYTES_PER_CELL and v[1] > 0}
+    dom = max(cands, key=lambda k: cands[k][0])
+    ms, launches = cands[dom]
+    avg_s = ms / 1e3 / launches
+    alg_bytes = BYTES_PER_CELL[dom] * n_cells
+    achieved = alg_bytes / avg_s / 1e9
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": None, "avg_launch_us": round(avg_s * 1e6, 2),
+                "bytes_per_launch": alg_bytes}
+    kernels = {k: {"ms_total": round(v[0], 3), "launches": v[1],
+                   "avg_us": round(v[0] * 1e3 / v[1], 2) if v[1] else None,
+                   "GBps": (round(BYTES_PER_CELL[k] * n_cells / (v[0] / 1e3 / v[1]) / 1e9, 1)
+                            if v[1] and k in BYTES_PER_CELL else None)}
+               for k, v in prof.items() if v[1] > 0}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(wl, mean_iters, args.cpu_budget)
+
+    line = {
+        "metric": "Mcell-steps/s, FP64 vs mixed @≤1e-6 rel err; % HBM peak; adjoint step cost",
+        "value": round(value, 4),
+        "unit": "Mcell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"flow past a cylinder, {wl['nx']}x{wl['ny']}x{wl['nz']} uniform "
+                         f"cells over 20x10x0.5, Re={wl['re']:g}, dt={wl['dt']:g}, inflow U=1, "
+                         "convective outflow, slip y walls, periodic z, "
+                         f"{args.solver} ({args.precond}) tol {wl['tol']:g}, FP64"),
+            "grid": [wl["nx"], wl["ny"], wl["nz"]],
+            "solver": args.solver, "precond": args.precond, "tol": wl["tol"],
+            "iters_per_step": iters, "mean_iters": mean_iters,
+            "warmup_iters": [i["iters"] for i in warm],
+            "cd_last": infos[-1]["cd"], "cl_last": infos[-1]["cl"],
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernels": kernels,
+    }
+    print(json.dumps(line), flush=True)
+    s.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -30,8 +30,13 @@ EXPORTS = [
     "cfdmx_set_eta_cylinder_z", "cfdmx_set_eta_smoothed", "cfdmx_convdiff", "cfdmx_bc_main",
     "cfdmx_bc_copy", "cfdmx_pressure_rhs", "cfdmx_solve_bicgstab", "cfdmx_solve_rbsor",
     "cfdmx_transform", "cfdmx_project_ib", "cfdmx_swap_t0_t3", "cfdmx_cd_cl", "cfdmx_cfl",
-    "cfdmx_apply_operator", "cfdmx_step", "cfdmx_synchronize",
+    "cfdmx_apply_operator", "cfdmx_step", "cfdmx_synchronize", "cfdmx_profile",
+    "cfdmx_profile_read",
 ]
+
+# kernel classes of cfdmx_profile_read
+KERNELS = {"convdiff": 0, "bc": 1, "rhs": 2, "bicg_init": 3, "bicg_A": 4, "bicg_B": 5,
+           "bicg_C": 6, "bicg_fin": 7, "sor_half": 8, "project": 9}
 
 
 class Config(C.Structure):
@@ -98,6 +103,8 @@ def load_library(path=LIB_PATH):
     L.cfdmx_cfl.argtypes = [vp, DP]
     L.cfdmx_apply_operator.argtypes = [vp, DP, DP]
     L.cfdmx_step.argtypes = [vp, C.c_int, C.POINTER(StepInfo)]
+    L.cfdmx_profile.argtypes = [vp, C.c_int]
+    L.cfdmx_profile_read.argtypes = [vp, C.c_int, DP, C.POINTER(C.c_longlong)]
     for n in EXPORTS:
         if n != "cfdmx_last_error":
             getattr(L, n).restype = C.c_int
@@ -289,3 +296,16 @@ class FlowSolver:
 
     def synchronize(self):
         self._chk(self.L.cfdmx_synchronize(self.h), "synchronize")
+
+    def profile(self, enable):
+        self._chk(self.L.cfdmx_profile(self.h, 1 if enable else 0), "profile")
+
+    def profile_read(self):
+        """{kernel: (total_ms, launches)} for every kernel class"""
+        out = {}
+        for name, kid in KERNELS.items():
+            ms, n = C.c_double(), C.c_longlong()
+            self._chk(self.L.cfdmx_profile_read(self.h, kid, C.byref(ms), C.byref(n)),
+                      "profile_read")
+            out[name] = (ms.value, n.value)
+        return out

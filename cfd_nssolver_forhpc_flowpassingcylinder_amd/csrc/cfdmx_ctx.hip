@@ -42,6 +42,16 @@ struct cfdmx_ctx {
     double time = 0.0;
     int resets = 0;
     int started = 0;
+    // per-kernel HIP-event timing on the context's stream (cfdmx_profile)
+    struct ProfRec {
+        int id;
+        hipEvent_t a, b;
+    };
+    int prof_on = 0;
+    double prof_ms[CFDMX_K_COUNT] = {};
+    long long prof_n[CFDMX_K_COUNT] = {};
+    std::vector<hipEvent_t> prof_pool;
+    std::vector<ProfRec> prof_pending;
 };
 
 static thread_local std::string g_create_error;
@@ -63,6 +73,51 @@ static thread_local std::string g_create_error;
     } while (0)
 
 #define LAUNCHCHK() HIPCHK(hipGetLastError())
+
+static hipEvent_t prof_event(cfdmx_ctx *ctx)
+{
+    if (!ctx->prof_pool.empty()) {
+        hipEvent_t e = ctx->prof_pool.back();
+        ctx->prof_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// HIP events around one launch when profiling is on
+#define PROF(id, stmt)                                                                         \
+    do {                                                                                       \
+        hipEvent_t a_ = nullptr, b_ = nullptr;                                                 \
+        if (ctx->prof_on && (a_ = prof_event(ctx)) != nullptr)                                 \
+            (void)hipEventRecord(a_, ctx->stream);                                             \
+        stmt;                                                                                  \
+        if (a_) {                                                                              \
+            b_ = prof_event(ctx);                                                              \
+            if (b_) {                                                                          \
+                (void)hipEventRecord(b_, ctx->stream);                                         \
+                ctx->prof_pending.push_back({(id), a_, b_});                                   \
+            } else {                                                                           \
+                ctx->prof_pool.push_back(a_);                                                  \
+            }                                                                                  \
+        }                                                                                      \
+    } while (0)
+
+// accumulate finished event pairs (call after the stream has been synchronised)
+static void prof_harvest(cfdmx_ctx *ctx)
+{
+    for (auto &r : ctx->prof_pending) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+            ctx->prof_ms[r.id] += ms;
+            ctx->prof_n[r.id] += 1;
+        }
+        ctx->prof_pool.push_back(r.a);
+        ctx->prof_pool.push_back(r.b);
+    }
+    ctx->prof_pending.clear();
+}
 
 // ---------------------------------------------------------------------------
 // grid generation: grid::init_uniform (structureGrid.hpp:336-367),
@@ -401,6 +456,8 @@ int cfdmx_destroy(cfdmx_ctx *ctx)
     if (!ctx) return CFDMX_OK;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (void *p : ctx->allocs) (void)hipFree(p);
+    prof_harvest(ctx);
+    for (hipEvent_t e : ctx->prof_pool) (void)hipEventDestroy(e);
     if (ctx->kst_h) (void)hipHostFree(ctx->kst_h);
     if (ctx->sst_h) (void)hipHostFree(ctx->sst_h);
     if (ctx->scal_h) (void)hipHostFree(ctx->scal_h);
@@ -623,8 +680,9 @@ int cfdmx_set_eta_smoothed(cfdmx_ctx *ctx, double cx, double cy, double R, doubl
 int cfdmx_convdiff(cfdmx_ctx *ctx)
 {
     if (!ctx) return CFDMX_ERR_ARG;
-    launch_convdiff(ctx->g, ctx->stream, ctx->t0[0], ctx->t0[1], ctx->t0[2], ctx->t1[0],
-                    ctx->t1[1], ctx->t1[2], ctx->cfg.dt, 1.0 / ctx->cfg.re, ctx->cfl_part);
+    PROF(CFDMX_K_CONVDIFF,
+         launch_convdiff(ctx->g, ctx->stream, ctx->t0[0], ctx->t0[1], ctx->t0[2], ctx->t1[0],
+                         ctx->t1[1], ctx->t1[2], ctx->cfg.dt, 1.0 / ctx->cfg.re, ctx->cfl_part));
     launch_max_fin(ctx->g, ctx->stream, ctx->cfl_part, ctx->scal + 0);
     LAUNCHCHK();
     return CFDMX_OK;
@@ -651,8 +709,8 @@ int cfdmx_bc_copy(cfdmx_ctx *ctx)
 int cfdmx_pressure_rhs(cfdmx_ctx *ctx)
 {
     if (!ctx) return CFDMX_ERR_ARG;
-    launch_rhs(ctx->g, ctx->stream, ctx->cfg.precond, ctx->t1[0], ctx->t1[1], ctx->t1[2],
-               ctx->cfg.dt, ctx->b);
+    PROF(CFDMX_K_RHS, launch_rhs(ctx->g, ctx->stream, ctx->cfg.precond, ctx->t1[0], ctx->t1[1],
+                                 ctx->t1[2], ctx->cfg.dt, ctx->b));
     LAUNCHCHK();
     return CFDMX_OK;
 }
@@ -663,16 +721,24 @@ static int solve_bicgstab_impl(cfdmx_ctx *ctx, int *iters, double *res)
     const Geo &g = ctx->g;
     const int pre = ctx->cfg.precond;
     std::swap(ctx->px1, ctx->px2); // px2 <- px1, then px1 <- x (bicgstabRe2.hpp:94-95)
-    launch_bicg_init(g, s, pre, ctx->b, ctx->x, ctx->B.r0, ctx->B.r, ctx->px1, ctx->part,
-                     ctx->kst);
+    PROF(CFDMX_K_BICG_INIT, launch_bicg_init(g, s, pre, ctx->b, ctx->x, ctx->B.r0, ctx->B.r,
+                                             ctx->px1, ctx->part, ctx->kst));
+    PROF(CFDMX_K_BICG_FIN, launch_bicg_fin(g, s, ctx->part, ctx->kst, 0));
     LAUNCHCHK();
     int batch = 4;
     for (;;) {
-        for (int q = 0; q < batch; ++q)
-            launch_bicg_iter(g, s, pre, ctx->B, ctx->x, ctx->part, ctx->kst);
+        for (int q = 0; q < batch; ++q) {
+            PROF(CFDMX_K_BICG_A, launch_bicg_A(g, s, pre, ctx->B, ctx->part, ctx->kst));
+            PROF(CFDMX_K_BICG_FIN, launch_bicg_fin(g, s, ctx->part, ctx->kst, 1));
+            PROF(CFDMX_K_BICG_B, launch_bicg_B(g, s, pre, ctx->B, ctx->part, ctx->kst));
+            PROF(CFDMX_K_BICG_FIN, launch_bicg_fin(g, s, ctx->part, ctx->kst, 2));
+            PROF(CFDMX_K_BICG_C, launch_bicg_C(g, s, ctx->B, ctx->x, ctx->part, ctx->kst));
+            PROF(CFDMX_K_BICG_FIN, launch_bicg_fin(g, s, ctx->part, ctx->kst, 3));
+        }
         LAUNCHCHK();
         HIPCHK(hipMemcpyAsync(ctx->kst_h, ctx->kst, sizeof(KState), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
+        prof_harvest(ctx);
         if (ctx->kst_h->done) break;
         batch = batch < 64 ? batch * 2 : 64;
     }
@@ -712,15 +778,21 @@ static int solve_rbsor_impl(cfdmx_ctx *ctx, int *iters, double *err)
     ss.itmax = ctx->cfg.maxit;
     *ctx->sst_h = ss;
     HIPCHK(hipMemcpyAsync(ctx->sst, ctx->sst_h, sizeof ss, hipMemcpyHostToDevice, s));
-    launch_sor_rhs(g, s, ctx->t1[0], ctx->t1[1], ctx->t1[2], ctx->cfg.dt, ctx->b);
+    PROF(CFDMX_K_RHS, launch_sor_rhs(g, s, ctx->t1[0], ctx->t1[1], ctx->t1[2], ctx->cfg.dt,
+                                     ctx->b));
     LAUNCHCHK();
     int batch = 4;
     for (;;) {
-        for (int q = 0; q < batch; ++q)
-            launch_sor_sweep(g, s, ctx->cfg.sor_omega, ctx->b, ctx->p, ctx->part, ctx->sst);
+        for (int q = 0; q < batch; ++q) {
+            for (int colour = 0; colour < 2; ++colour)
+                PROF(CFDMX_K_SOR, launch_sor_half(g, s, colour, ctx->cfg.sor_omega, ctx->b,
+                                                  ctx->p, ctx->part, ctx->sst));
+            launch_sor_fin(g, s, ctx->part, ctx->sst);
+        }
         LAUNCHCHK();
         HIPCHK(hipMemcpyAsync(ctx->sst_h, ctx->sst, sizeof ss, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
+        prof_harvest(ctx);
         if (ctx->sst_h->done) break;
         batch = batch < 64 ? batch * 2 : 64;
     }
@@ -766,6 +838,7 @@ static int read_scalars(cfdmx_ctx *ctx)
     HIPCHK(hipMemcpyAsync(ctx->scal_h, ctx->scal, 16 * sizeof(double), hipMemcpyDeviceToHost,
                           ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    prof_harvest(ctx);
     return CFDMX_OK;
 }
 
@@ -823,13 +896,14 @@ static int step_once(cfdmx_ctx *ctx, int *iters, double *err, int *solved)
         if (rc) return rc;
     }
     const int from_x = (c.solver == CFDMX_SOLVER_BICGSTAB && *solved) ? 1 : 0;
-    launch_project(g, s, from_x, c.dt, c.outflow_convective, ctx->x, ctx->p, ctx->eta,
-                   ctx->t1[0], ctx->t1[1], ctx->t1[2], ctx->t3[0], ctx->t3[1], ctx->t3[2],
-                   ctx->f, ctx->part);
+    PROF(CFDMX_K_PROJECT,
+         launch_project(g, s, from_x, c.dt, c.outflow_convective, ctx->x, ctx->p, ctx->eta,
+                        ctx->t1[0], ctx->t1[1], ctx->t1[2], ctx->t3[0], ctx->t3[1], ctx->t3[2],
+                        ctx->f, ctx->part));
     launch_cdcl_fin(g, s, ctx->part, c.lz, ctx->scal + 1);
     for (int q = 0; q < 3; ++q) std::swap(ctx->t0[q], ctx->t3[q]);
-    launch_bc(g, s, ctx->bc, 1, ctx->t0[0], ctx->t0[1], ctx->t0[2], ctx->p, ctx->t1[0],
-              ctx->t1[1], ctx->t1[2], ctx->flux_part);
+    PROF(CFDMX_K_BC, launch_bc(g, s, ctx->bc, 1, ctx->t0[0], ctx->t0[1], ctx->t0[2], ctx->p,
+                               ctx->t1[0], ctx->t1[1], ctx->t1[2], ctx->flux_part));
     LAUNCHCHK();
     ctx->time += c.dt;
     ctx->loop += 1;
@@ -870,6 +944,30 @@ int cfdmx_synchronize(cfdmx_ctx *ctx)
 {
     if (!ctx) return CFDMX_ERR_ARG;
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    prof_harvest(ctx);
+    return CFDMX_OK;
+}
+
+int cfdmx_profile(cfdmx_ctx *ctx, int enable)
+{
+    if (!ctx) return CFDMX_ERR_ARG;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    prof_harvest(ctx);
+    for (int q = 0; q < CFDMX_K_COUNT; ++q) {
+        ctx->prof_ms[q] = 0.0;
+        ctx->prof_n[q] = 0;
+    }
+    ctx->prof_on = enable ? 1 : 0;
+    return CFDMX_OK;
+}
+
+int cfdmx_profile_read(cfdmx_ctx *ctx, int kernel, double *total_ms, long long *launches)
+{
+    if (!ctx || kernel < 0 || kernel >= CFDMX_K_COUNT) return CFDMX_ERR_ARG;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    prof_harvest(ctx);
+    if (total_ms) *total_ms = ctx->prof_ms[kernel];
+    if (launches) *launches = ctx->prof_n[kernel];
     return CFDMX_OK;
 }
 

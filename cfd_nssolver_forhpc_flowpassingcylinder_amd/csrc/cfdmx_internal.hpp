@@ -60,15 +60,21 @@ void launch_rhs(const Geo &g, hipStream_t s, int pre, const double *us, const do
 void launch_apply(const Geo &g, hipStream_t s, int pre, const double *x, double *r);
 void launch_bicg_init(const Geo &g, hipStream_t s, int pre, const double *b, const double *x,
                       double *r0, double *r, double *px1, double *part, KState *st);
-void launch_bicg_iter(const Geo &g, hipStream_t s, int pre, BicgBuffers &B, double *x,
-                      double *part, KState *st);
+void launch_bicg_A(const Geo &g, hipStream_t s, int pre, BicgBuffers &B, double *part,
+                   KState *st);
+void launch_bicg_B(const Geo &g, hipStream_t s, int pre, BicgBuffers &B, double *part,
+                   KState *st);
+void launch_bicg_C(const Geo &g, hipStream_t s, BicgBuffers &B, double *x, double *part,
+                   KState *st);
+void launch_bicg_fin(const Geo &g, hipStream_t s, double *part, KState *st, int phase);
 void launch_nanfix(hipStream_t s, int n, double *x, const double *px1, int *count);
 void launch_copy(hipStream_t s, int n, const double *a, double *b);
 void launch_fill(hipStream_t s, int n, double val, double *a);
 void launch_sor_rhs(const Geo &g, hipStream_t s, const double *us, const double *vs,
                     const double *ws, double dt, double *cf6);
-void launch_sor_sweep(const Geo &g, hipStream_t s, double omega, const double *cf6, double *p,
-                      double *part, SorState *st);
+void launch_sor_half(const Geo &g, hipStream_t s, int colour, double omega, const double *cf6,
+                     double *p, double *part, SorState *st);
+void launch_sor_fin(const Geo &g, hipStream_t s, double *part, SorState *st);
 void launch_project(const Geo &g, hipStream_t s, int from_x, double dt, int convective,
                     const double *xres, double *p, const double *eta, const double *us,
                     const double *vs, const double *ws, double *u, double *v, double *w,

@@ -1151,11 +1151,10 @@ void launch_bicg_init(const Geo &g, hipStream_t s, int pre, const double *b, con
 {
     hipLaunchKernelGGL(k_bicg_init, dim3(grid_blocks(g)), dim3(kThreads), 0, s, g, pre, b, x,
                        r0, r, px1, part, st);
-    hipLaunchKernelGGL(k_bicg_fin, dim3(1), dim3(kThreads), 0, s, part, g.nblocks, st, 0);
 }
 
-void launch_bicg_iter(const Geo &g, hipStream_t s, int pre, BicgBuffers &B, double *x,
-                      double *part, KState *st)
+void launch_bicg_A(const Geo &g, hipStream_t s, int pre, BicgBuffers &B, double *part,
+                   KState *st)
 {
     const dim3 gb(grid_blocks(g)), tb(kThreads);
     if (pre)
@@ -1165,15 +1164,28 @@ void launch_bicg_iter(const Geo &g, hipStream_t s, int pre, BicgBuffers &B, doub
         hipLaunchKernelGGL(k_bicg_A<0>, gb, tb, 0, s, g, B.r, B.p[B.cur], B.ap[B.cur], B.r0,
                            B.p[1 - B.cur], B.ap[1 - B.cur], part, st);
     B.cur = 1 - B.cur;
-    hipLaunchKernelGGL(k_bicg_fin, dim3(1), tb, 0, s, part, g.nblocks, st, 1);
+}
+
+void launch_bicg_B(const Geo &g, hipStream_t s, int pre, BicgBuffers &B, double *part,
+                   KState *st)
+{
+    const dim3 gb(grid_blocks(g)), tb(kThreads);
     if (pre)
         hipLaunchKernelGGL(k_bicg_B<1>, gb, tb, 0, s, g, B.r, B.ap[B.cur], B.as, part, st);
     else
         hipLaunchKernelGGL(k_bicg_B<0>, gb, tb, 0, s, g, B.r, B.ap[B.cur], B.as, part, st);
-    hipLaunchKernelGGL(k_bicg_fin, dim3(1), tb, 0, s, part, g.nblocks, st, 2);
-    hipLaunchKernelGGL(k_bicg_C, gb, tb, 0, s, g, B.r, B.ap[B.cur], B.p[B.cur], B.as, B.r0, x,
-                       part, st);
-    hipLaunchKernelGGL(k_bicg_fin, dim3(1), tb, 0, s, part, g.nblocks, st, 3);
+}
+
+void launch_bicg_C(const Geo &g, hipStream_t s, BicgBuffers &B, double *x, double *part,
+                   KState *st)
+{
+    hipLaunchKernelGGL(k_bicg_C, dim3(grid_blocks(g)), dim3(kThreads), 0, s, g, B.r,
+                       B.ap[B.cur], B.p[B.cur], B.as, B.r0, x, part, st);
+}
+
+void launch_bicg_fin(const Geo &g, hipStream_t s, double *part, KState *st, int phase)
+{
+    hipLaunchKernelGGL(k_bicg_fin, dim3(1), dim3(kThreads), 0, s, part, g.nblocks, st, phase);
 }
 
 void launch_nanfix(hipStream_t s, int n, double *x, const double *px1, int *count)
@@ -1198,13 +1210,15 @@ void launch_sor_rhs(const Geo &g, hipStream_t s, const double *us, const double 
                        1.0 / dt, cf6);
 }
 
-void launch_sor_sweep(const Geo &g, hipStream_t s, double omega, const double *cf6, double *p,
-                      double *part, SorState *st)
+void launch_sor_half(const Geo &g, hipStream_t s, int colour, double omega, const double *cf6,
+                     double *p, double *part, SorState *st)
 {
-    hipLaunchKernelGGL(k_sor_half, dim3(grid_blocks(g)), dim3(kThreads), 0, s, g, 0, omega, cf6,
-                       p, part, st);
-    hipLaunchKernelGGL(k_sor_half, dim3(grid_blocks(g)), dim3(kThreads), 0, s, g, 1, omega, cf6,
-                       p, part, st);
+    hipLaunchKernelGGL(k_sor_half, dim3(grid_blocks(g)), dim3(kThreads), 0, s, g, colour, omega,
+                       cf6, p, part, st);
+}
+
+void launch_sor_fin(const Geo &g, hipStream_t s, double *part, SorState *st)
+{
     hipLaunchKernelGGL(k_sor_fin, dim3(1), dim3(kThreads), 0, s, part, g.nblocks, st);
 }
 

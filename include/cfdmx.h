@@ -166,6 +166,25 @@ int cfdmx_step(cfdmx_ctx *ctx, int nsteps, cfdmx_step_info *info);
 
 int cfdmx_synchronize(cfdmx_ctx *ctx);
 
+/* ---- measurement (no reference counterpart: the reference times phases with
+ * stopWatch, STL_clock.hpp:8-42) ---- */
+#define CFDMX_K_CONVDIFF 0
+#define CFDMX_K_BC 1
+#define CFDMX_K_RHS 2
+#define CFDMX_K_BICG_INIT 3
+#define CFDMX_K_BICG_A 4
+#define CFDMX_K_BICG_B 5
+#define CFDMX_K_BICG_C 6
+#define CFDMX_K_BICG_FIN 7
+#define CFDMX_K_SOR 8
+#define CFDMX_K_PROJECT 9
+#define CFDMX_K_COUNT 10
+/* Enable (1) or disable (0) HIP-event timing of every launch of the kernels above on
+ * the context's stream; resets the totals. */
+int cfdmx_profile(cfdmx_ctx *ctx, int enable);
+/* Accumulated device time and launch count of one kernel class. */
+int cfdmx_profile_read(cfdmx_ctx *ctx, int kernel, double *total_ms, long long *launches);
+
 #ifdef __cplusplus
 }
 #endif

@@ -20,6 +20,23 @@
 
 namespace {
 
+// bicgstabRe2::iters_max_ is private and fixed at 5000 (bicgstabRe2.hpp:47); the CPU
+// baseline times a capped solve.  Explicit instantiation may name private members, so
+// a pointer to the member is captured without touching the reference sources.
+template <class Tag, typename Tag::type M> struct Expose {
+    friend typename Tag::type member(Tag) { return M; }
+};
+struct SpeMaxit {
+    typedef int solver::bicgstabRe2<mat::SPE_matrix<double>>::*type;
+    friend type member(SpeMaxit);
+};
+struct CsrMaxit {
+    typedef int solver::bicgstabRe2<mat::CSR_matrix<double>>::*type;
+    friend type member(CsrMaxit);
+};
+template struct Expose<SpeMaxit, &solver::bicgstabRe2<mat::SPE_matrix<double>>::iters_max_>;
+template struct Expose<CsrMaxit, &solver::bicgstabRe2<mat::CSR_matrix<double>>::iters_max_>;
+
 struct RefCtx {
     std::unique_ptr<grid> g;
     simpulationVariable simu;
@@ -264,5 +281,22 @@ void ref_step(void *h, int loop, double *out)
 }
 
 int ref_threads(void) { return omp_get_max_threads(); }
+
+void ref_set_threads(int threads) { omp_set_num_threads(threads); }
+
+void ref_set_maxit(void *h, int maxit)
+{
+    auto *c = static_cast<RefCtx *>(h);
+    if (c->precond) (*c->csr_solver).*member(CsrMaxit()) = maxit;
+    else (*c->spe_solver).*member(SpeMaxit()) = maxit;
+}
+
+void ref_set_tol(void *h, double tol)
+{
+    auto *c = static_cast<RefCtx *>(h);
+    c->simu.p_criteria = tol;
+    if (c->precond) c->csr_solver->setTolerance(tol);
+    else c->spe_solver->setTolerance(tol);
+}
 
 }  // extern "C"

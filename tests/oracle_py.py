@@ -296,6 +296,9 @@ class RefHarness:
         L.ref_apply_A.argtypes = [C.c_void_p, DP, DP]
         L.ref_sor_seq.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int, DP]
         L.ref_step.argtypes = [C.c_void_p, C.c_int, DP]
+        L.ref_set_maxit.argtypes = [C.c_void_p, C.c_int]
+        L.ref_set_tol.argtypes = [C.c_void_p, C.c_double]
+        L.ref_set_threads.argtypes = [C.c_int]
         self.nx, self.ny, self.nz = nxc + 4, nyc + 4, nzc + 4
         self.G = self.nx * self.ny * self.nz
         self.N = nxc * nyc * nzc
@@ -341,6 +344,15 @@ class RefHarness:
         err = C.c_double()
         it = self.lib.ref_sor_seq(self.h, omega, tol, itmax, C.byref(err))
         return it, err.value
+
+    def set_maxit(self, maxit):
+        self.lib.ref_set_maxit(self.h, maxit)
+
+    def set_tol(self, tol):
+        self.lib.ref_set_tol(self.h, tol)
+
+    def set_threads(self, threads):
+        self.lib.ref_set_threads(threads)
 
     def step(self, loop):
         out = np.zeros(6)
