@@ -723,17 +723,13 @@ static int solve_bicgstab_impl(cfdmx_ctx *ctx, int *iters, double *res)
     std::swap(ctx->px1, ctx->px2); // px2 <- px1, then px1 <- x (bicgstabRe2.hpp:94-95)
     PROF(CFDMX_K_BICG_INIT, launch_bicg_init(g, s, pre, ctx->b, ctx->x, ctx->B.r0, ctx->B.r,
                                              ctx->px1, ctx->part, ctx->kst));
-    PROF(CFDMX_K_BICG_FIN, launch_bicg_fin(g, s, ctx->part, ctx->kst, 0));
     LAUNCHCHK();
     int batch = 4;
     for (;;) {
         for (int q = 0; q < batch; ++q) {
             PROF(CFDMX_K_BICG_A, launch_bicg_A(g, s, pre, ctx->B, ctx->part, ctx->kst));
-            PROF(CFDMX_K_BICG_FIN, launch_bicg_fin(g, s, ctx->part, ctx->kst, 1));
             PROF(CFDMX_K_BICG_B, launch_bicg_B(g, s, pre, ctx->B, ctx->part, ctx->kst));
-            PROF(CFDMX_K_BICG_FIN, launch_bicg_fin(g, s, ctx->part, ctx->kst, 2));
             PROF(CFDMX_K_BICG_C, launch_bicg_C(g, s, ctx->B, ctx->x, ctx->part, ctx->kst));
-            PROF(CFDMX_K_BICG_FIN, launch_bicg_fin(g, s, ctx->part, ctx->kst, 3));
         }
         LAUNCHCHK();
         HIPCHK(hipMemcpyAsync(ctx->kst_h, ctx->kst, sizeof(KState), hipMemcpyDeviceToHost, s));

@@ -160,6 +160,29 @@ __device__ __forceinline__ void fold_partials(const double *partials, int stride
     __syncthreads();
 }
 
+// Called by every block after block_sums: the block that finishes last folds all
+// partials (same fixed order as fold_partials) and returns true with the totals in
+// tot; the others return false.  Thread 0 of each block wrote its partial, so its
+// release fence + counter increment publishes it; the last block's acquire fence
+// makes every partial visible across XCDs.  The counter is reset for the next kernel.
+template <int NS>
+__device__ __forceinline__ bool last_block_fold(const double *partials, int stride, int nb,
+                                                unsigned *counter, double (&tot)[NS])
+{
+    __shared__ int is_last;
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const unsigned prev = atomicAdd(counter, 1u);
+        is_last = (prev == (unsigned)nb - 1u) ? 1 : 0;
+    }
+    __syncthreads();
+    if (!is_last) return false;
+    __threadfence();
+    fold_partials<NS>(partials, stride, nb, tot);
+    if (threadIdx.x == 0) *counter = 0u;
+    return true;
+}
+
 __device__ __forceinline__ double wave_max(double v)
 {
 #pragma unroll

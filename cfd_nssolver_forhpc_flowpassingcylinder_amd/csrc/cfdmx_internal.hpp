@@ -30,6 +30,8 @@ struct KState {
     int mode; // next phase A: 0 update p, 1 restart, 2 first iteration
     int timestep;
     int restarts;
+    unsigned counter; // blocks finished in the current phase kernel
+    int pad;
 };
 
 struct SorState {
@@ -66,7 +68,6 @@ void launch_bicg_B(const Geo &g, hipStream_t s, int pre, BicgBuffers &B, double 
                    KState *st);
 void launch_bicg_C(const Geo &g, hipStream_t s, BicgBuffers &B, double *x, double *part,
                    KState *st);
-void launch_bicg_fin(const Geo &g, hipStream_t s, double *part, KState *st, int phase);
 void launch_nanfix(hipStream_t s, int n, double *x, const double *px1, int *count);
 void launch_copy(hipStream_t s, int n, const double *a, double *b);
 void launch_fill(hipStream_t s, int n, double val, double *a);

@@ -338,12 +338,15 @@ __global__ void k_bc_z(Geo g, BCParams bc, double *u, double *v, double *w, doub
                        double *u1, double *v1, double *w1)
 {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= g.nxc * g.nyc) return;
-    const int I = m / g.nyc + kGC, J = m % g.nyc + kGC;
     const int nz = g.nz, nzc = g.nzc;
-    const int b = gidx(g, I, J, 0);
     if (g.zper) {
-        // periodic: ghost k <- k + nzc (k < 2) or k - nzc (k >= nz-2), u, v, w and p
+        // periodic: ghost k <- k + nzc (k < 2) or k - nzc (k >= nz-2) for u, v, w and p
+        // in every (i,j) column (the face k = nz-3 reads x/y ghost columns); T1 receives
+        // the interior columns, as BC_staggered_copy does
+        if (m >= g.nx * g.ny) return;
+        const int I = m / g.ny, J = m % g.ny;
+        const int b = gidx(g, I, J, 0);
+        const bool inner = I >= kGC && I < g.nx - kGC && J >= kGC && J < g.ny - kGC;
         double *f[4] = {u, v, w, p};
         double *f1[4] = {u1, v1, w1, nullptr};
         for (int q = 0; q < 4; ++q) {
@@ -353,7 +356,7 @@ __global__ void k_bc_z(Geo g, BCParams bc, double *u, double *v, double *w, doub
             f[q][b + 0] = a0;
             f[q][b + nz - 2] = a2;
             f[q][b + nz - 1] = a3;
-            if (f1[q]) {
+            if (f1[q] && inner) {
                 f1[q][b + 1] = a1;
                 f1[q][b + 0] = a0;
                 f1[q][b + nz - 2] = a2;
@@ -363,6 +366,9 @@ __global__ void k_bc_z(Geo g, BCParams bc, double *u, double *v, double *w, doub
         }
         return;
     }
+    if (m >= g.nxc * g.nyc) return;
+    const int I = m / g.nyc + kGC, J = m % g.nyc + kGC;
+    const int b = gidx(g, I, J, 0);
     // face 4 (:121-129)
     put2(u, u1, b + 1, b, bc.dir[4][0] * 2. + (2.0 * bc.num[4][0] - 1.0) * u[b + 2]);
     put2(v, v1, b + 1, b, bc.dir[4][1] * 2. + (2.0 * bc.num[4][1] - 1.0) * v[b + 2]);
@@ -588,7 +594,10 @@ k_rhs(Geo g, const double *__restrict__ us, const double *__restrict__ vs,
 //   B: s = r - alpha*Ap, As = A s, partials (As, s), (As, As)       -> omega
 //   C: x += alpha*p + omega*s, r = s - omega*As, partials (r,r), (r,r0)
 //      -> convergence test, beta, restart decision
-// p and s are recomputed at the stencil neighbours from r, p, Ap (no extra pass).
+// The block that finishes last folds the partials and updates the scalars.
+// In A and B the new vector (p or s) of the block's own 1024 cells is formed once
+// into LDS; stencil neighbours outside that range are formed from global memory
+// with the same expression, so every value is bit-identical wherever it is formed.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kThreads)
 k_bicg_init(Geo g, int pre, const double *__restrict__ b, const double *__restrict__ x,
@@ -616,6 +625,15 @@ k_bicg_init(Geo g, int pre, const double *__restrict__ b, const double *__restri
         acc[1] += rr * rr;
     }
     block_sums<2>(acc, part, g.nblocks, L);
+    double t[2];
+    if (last_block_fold<2>(part, g.nblocks, g.nblocks, &st->counter, t) && threadIdx.x == 0) {
+        st->timestep += 1;
+        st->norm0 = sqrt(t[0]); // L2Norm(rhs)
+        st->rho = t[1];         // dot(r1, r0)
+        st->iter = 0;
+        st->done = 0;
+        st->mode = 2;
+    }
 }
 
 template <int PRE>
@@ -623,26 +641,37 @@ __global__ void __launch_bounds__(kThreads)
 k_bicg_A(Geo g, const double *__restrict__ r, const double *__restrict__ p_in,
          const double *__restrict__ ap_in, double *__restrict__ r0,
          double *__restrict__ p_out, double *__restrict__ ap_out, double *__restrict__ part,
-         const KState *st)
+         KState *st)
 {
     if (st->done) return;
     const int L = logical_block(g);
     if (L >= g.nblocks) return;
+    __shared__ double tile[kPerBlock];
     const int mode = st->mode;
     const double beta = st->beta, omega = st->omega;
+    const int base = L * kPerBlock;
+    const int hi = min(base + kPerBlock, g.N);
+    auto pnew = [&](int m) {
+        return mode == 0 ? r[m] + beta * (p_in[m] - omega * ap_in[m]) : r[m];
+    };
+#pragma unroll
+    for (int c = 0; c < kCells; ++c) {
+        const int n = base + c * kThreads + threadIdx.x;
+        if (n < hi) tile[c * kThreads + threadIdx.x] = pnew(n);
+    }
+    __syncthreads();
     double acc[1] = {0.0};
     for (int c = 0; c < kCells; ++c) {
-        const int n = L * kPerBlock + c * kThreads + threadIdx.x;
-        if (n >= g.N) break;
+        const int n = base + c * kThreads + threadIdx.x;
+        if (n >= hi) break;
         int i, j, k;
         decode(g, n, i, j, k);
         RowCoef rc;
         row_coef<PRE>(g, i, j, k, rc);
-        auto pnew = [&](int m) {
-            return mode == 0 ? r[m] + beta * (p_in[m] - omega * ap_in[m]) : r[m];
-        };
-        const double apn = row_apply(rc, n, pnew);
-        const double pn = pnew(n);
+        const double apn = row_apply(rc, n, [&](int m) {
+            return (m >= base && m < hi) ? tile[m - base] : pnew(m);
+        });
+        const double pn = tile[c * kThreads + threadIdx.x];
         p_out[n] = pn;
         ap_out[n] = apn;
         double r0n;
@@ -655,105 +684,95 @@ k_bicg_A(Geo g, const double *__restrict__ r, const double *__restrict__ p_in,
         acc[0] += apn * r0n;
     }
     block_sums<1>(acc, part, g.nblocks, L);
+    double t[1];
+    if (last_block_fold<1>(part, g.nblocks, g.nblocks, &st->counter, t) && threadIdx.x == 0)
+        st->alpha = st->rho / t[0]; // a = r1r0 / dot(ap, r0)
 }
 
 template <int PRE>
 __global__ void __launch_bounds__(kThreads)
 k_bicg_B(Geo g, const double *__restrict__ r, const double *__restrict__ ap,
-         double *__restrict__ as, double *__restrict__ part, const KState *st)
+         double *__restrict__ as, double *__restrict__ part, KState *st)
 {
     if (st->done) return;
     const int L = logical_block(g);
     if (L >= g.nblocks) return;
+    __shared__ double tile[kPerBlock];
     const double alpha = st->alpha;
+    const int base = L * kPerBlock;
+    const int hi = min(base + kPerBlock, g.N);
+    auto snew = [&](int m) { return r[m] - alpha * ap[m]; };
+#pragma unroll
+    for (int c = 0; c < kCells; ++c) {
+        const int n = base + c * kThreads + threadIdx.x;
+        if (n < hi) tile[c * kThreads + threadIdx.x] = snew(n);
+    }
+    __syncthreads();
     double acc[2] = {0.0, 0.0};
     for (int c = 0; c < kCells; ++c) {
-        const int n = L * kPerBlock + c * kThreads + threadIdx.x;
-        if (n >= g.N) break;
+        const int n = base + c * kThreads + threadIdx.x;
+        if (n >= hi) break;
         int i, j, k;
         decode(g, n, i, j, k);
         RowCoef rc;
         row_coef<PRE>(g, i, j, k, rc);
-        auto s = [&](int m) { return r[m] - alpha * ap[m]; };
-        const double asn = row_apply(rc, n, s);
-        const double sn = s(n);
+        const double asn = row_apply(rc, n, [&](int m) {
+            return (m >= base && m < hi) ? tile[m - base] : snew(m);
+        });
+        const double sn = tile[c * kThreads + threadIdx.x];
         as[n] = asn;
         acc[0] += asn * sn;
         acc[1] += asn * asn;
     }
     block_sums<2>(acc, part, g.nblocks, L);
+    double t[2];
+    if (last_block_fold<2>(part, g.nblocks, g.nblocks, &st->counter, t) && threadIdx.x == 0)
+        st->omega = t[0] / t[1]; // w = dot(as, s) / dot(as, as)
 }
 
 __global__ void __launch_bounds__(kThreads)
 k_bicg_C(Geo g, double *__restrict__ r, const double *__restrict__ ap,
          const double *__restrict__ p, const double *__restrict__ as,
          const double *__restrict__ r0, double *__restrict__ x, double *__restrict__ part,
-         const KState *st)
+         KState *st)
 {
     if (st->done) return;
     const int L = logical_block(g);
     if (L >= g.nblocks) return;
     const double alpha = st->alpha, omega = st->omega;
     double acc[2] = {0.0, 0.0};
+#pragma unroll
     for (int c = 0; c < kCells; ++c) {
         const int n = L * kPerBlock + c * kThreads + threadIdx.x;
-        if (n >= g.N) break;
-        const double sn = r[n] - alpha * ap[n];
-        x[n] += alpha * p[n] + omega * sn;
-        const double rn = sn - omega * as[n];
-        r[n] = rn;
-        acc[0] += rn * rn;
-        acc[1] += rn * r0[n];
+        if (n < g.N) {
+            const double sn = r[n] - alpha * ap[n];
+            x[n] += alpha * p[n] + omega * sn;
+            const double rn = sn - omega * as[n];
+            r[n] = rn;
+            acc[0] += rn * rn;
+            acc[1] += rn * r0[n];
+        }
     }
     block_sums<2>(acc, part, g.nblocks, L);
-}
-
-// scalar updates between the phases (one block)
-__global__ void __launch_bounds__(kThreads)
-k_bicg_fin(const double *part, int nb, KState *st, int phase)
-{
-    if (phase != 0 && st->done) return;
-    if (phase == 0) {
-        double t[2];
-        fold_partials<2>(part, nb, nb, t);
-        if (threadIdx.x == 0) {
-            st->timestep += 1;
-            st->norm0 = sqrt(t[0]);
+    double t[2];
+    if (last_block_fold<2>(part, g.nblocks, g.nblocks, &st->counter, t) && threadIdx.x == 0) {
+        const double res = sqrt(t[0]) / st->norm0; // L2Norm(r1) / norm0
+        st->res = res;
+        if (res < st->tol) {
+            st->done = 1;
+        } else if (st->iter >= st->maxit) {
+            st->done = 2;
+        } else {
+            const double rho_l = st->rho;
             st->rho = t[1];
-            st->iter = 0;
-            st->done = 0;
-            st->mode = 2;
-        }
-    } else if (phase == 1) {
-        double t[1];
-        fold_partials<1>(part, nb, nb, t);
-        if (threadIdx.x == 0) st->alpha = st->rho / t[0];
-    } else if (phase == 2) {
-        double t[2];
-        fold_partials<2>(part, nb, nb, t);
-        if (threadIdx.x == 0) st->omega = t[0] / t[1];
-    } else {
-        double t[2];
-        fold_partials<2>(part, nb, nb, t);
-        if (threadIdx.x == 0) {
-            const double res = sqrt(t[0]) / st->norm0;
-            st->res = res;
-            if (res < st->tol) {
-                st->done = 1;
-            } else if (st->iter >= st->maxit) {
-                st->done = 2;
+            st->beta = (st->alpha / st->omega) * (st->rho / rho_l);
+            if (st->timestep < 10 || fabs(rho_l) > st->tol * st->restart_factor) {
+                st->mode = 0;
             } else {
-                const double rho_l = st->rho;
-                st->rho = t[1];
-                st->beta = (st->alpha / st->omega) * (st->rho / rho_l);
-                if (st->timestep < 10 || fabs(rho_l) > st->tol * st->restart_factor) {
-                    st->mode = 0;
-                } else {
-                    st->mode = 1;
-                    st->restarts += 1;
-                }
-                st->iter += 1;
+                st->mode = 1;
+                st->restarts += 1;
             }
+            st->iter += 1;
         }
     }
 }
@@ -1114,8 +1133,9 @@ void launch_bc(const Geo &g, hipStream_t s, const BCParams &bc, int advance, dou
     }
     hipLaunchKernelGGL(k_bc_y, dim3(cdiv(g.nxc * g.nzc, 256)), dim3(256), 0, s, g, bc, u, v, w,
                        p, u1, v1, w1);
-    hipLaunchKernelGGL(k_bc_z, dim3(cdiv(g.nxc * g.nyc, 256)), dim3(256), 0, s, g, bc, u, v, w,
-                       p, u1, v1, w1);
+    const int nzcols = g.zper ? g.nx * g.ny : g.nxc * g.nyc;
+    hipLaunchKernelGGL(k_bc_z, dim3(cdiv(nzcols, 256)), dim3(256), 0, s, g, bc, u, v, w, p, u1,
+                       v1, w1);
 }
 
 void launch_bc_copy(const Geo &g, hipStream_t s, const double *u0, const double *v0,
@@ -1183,10 +1203,6 @@ void launch_bicg_C(const Geo &g, hipStream_t s, BicgBuffers &B, double *x, doubl
                        B.ap[B.cur], B.p[B.cur], B.as, B.r0, x, part, st);
 }
 
-void launch_bicg_fin(const Geo &g, hipStream_t s, double *part, KState *st, int phase)
-{
-    hipLaunchKernelGGL(k_bicg_fin, dim3(1), dim3(kThreads), 0, s, part, g.nblocks, st, phase);
-}
 
 void launch_nanfix(hipStream_t s, int n, double *x, const double *px1, int *count)
 {

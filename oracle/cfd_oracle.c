@@ -24,7 +24,7 @@ static inline long ICAL(const orc_grid *g, long i, long j, long k)
     return ((i - GC) * g->nyc + (j - GC)) * g->nzc + (k - GC);
 }
 
-static void periodic_z_fill(const orc_grid *g, double *a);
+static void periodic_z_fill(const orc_grid *g, double *a, int all_columns);
 
 /* ------------------------------------------------------------------------ */
 /* grid: init_uniform structureGrid.hpp:336-367; unit_SML :250-310;          */
@@ -341,9 +341,9 @@ void orc_convdiff(const orc_grid *g, double dt, double nu, int z_periodic,
     if (z_periodic) {
         /* u* z-ghosts are images of the new interior (the divergence at k = 2 reads
          * the face k = 1 == nz-3); no reference counterpart */
-        periodic_z_fill(g, us);
-        periodic_z_fill(g, vs);
-        periodic_z_fill(g, ws);
+        periodic_z_fill(g, us, 0);
+        periodic_z_fill(g, vs, 0);
+        periodic_z_fill(g, ws, 0);
     }
 }
 
@@ -352,11 +352,15 @@ void orc_convdiff(const orc_grid *g, double dt, double nu, int z_periodic,
 /* (faces applied in order 0..5, then p), FLOWpassing table FLOWpassing.hpp. */
 /* Periodic z and convective outflow have no reference counterpart.         */
 /* ------------------------------------------------------------------------ */
-static void periodic_z_fill(const orc_grid *g, double *a)
+/* periodic images along z: ghost k <- k + nzc (k < 2), k - nzc (k >= nz-2).  The
+ * boundary-condition fill covers every (i,j) column, so x/y ghost columns also get
+ * images (the w face k = nz-3, computed when z is periodic, reads them). */
+static void periodic_z_fill(const orc_grid *g, double *a, int all_columns)
 {
     const int nx = g->nx, ny = g->ny, nz = g->nz, nzc = g->nzc;
-    for (int i = GC; i < nx - GC; ++i)
-        for (int j = GC; j < ny - GC; ++j) {
+    const int lo = all_columns ? 0 : GC;
+    for (int i = lo; i < nx - lo; ++i)
+        for (int j = lo; j < ny - lo; ++j) {
             const long c = IC(g, i, j, 0);
             a[c + 1] = a[c + 1 + nzc];
             a[c + 0] = a[c + 0 + nzc];
@@ -446,9 +450,9 @@ void orc_bc_main(const orc_grid *g, const orc_bc *bc, int advance_outflow, doubl
                 Dir[b][2] * 2. + (2.0 * Num[b][2] - 1.0) * w[IC(g, i, ny - 3, k)];
         }
     if (bc->z_periodic) {
-        periodic_z_fill(g, u);
-        periodic_z_fill(g, v);
-        periodic_z_fill(g, w);
+        periodic_z_fill(g, u, 1);
+        periodic_z_fill(g, v, 1);
+        periodic_z_fill(g, w, 1);
     } else {
         /* face 4: :121-129 */
         for (int i = GC; i < nx - GC; ++i)
@@ -484,7 +488,7 @@ void orc_bc_main(const orc_grid *g, const orc_bc *bc, int advance_outflow, doubl
             p[IC(g, i, ny - 1, k)] = p[IC(g, i, ny - 2, k)] = p[IC(g, i, ny - 3, k)];
         }
     if (bc->z_periodic) {
-        periodic_z_fill(g, p);
+        periodic_z_fill(g, p, 1);
     } else {
         for (int i = GC; i < nx - GC; ++i)
             for (int j = GC; j < ny - GC; ++j) {
