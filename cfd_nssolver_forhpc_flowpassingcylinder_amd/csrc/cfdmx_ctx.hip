@@ -261,6 +261,9 @@ static int setup_device(cfdmx_ctx *ctx)
     g.xcd_remap = c.xcd_remap ? 1 : 0;
     g.dz = make_fastdiv((unsigned)c.nz);
     g.dy = make_fastdiv((unsigned)c.ny);
+    g.ntile = (g.cyz + kTile - 1) / kTile;
+    g.nchunk = (c.nx + kChunk - 1) / kChunk;
+    g.nmarch = g.ntile * g.nchunk;
 
     // metric tables in one buffer
     std::vector<double> buf;
@@ -327,7 +330,7 @@ static int setup_device(cfdmx_ctx *ctx)
     ALLOC(ctx->B.p[1], N);
     ALLOC(ctx->B.ap[0], N);
     ALLOC(ctx->B.ap[1], N);
-    ALLOC(ctx->part, 4 * (size_t)g.nblocks + 64);
+    ALLOC(ctx->part, 4 * (size_t)(g.nblocks > g.nmarch ? g.nblocks : g.nmarch) + 64);
     ALLOC(ctx->cfl_part, (size_t)g.nblocks + 8);
     ALLOC(ctx->flux_part, 3 * ((size_t)g.cyz / kPerBlock + 2));
     ALLOC(ctx->scal, 16);
@@ -396,6 +399,7 @@ static int validate(cfdmx_ctx *ctx)
     if (c.maxit < 0) FAIL(CFDMX_ERR_ARG, "maxit must be >= 0");
     if (c.precision != CFDMX_PRECISION_FP64)
         FAIL(CFDMX_ERR_ARG, "precision %d is not available", c.precision);
+    if (c.nz > 640) FAIL(CFDMX_ERR_ARG, "nz above 640 is not supported (LDS ring of the stencil)");
     if (c.z_periodic && c.solver == CFDMX_SOLVER_RBSOR && (c.nz % 2) != 0)
         FAIL(CFDMX_ERR_ARG, "red-black SOR with periodic z needs an even nz");
     if (c.flux_correction) {
@@ -723,13 +727,17 @@ static int solve_bicgstab_impl(cfdmx_ctx *ctx, int *iters, double *res)
     std::swap(ctx->px1, ctx->px2); // px2 <- px1, then px1 <- x (bicgstabRe2.hpp:94-95)
     PROF(CFDMX_K_BICG_INIT, launch_bicg_init(g, s, pre, ctx->b, ctx->x, ctx->B.r0, ctx->B.r,
                                              ctx->px1, ctx->part, ctx->kst));
+    PROF(CFDMX_K_BICG_FIN, launch_bicg_fin(g, s, ctx->part, ctx->kst, 0));
     LAUNCHCHK();
     int batch = 4;
     for (;;) {
         for (int q = 0; q < batch; ++q) {
             PROF(CFDMX_K_BICG_A, launch_bicg_A(g, s, pre, ctx->B, ctx->part, ctx->kst));
+            PROF(CFDMX_K_BICG_FIN, launch_bicg_fin(g, s, ctx->part, ctx->kst, 1));
             PROF(CFDMX_K_BICG_B, launch_bicg_B(g, s, pre, ctx->B, ctx->part, ctx->kst));
+            PROF(CFDMX_K_BICG_FIN, launch_bicg_fin(g, s, ctx->part, ctx->kst, 2));
             PROF(CFDMX_K_BICG_C, launch_bicg_C(g, s, ctx->B, ctx->x, ctx->part, ctx->kst));
+            PROF(CFDMX_K_BICG_FIN, launch_bicg_fin(g, s, ctx->part, ctx->kst, 3));
         }
         LAUNCHCHK();
         HIPCHK(hipMemcpyAsync(ctx->kst_h, ctx->kst, sizeof(KState), hipMemcpyDeviceToHost, s));

@@ -5,10 +5,14 @@
 // index n = (i*nyc + j)*nzc + k, i = I-2 etc. (structureGrid.hpp:176-181), which is
 // also the layout of the pressure unknowns.
 //
-// Canonical work shape: every kernel that produces a sum uses blocks of 256 threads,
-// block L covering n in [1024L, 1024L+1024) with thread t taking n = 1024L + 256c + t
-// for c = 0..3.  Each thread accumulates its cells in c order, each 64-lane wave
-// folds with a[l] += a[l+s] for s = 32..1, and the block combines its waves as
+// Canonical work shapes: kernels that produce a sum use blocks of 256 threads.
+//  linear: block L covers n in [1024L, 1024L+1024), thread t takes n = 1024L + 256c + t
+//          for c = 0..3;
+//  march:  block b = chunk*ntile + tile covers planes [8*chunk, 8*chunk+8) of the
+//          plane-index tile [1024*tile, 1024*tile+1024), thread t takes, plane by plane,
+//          q = 1024*tile + 256c + t for c = 0..3.
+// Each thread accumulates its cells in that order, each 64-lane wave folds with
+// a[l] += a[l+s] for s = 32..1, and the block combines its waves as
 // (w0 + w1) + (w2 + w3).  The per-block partials are folded the same way by one
 // block (thread t sums partials t, t+256, ...).  The result is independent of
 // scheduling and is reproduced exactly by the CPU oracle.
@@ -22,6 +26,11 @@ constexpr int kThreads = 256;
 constexpr int kCells = 4;
 constexpr int kPerBlock = kThreads * kCells;
 constexpr int kGC = 2;
+// plane-marching stencil kernels: a block owns kTile consecutive cells of the (j,k)
+// plane index q = j*nzc + k and marches over kChunk planes i; thread t takes
+// q = q0 + 256c + t, c = 0..3, on every plane, and accumulates plane by plane
+constexpr int kTile = 1024;
+constexpr int kChunk = 8;
 
 // n / d for n * d < 2^40, with m = ceil(2^40 / d)
 struct FastDiv {
@@ -83,6 +92,7 @@ struct Geo {
     int nblocks;       // ceil(N / 1024)
     int xcd_remap;
     FastDiv dz, dy;    // by nzc, by nyc
+    int ntile, nchunk, nmarch; // plane tiles, plane chunks, march blocks
     Axis ax[3];
     PAxis pa[3];
 };
@@ -147,9 +157,20 @@ __device__ __forceinline__ void fold_partials(const double *partials, int stride
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
+        // same order as one add per partial; loads are issued 8 at a time and
+        // out-of-range slots add +0.0, an identity here (the sum starts at +0.0
+        // and so never holds -0.0)
         double acc = 0.0;
-        for (int m = threadIdx.x; m < nb; m += kThreads)
-            acc += partials[s * stride + m];
+        for (int m0 = threadIdx.x; m0 < nb; m0 += 8 * kThreads) {
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int m = m0 + u * kThreads;
+                v[u] = m < nb ? partials[s * stride + m] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) acc += v[u];
+        }
         const double w = wave_fold(acc);
         if (lane == 0) red[s][wave] = w;
     }
@@ -158,29 +179,6 @@ __device__ __forceinline__ void fold_partials(const double *partials, int stride
     for (int s = 0; s < NS; ++s)
         tot[s] = (red[s][0] + red[s][1]) + (red[s][2] + red[s][3]);
     __syncthreads();
-}
-
-// Called by every block after block_sums: the block that finishes last folds all
-// partials (same fixed order as fold_partials) and returns true with the totals in
-// tot; the others return false.  Thread 0 of each block wrote its partial, so its
-// release fence + counter increment publishes it; the last block's acquire fence
-// makes every partial visible across XCDs.  The counter is reset for the next kernel.
-template <int NS>
-__device__ __forceinline__ bool last_block_fold(const double *partials, int stride, int nb,
-                                                unsigned *counter, double (&tot)[NS])
-{
-    __shared__ int is_last;
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const unsigned prev = atomicAdd(counter, 1u);
-        is_last = (prev == (unsigned)nb - 1u) ? 1 : 0;
-    }
-    __syncthreads();
-    if (!is_last) return false;
-    __threadfence();
-    fold_partials<NS>(partials, stride, nb, tot);
-    if (threadIdx.x == 0) *counter = 0u;
-    return true;
 }
 
 __device__ __forceinline__ double wave_max(double v)

@@ -30,8 +30,6 @@ struct KState {
     int mode; // next phase A: 0 update p, 1 restart, 2 first iteration
     int timestep;
     int restarts;
-    unsigned counter; // blocks finished in the current phase kernel
-    int pad;
 };
 
 struct SorState {
@@ -66,6 +64,7 @@ void launch_bicg_A(const Geo &g, hipStream_t s, int pre, BicgBuffers &B, double 
                    KState *st);
 void launch_bicg_B(const Geo &g, hipStream_t s, int pre, BicgBuffers &B, double *part,
                    KState *st);
+void launch_bicg_fin(const Geo &g, hipStream_t s, double *part, KState *st, int phase);
 void launch_bicg_C(const Geo &g, hipStream_t s, BicgBuffers &B, double *x, double *part,
                    KState *st);
 void launch_nanfix(hipStream_t s, int n, double *x, const double *px1, int *count);

@@ -594,7 +594,7 @@ k_rhs(Geo g, const double *__restrict__ us, const double *__restrict__ vs,
 //   B: s = r - alpha*Ap, As = A s, partials (As, s), (As, As)       -> omega
 //   C: x += alpha*p + omega*s, r = s - omega*As, partials (r,r), (r,r0)
 //      -> convergence test, beta, restart decision
-// The block that finishes last folds the partials and updates the scalars.
+// One-block kernels (k_bicg_fin) fold the partials and update the scalars.
 // In A and B the new vector (p or s) of the block's own 1024 cells is formed once
 // into LDS; stencil neighbours outside that range are formed from global memory
 // with the same expression, so every value is bit-identical wherever it is formed.
@@ -625,109 +625,159 @@ k_bicg_init(Geo g, int pre, const double *__restrict__ b, const double *__restri
         acc[1] += rr * rr;
     }
     block_sums<2>(acc, part, g.nblocks, L);
-    double t[2];
-    if (last_block_fold<2>(part, g.nblocks, g.nblocks, &st->counter, t) && threadIdx.x == 0) {
-        st->timestep += 1;
-        st->norm0 = sqrt(t[0]); // L2Norm(rhs)
-        st->rho = t[1];         // dot(r1, r0)
-        st->iter = 0;
-        st->done = 0;
-        st->mode = 2;
-    }
 }
 
+// Plane-marching stencil phases.  A block owns the plane-index tile
+// q in [q0, q0+1024) (q = j*nzc + k) for the planes i in [i0, i0+8); an LDS ring
+// holds the new vector (p for A, s for B) on planes i-1, i, i+1 over the tile plus
+// one z-row (nzc cells) on each side, so every stencil neighbour is read from LDS.
+// Each ring value is formed once per block from r, p, Ap with the reference's
+// expression; cells of neighbouring tiles/chunks are formed again there, identically.
+struct MarchIO {
+    const double *r, *p, *ap; // inputs
+    double *r0;               // A: restart target
+    double *out_p, *out_ap;   // A outputs
+    double *out_as;           // B output
+    double *part;
+};
+
 template <int PRE>
-__global__ void __launch_bounds__(kThreads)
-k_bicg_A(Geo g, const double *__restrict__ r, const double *__restrict__ p_in,
-         const double *__restrict__ ap_in, double *__restrict__ r0,
-         double *__restrict__ p_out, double *__restrict__ ap_out, double *__restrict__ part,
-         KState *st)
+__device__ __forceinline__ void march_row(const Geo &g, int i, int j, int k, double xm, double xp,
+                                          double ym, double yp, double zm, double zp,
+                                          double v[7])
 {
-    if (st->done) return;
-    const int L = logical_block(g);
-    if (L >= g.nblocks) return;
-    __shared__ double tile[kPerBlock];
-    const int mode = st->mode;
-    const double beta = st->beta, omega = st->omega;
-    const int base = L * kPerBlock;
-    const int hi = min(base + kPerBlock, g.N);
-    auto pnew = [&](int m) {
-        return mode == 0 ? r[m] + beta * (p_in[m] - omega * ap_in[m]) : r[m];
-    };
-#pragma unroll
-    for (int c = 0; c < kCells; ++c) {
-        const int n = base + c * kThreads + threadIdx.x;
-        if (n < hi) tile[c * kThreads + threadIdx.x] = pnew(n);
-    }
-    __syncthreads();
-    double acc[1] = {0.0};
-    for (int c = 0; c < kCells; ++c) {
-        const int n = base + c * kThreads + threadIdx.x;
-        if (n >= hi) break;
-        int i, j, k;
-        decode(g, n, i, j, k);
-        RowCoef rc;
-        row_coef<PRE>(g, i, j, k, rc);
-        const double apn = row_apply(rc, n, [&](int m) {
-            return (m >= base && m < hi) ? tile[m - base] : pnew(m);
-        });
-        const double pn = tile[c * kThreads + threadIdx.x];
-        p_out[n] = pn;
-        ap_out[n] = apn;
-        double r0n;
-        if (mode == 1) {
-            r0n = r[n];
-            r0[n] = r0n;
-        } else {
-            r0n = r0[n];
+    if (PRE == 0) {
+        double ijk = xm + ym + zm + xp + yp + zp;
+        if (i == 0) ijk -= xm;
+        if (i == g.nxc - 1) ijk -= xp;
+        if (j == 0) ijk -= ym;
+        if (j == g.nyc - 1) ijk -= yp;
+        if (!g.zper) {
+            if (k == 0) ijk -= zm;
+            if (k == g.nzc - 1) ijk -= zp;
         }
-        acc[0] += apn * r0n;
+        v[0] = -xm;
+        v[1] = -yp;
+        v[2] = -zm;
+        v[3] = ijk;
+        v[4] = -zp;
+        v[5] = -ym;
+        v[6] = -xp;
+    } else {
+        double aa = 0.0, bb = 0.0, cc = 0.0;
+        if (i == 0) aa = -xm;
+        if (i == g.nxc - 1) aa = -xp;
+        if (j == 0) bb = -ym;
+        if (j == g.nyc - 1) bb = -yp;
+        if (!g.zper) {
+            if (k == 0) cc = -zm;
+            if (k == g.nzc - 1) cc = -zp;
+        }
+        const double ijk = xm + ym + zm + xp + yp + zp + aa + bb + cc;
+        const double jp = 1.0 / ijk;
+        v[0] = -xm * jp;
+        v[1] = -ym * jp;
+        v[2] = -zm * jp;
+        v[3] = ijk * jp;
+        v[4] = -zp * jp;
+        v[5] = -yp * jp;
+        v[6] = -xp * jp;
     }
-    block_sums<1>(acc, part, g.nblocks, L);
-    double t[1];
-    if (last_block_fold<1>(part, g.nblocks, g.nblocks, &st->counter, t) && threadIdx.x == 0)
-        st->alpha = st->rho / t[0]; // a = r1r0 / dot(ap, r0)
 }
 
-template <int PRE>
+template <int PHASE, int PRE>
 __global__ void __launch_bounds__(kThreads)
-k_bicg_B(Geo g, const double *__restrict__ r, const double *__restrict__ ap,
-         double *__restrict__ as, double *__restrict__ part, KState *st)
+k_march(Geo g, MarchIO io, KState *st)
 {
     if (st->done) return;
-    const int L = logical_block(g);
-    if (L >= g.nblocks) return;
-    __shared__ double tile[kPerBlock];
-    const double alpha = st->alpha;
-    const int base = L * kPerBlock;
-    const int hi = min(base + kPerBlock, g.N);
-    auto snew = [&](int m) { return r[m] - alpha * ap[m]; };
+    int b = blockIdx.x;
+    if (g.xcd_remap) b = (b & 7) * ((gridDim.x + 7) >> 3) + (b >> 3);
+    if (b >= g.nmarch) return;
+    extern __shared__ double ring[];
+    const int tile = b % g.ntile, chunk = b / g.ntile;
+    const int i0 = chunk * kChunk, i1 = min(i0 + kChunk, g.nxc);
+    const int q0 = tile * kTile, qn = min(kTile, g.cyz - q0);
+    const int H = g.nzc, W = qn + 2 * H, WS = kTile + 2 * H;
+    const int mode = st->mode;
+    const double beta = st->beta, omega = st->omega, alpha = st->alpha;
+    auto f = [&](int n) -> double {
+        if (PHASE == 0) return mode == 0 ? io.r[n] + beta * (io.p[n] - omega * io.ap[n]) : io.r[n];
+        return io.r[n] - alpha * io.ap[n];
+    };
+    auto fill = [&](int slot, int i) {
+        double *R = ring + slot * WS;
+        const bool in = (i >= 0 && i < g.nxc);
+        for (int t = threadIdx.x; t < W; t += kThreads) {
+            const int q = q0 - H + t;
+            R[t] = (in && q >= 0 && q < g.cyz) ? f(i * g.cyz + q) : 0.0;
+        }
+    };
+    int jc[kCells], kc[kCells];
+    double ymv[kCells], ypv[kCells], zmv[kCells], zpv[kCells];
 #pragma unroll
     for (int c = 0; c < kCells; ++c) {
-        const int n = base + c * kThreads + threadIdx.x;
-        if (n < hi) tile[c * kThreads + threadIdx.x] = snew(n);
+        const int q = q0 + c * kThreads + threadIdx.x;
+        const int j = (int)fdiv((unsigned)q, g.dz);
+        jc[c] = j;
+        kc[c] = q - j * g.nzc;
+        const bool ok = (c * kThreads + (int)threadIdx.x) < qn;
+        ymv[c] = ok ? g.pa[1].m[j] : 0.0;
+        ypv[c] = ok ? g.pa[1].p[j] : 0.0;
+        zmv[c] = ok ? g.pa[2].m[kc[c]] : 0.0;
+        zpv[c] = ok ? g.pa[2].p[kc[c]] : 0.0;
     }
-    __syncthreads();
+    fill(0, i0 - 1);
+    fill(1, i0);
     double acc[2] = {0.0, 0.0};
-    for (int c = 0; c < kCells; ++c) {
-        const int n = base + c * kThreads + threadIdx.x;
-        if (n >= hi) break;
-        int i, j, k;
-        decode(g, n, i, j, k);
-        RowCoef rc;
-        row_coef<PRE>(g, i, j, k, rc);
-        const double asn = row_apply(rc, n, [&](int m) {
-            return (m >= base && m < hi) ? tile[m - base] : snew(m);
-        });
-        const double sn = tile[c * kThreads + threadIdx.x];
-        as[n] = asn;
-        acc[0] += asn * sn;
-        acc[1] += asn * asn;
+    for (int i = i0; i < i1; ++i) {
+        const int sm = (i - i0) % 3, sc = (i - i0 + 1) % 3, sp = (i - i0 + 2) % 3;
+        fill(sp, i + 1);
+        __syncthreads();
+        const double *Rm = ring + sm * WS, *Rc = ring + sc * WS, *Rp = ring + sp * WS;
+        const double xm = g.pa[0].m[i], xp = g.pa[0].p[i];
+#pragma unroll
+        for (int c = 0; c < kCells; ++c) {
+            const int tq = c * kThreads + threadIdx.x;
+            if (tq >= qn) break;
+            const int j = jc[c], k = kc[c];
+            double v[7];
+            march_row<PRE>(g, i, j, k, xm, xp, ymv[c], ypv[c], zmv[c], zpv[c], v);
+            const int tc = tq + H;
+            double s = 0.0;
+            if (i != 0) s += v[0] * Rm[tc];
+            if (j != 0) s += v[1] * Rc[tc - g.nzc];
+            if (k != 0 || g.zper) s += v[2] * Rc[k == 0 ? tc + g.nzc - 1 : tc - 1];
+            s += v[3] * Rc[tc];
+            if (k != g.nzc - 1 || g.zper) s += v[4] * Rc[k == g.nzc - 1 ? tc - (g.nzc - 1) : tc + 1];
+            if (j != g.nyc - 1) s += v[5] * Rc[tc + g.nzc];
+            if (i != g.nxc - 1) s += v[6] * Rp[tc];
+            const int n = i * g.cyz + q0 + tq;
+            const double own = Rc[tc];
+            if (PHASE == 0) {
+                io.out_p[n] = own;
+                io.out_ap[n] = s;
+                double r0n;
+                if (mode == 1) {
+                    r0n = io.r[n];
+                    io.r0[n] = r0n;
+                } else {
+                    r0n = io.r0[n];
+                }
+                acc[0] += s * r0n;
+            } else {
+                io.out_as[n] = s;
+                acc[0] += s * own;
+                acc[1] += s * s;
+            }
+        }
+        __syncthreads();
     }
-    block_sums<2>(acc, part, g.nblocks, L);
-    double t[2];
-    if (last_block_fold<2>(part, g.nblocks, g.nblocks, &st->counter, t) && threadIdx.x == 0)
-        st->omega = t[0] / t[1]; // w = dot(as, s) / dot(as, as)
+    if (PHASE == 0) {
+        double a1[1] = {acc[0]};
+        block_sums<1>(a1, io.part, g.nmarch, b);
+    } else {
+        block_sums<2>(acc, io.part, g.nmarch, b);
+    }
 }
 
 __global__ void __launch_bounds__(kThreads)
@@ -754,25 +804,54 @@ k_bicg_C(Geo g, double *__restrict__ r, const double *__restrict__ ap,
         }
     }
     block_sums<2>(acc, part, g.nblocks, L);
-    double t[2];
-    if (last_block_fold<2>(part, g.nblocks, g.nblocks, &st->counter, t) && threadIdx.x == 0) {
-        const double res = sqrt(t[0]) / st->norm0; // L2Norm(r1) / norm0
-        st->res = res;
-        if (res < st->tol) {
-            st->done = 1;
-        } else if (st->iter >= st->maxit) {
-            st->done = 2;
-        } else {
-            const double rho_l = st->rho;
-            st->rho = t[1];
-            st->beta = (st->alpha / st->omega) * (st->rho / rho_l);
-            if (st->timestep < 10 || fabs(rho_l) > st->tol * st->restart_factor) {
-                st->mode = 0;
+}
+
+// scalar updates between the phases (one block)
+__global__ void __launch_bounds__(kThreads)
+k_bicg_fin(const double *part, int nb, KState *st, int phase)
+{
+    if (phase != 0 && st->done) return;
+    if (phase == 0) {
+        double t[2];
+        fold_partials<2>(part, nb, nb, t);
+        if (threadIdx.x == 0) {
+            st->timestep += 1;
+            st->norm0 = sqrt(t[0]); // L2Norm(rhs)
+            st->rho = t[1];         // dot(r1, r0)
+            st->iter = 0;
+            st->done = 0;
+            st->mode = 2;
+        }
+    } else if (phase == 1) {
+        double t[1];
+        fold_partials<1>(part, nb, nb, t);
+        if (threadIdx.x == 0) st->alpha = st->rho / t[0]; // a = r1r0 / dot(ap, r0)
+    } else if (phase == 2) {
+        double t[2];
+        fold_partials<2>(part, nb, nb, t);
+        if (threadIdx.x == 0) st->omega = t[0] / t[1]; // w = dot(as, s) / dot(as, as)
+    } else {
+        double t[2];
+        fold_partials<2>(part, nb, nb, t);
+        if (threadIdx.x == 0) {
+            const double res = sqrt(t[0]) / st->norm0; // L2Norm(r1) / norm0
+            st->res = res;
+            if (res < st->tol) {
+                st->done = 1;
+            } else if (st->iter >= st->maxit) {
+                st->done = 2;
             } else {
-                st->mode = 1;
-                st->restarts += 1;
+                const double rho_l = st->rho;
+                st->rho = t[1];
+                st->beta = (st->alpha / st->omega) * (st->rho / rho_l);
+                if (st->timestep < 10 || fabs(rho_l) > st->tol * st->restart_factor) {
+                    st->mode = 0;
+                } else {
+                    st->mode = 1;
+                    st->restarts += 1;
+                }
+                st->iter += 1;
             }
-            st->iter += 1;
         }
     }
 }
@@ -1173,27 +1252,55 @@ void launch_bicg_init(const Geo &g, hipStream_t s, int pre, const double *b, con
                        r0, r, px1, part, st);
 }
 
+static inline int march_blocks(const Geo &g)
+{
+    return g.xcd_remap ? ((g.nmarch + 7) / 8) * 8 : g.nmarch;
+}
+static inline size_t march_lds(const Geo &g)
+{
+    return (size_t)3 * (kTile + 2 * g.nzc) * sizeof(double);
+}
+
 void launch_bicg_A(const Geo &g, hipStream_t s, int pre, BicgBuffers &B, double *part,
                    KState *st)
 {
-    const dim3 gb(grid_blocks(g)), tb(kThreads);
+    MarchIO io = {};
+    io.r = B.r;
+    io.p = B.p[B.cur];
+    io.ap = B.ap[B.cur];
+    io.r0 = B.r0;
+    io.out_p = B.p[1 - B.cur];
+    io.out_ap = B.ap[1 - B.cur];
+    io.part = part;
     if (pre)
-        hipLaunchKernelGGL(k_bicg_A<1>, gb, tb, 0, s, g, B.r, B.p[B.cur], B.ap[B.cur], B.r0,
-                           B.p[1 - B.cur], B.ap[1 - B.cur], part, st);
+        hipLaunchKernelGGL((k_march<0, 1>), dim3(march_blocks(g)), dim3(kThreads), march_lds(g),
+                           s, g, io, st);
     else
-        hipLaunchKernelGGL(k_bicg_A<0>, gb, tb, 0, s, g, B.r, B.p[B.cur], B.ap[B.cur], B.r0,
-                           B.p[1 - B.cur], B.ap[1 - B.cur], part, st);
+        hipLaunchKernelGGL((k_march<0, 0>), dim3(march_blocks(g)), dim3(kThreads), march_lds(g),
+                           s, g, io, st);
     B.cur = 1 - B.cur;
 }
 
 void launch_bicg_B(const Geo &g, hipStream_t s, int pre, BicgBuffers &B, double *part,
                    KState *st)
 {
-    const dim3 gb(grid_blocks(g)), tb(kThreads);
+    MarchIO io = {};
+    io.r = B.r;
+    io.ap = B.ap[B.cur];
+    io.out_as = B.as;
+    io.part = part;
     if (pre)
-        hipLaunchKernelGGL(k_bicg_B<1>, gb, tb, 0, s, g, B.r, B.ap[B.cur], B.as, part, st);
+        hipLaunchKernelGGL((k_march<1, 1>), dim3(march_blocks(g)), dim3(kThreads), march_lds(g),
+                           s, g, io, st);
     else
-        hipLaunchKernelGGL(k_bicg_B<0>, gb, tb, 0, s, g, B.r, B.ap[B.cur], B.as, part, st);
+        hipLaunchKernelGGL((k_march<1, 0>), dim3(march_blocks(g)), dim3(kThreads), march_lds(g),
+                           s, g, io, st);
+}
+
+void launch_bicg_fin(const Geo &g, hipStream_t s, double *part, KState *st, int phase)
+{
+    const int nb = (phase == 1 || phase == 2) ? g.nmarch : g.nblocks;
+    hipLaunchKernelGGL(k_bicg_fin, dim3(1), dim3(kThreads), 0, s, part, nb, st, phase);
 }
 
 void launch_bicg_C(const Geo &g, hipStream_t s, BicgBuffers &B, double *x, double *part,

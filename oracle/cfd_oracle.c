@@ -175,6 +175,46 @@ double orc_blocked_sum(const double *v, long n)
     return r;
 }
 
+/* march order of the device's plane-marching stencil kernels: block b = chunk*ntile +
+ * tile covers planes [8*chunk, 8*chunk+8) of the plane-index tile [1024*tile,
+ * 1024*tile+1024); thread t accumulates plane by plane the cells q = 1024*tile +
+ * 256c + t, c = 0..3; then the same block fold and partial fold as orc_blocked_sum. */
+double orc_march_sum(const orc_grid *g, const double *v)
+{
+    const long cyz = (long)g->nyc * g->nzc;
+    const int ntile = (int)((cyz + ORC_MARCH_TILE - 1) / ORC_MARCH_TILE);
+    const int nchunk = (g->nxc + ORC_MARCH_CHUNK - 1) / ORC_MARCH_CHUNK;
+    const long nb = (long)ntile * nchunk;
+    double *part = (double *)malloc(sizeof(double) * nb);
+    double acc[ORC_RB_THREADS];
+    for (long b = 0; b < nb; ++b) {
+        const int tile = (int)(b % ntile), chunk = (int)(b / ntile);
+        const int i0 = chunk * ORC_MARCH_CHUNK;
+        const int i1 = i0 + ORC_MARCH_CHUNK < g->nxc ? i0 + ORC_MARCH_CHUNK : g->nxc;
+        const long q0 = (long)tile * ORC_MARCH_TILE;
+        for (int t = 0; t < ORC_RB_THREADS; ++t) {
+            double s = 0.0;
+            for (int i = i0; i < i1; ++i)
+                for (int c = 0; c < ORC_RB_CELLS; ++c) {
+                    const long q = q0 + (long)c * ORC_RB_THREADS + t;
+                    if (q < cyz)
+                        s += v[(long)i * cyz + q];
+                }
+            acc[t] = s;
+        }
+        part[b] = tree256(acc);
+    }
+    for (int t = 0; t < ORC_RB_THREADS; ++t) {
+        double s = 0.0;
+        for (long m = t; m < nb; m += ORC_RB_THREADS)
+            s += part[m];
+        acc[t] = s;
+    }
+    const double r = tree256(acc);
+    free(part);
+    return r;
+}
+
 static double reduce(int mode, const double *v, long n)
 {
     return mode == ORC_RED_SERIAL ? orc_serial_sum(v, n) : orc_blocked_sum(v, n);
@@ -191,6 +231,17 @@ static double dot_mode(int mode, const double *a, const double *b, long n, doubl
     for (long i = 0; i < n; ++i)
         tmp[i] = a[i] * b[i];
     return orc_blocked_sum(tmp, n);
+}
+
+/* dot product in the march order (device) or index order (reference) */
+static double dot_march(int mode, const orc_grid *g, const double *a, const double *b, long n,
+                        double *tmp)
+{
+    if (mode == ORC_RED_SERIAL)
+        return dot_mode(mode, a, b, n, tmp);
+    for (long i = 0; i < n; ++i)
+        tmp[i] = a[i] * b[i];
+    return orc_march_sum(g, tmp);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -764,11 +815,12 @@ int orc_bicg_solve(orc_bicg *S, const orc_grid *g, int precond, int zp, int mode
     int pIter = 0;
     for (;;) {
         orc_apply_A(g, precond, zp, S->p1, S->ap);
-        a = r1r0 / dot_mode(mode, S->ap, S->r0, n, S->tmp);
+        a = r1r0 / dot_march(mode, g, S->ap, S->r0, n, S->tmp);
         for (long i = 0; i < n; ++i)
             S->s1[i] = S->r1[i] - a * S->ap[i];
         orc_apply_A(g, precond, zp, S->s1, S->as);
-        w = dot_mode(mode, S->as, S->s1, n, S->tmp) / dot_mode(mode, S->as, S->as, n, S->tmp);
+        w = dot_march(mode, g, S->as, S->s1, n, S->tmp) /
+            dot_march(mode, g, S->as, S->as, n, S->tmp);
         for (long i = 0; i < n; ++i) {
             x[i] += a * S->p1[i] + w * S->s1[i];
             S->r1[i] = S->s1[i] - w * S->as[i];

@@ -14,7 +14,9 @@
  * Reductions come in two orders:
  *   ORC_RED_SERIAL  -- index order, exactly math::dot / L2Norm at one OpenMP
  *                      thread (math.hpp:13-36) and virtualF_Int (virtualForceIntergrator.hpp:79-86);
- *   ORC_RED_BLOCKED -- the device's canonical order (blocks of 256 threads x 4 cells,
+ *   ORC_RED_BLOCKED -- the device's canonical orders (blocks of 256 threads x 4 cells
+ *                      over the linear cell index, or the plane-marching order of the
+ *                      stencil kernels for the BiCGSTAB (Ap,r0), (As,s), (As,As) dots;
  *                      per-wave tree, fixed two-level combine), so that the HIP
  *                      library can be checked bit-for-bit.
  */
@@ -38,9 +40,11 @@ extern "C" {
 #define ORC_PRECOND_NONE 0   /* SPE 7-slot operator (SPE_sparseMatrix.hpp:42-106) */
 #define ORC_PRECOND_JACOBI 1 /* row-scaled CSR operator (structureGrid.hpp:557-654) */
 
-/* canonical device reduction shape */
+/* canonical device reduction shapes */
 #define ORC_RB_THREADS 256
 #define ORC_RB_CELLS 4
+#define ORC_MARCH_TILE 1024 /* plane cells per tile of the marching stencil kernels */
+#define ORC_MARCH_CHUNK 8   /* planes per marching block */
 
 typedef struct orc_grid {
     int nxc, nyc, nzc; /* interior cells (grid::nxCal, nyCal, nzCal) */
@@ -77,6 +81,7 @@ int orc_grid_build(orc_grid *g, int kind);
 
 /* ---- reductions ---- */
 double orc_blocked_sum(const double *v, long n);
+double orc_march_sum(const orc_grid *g, const double *v);
 double orc_serial_sum(const double *v, long n);
 
 /* ---- stages ---- */

@@ -110,6 +110,8 @@ class Oracle:
         L.orc_grid_build.argtypes = [C.POINTER(OrcGrid), C.c_int]
         L.orc_blocked_sum.restype = C.c_double
         L.orc_blocked_sum.argtypes = [DP, C.c_long]
+        L.orc_march_sum.restype = C.c_double
+        L.orc_march_sum.argtypes = [C.POINTER(OrcGrid), DP]
         L.orc_serial_sum.restype = C.c_double
         L.orc_serial_sum.argtypes = [DP, C.c_long]
         G = C.POINTER(OrcGrid)
@@ -181,6 +183,11 @@ class Oracle:
     def blocked_sum(self, v):
         v = np.ascontiguousarray(v, dtype=np.float64)
         return self.lib.orc_blocked_sum(ptr(v), v.size)
+
+    def march_sum(self, g, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        assert v.size == g.N
+        return self.lib.orc_march_sum(C.byref(g.c), ptr(v))
 
     def serial_sum(self, v):
         v = np.ascontiguousarray(v, dtype=np.float64)

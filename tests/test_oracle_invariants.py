@@ -58,6 +58,50 @@ def test_blocked_sum_order_brute_force(o):
     assert o.blocked_sum(v) == fold(acc)
 
 
+def test_march_sum_order_brute_force(o):
+    """March order of the plane-marching kernels, written out independently: block
+    b = chunk*ntile + tile, thread t sums plane by plane (8 planes per chunk) the
+    plane cells q = 1024*tile + 256c + t; block and partial folds as above."""
+    g = o.grid(20, 13, 7)  # cyz = 91 < 1024: one tile; 3 chunks (8, 8, 4 planes)
+    g2 = o.grid(9, 40, 60)  # cyz = 2400: three tiles, the last partial; 2 chunks
+    rng = np.random.default_rng(2)
+
+    def fold(a):
+        a = list(a)
+        for w in range(4):
+            s = 32
+            while s >= 1:
+                for l in range(s):
+                    a[64 * w + l] = a[64 * w + l] + a[64 * w + l + s]
+                s //= 2
+        return (a[0] + a[64]) + (a[128] + a[192])
+
+    for gg in (g, g2):
+        v = rng.standard_normal(gg.N) * 10.0 ** rng.integers(-6, 6, gg.N)
+        cyz = gg.nyc * gg.nzc
+        ntile, nchunk = -(-cyz // 1024), -(-gg.nxc // 8)
+        parts = []
+        for b in range(ntile * nchunk):
+            tile, chunk = b % ntile, b // ntile
+            acc = []
+            for t in range(256):
+                s = 0.0
+                for i in range(chunk * 8, min(chunk * 8 + 8, gg.nxc)):
+                    for c in range(4):
+                        q = tile * 1024 + c * 256 + t
+                        if q < cyz:
+                            s += v[i * cyz + q]
+                acc.append(s)
+            parts.append(fold(acc))
+        acc = []
+        for t in range(256):
+            s = 0.0
+            for m in range(t, len(parts), 256):
+                s += parts[m]
+            acc.append(s)
+        assert o.march_sum(gg, v) == fold(acc)
+
+
 # ---------------------------------------------------------------- grid
 def test_uniform_grid_closed_form(o):
     g = o.grid(40, 20, 8)
