@@ -25,13 +25,14 @@ EXPORTS = [
     "cfdmx_config_default", "cfdmx_create", "cfdmx_destroy", "cfdmx_last_error", "cfdmx_sizes",
     "cfdmx_grid_metrics", "cfdmx_upload_velocity", "cfdmx_download_velocity",
     "cfdmx_upload_pressure", "cfdmx_download_pressure", "cfdmx_upload_xresult",
-    "cfdmx_download_xresult", "cfdmx_download_rhs", "cfdmx_upload_eta", "cfdmx_download_eta",
+    "cfdmx_download_xresult", "cfdmx_download_rhs", "cfdmx_upload_rhs", "cfdmx_upload_eta",
+    "cfdmx_download_eta",
     "cfdmx_download_force", "cfdmx_set_inflow", "cfdmx_init_flow", "cfdmx_start",
     "cfdmx_set_eta_cylinder_z", "cfdmx_set_eta_smoothed", "cfdmx_convdiff", "cfdmx_bc_main",
     "cfdmx_bc_copy", "cfdmx_pressure_rhs", "cfdmx_solve_bicgstab", "cfdmx_solve_rbsor",
     "cfdmx_transform", "cfdmx_project_ib", "cfdmx_swap_t0_t3", "cfdmx_cd_cl", "cfdmx_cfl",
     "cfdmx_apply_operator", "cfdmx_step", "cfdmx_synchronize", "cfdmx_profile",
-    "cfdmx_profile_read",
+    "cfdmx_profile_read", "cfdmx_set_tolerance", "cfdmx_set_physics", "cfdmx_upload_force",
 ]
 
 # kernel classes of cfdmx_profile_read
@@ -87,7 +88,8 @@ def load_library(path=LIB_PATH):
     L.cfdmx_upload_velocity.argtypes = [vp, C.c_int, DP, DP, DP]
     L.cfdmx_download_velocity.argtypes = [vp, C.c_int, DP, DP, DP]
     for n in ("cfdmx_upload_pressure", "cfdmx_download_pressure", "cfdmx_upload_xresult",
-              "cfdmx_download_xresult", "cfdmx_download_rhs", "cfdmx_upload_eta",
+              "cfdmx_download_xresult", "cfdmx_download_rhs", "cfdmx_upload_rhs",
+              "cfdmx_upload_eta", "cfdmx_upload_force",
               "cfdmx_download_eta", "cfdmx_download_force", "cfdmx_set_inflow"):
         getattr(L, n).argtypes = [vp, DP]
     L.cfdmx_init_flow.argtypes = [vp, C.c_double, C.c_double, C.c_ulonglong]
@@ -104,6 +106,8 @@ def load_library(path=LIB_PATH):
     L.cfdmx_apply_operator.argtypes = [vp, DP, DP]
     L.cfdmx_step.argtypes = [vp, C.c_int, C.POINTER(StepInfo)]
     L.cfdmx_profile.argtypes = [vp, C.c_int]
+    L.cfdmx_set_tolerance.argtypes = [vp, C.c_double, C.c_int]
+    L.cfdmx_set_physics.argtypes = [vp, C.c_double, C.c_double]
     L.cfdmx_profile_read.argtypes = [vp, C.c_int, DP, C.POINTER(C.c_longlong)]
     for n in EXPORTS:
         if n != "cfdmx_last_error":
@@ -213,6 +217,9 @@ class FlowSolver:
     def rhs(self):
         return self._get("cfdmx_download_rhs", self.N)
 
+    def set_rhs(self, b):
+        self._put("cfdmx_upload_rhs", b)
+
     def eta(self):
         return self._get("cfdmx_download_eta", self.G)
 
@@ -296,6 +303,15 @@ class FlowSolver:
 
     def synchronize(self):
         self._chk(self.L.cfdmx_synchronize(self.h), "synchronize")
+
+    def set_force(self, f):
+        self._put("cfdmx_upload_force", f)
+
+    def set_physics(self, re, dt):
+        self._chk(self.L.cfdmx_set_physics(self.h, re, dt), "set_physics")
+
+    def set_tolerance(self, tol, maxit):
+        self._chk(self.L.cfdmx_set_tolerance(self.h, tol, maxit), "set_tolerance")
 
     def profile(self, enable):
         self._chk(self.L.cfdmx_profile(self.h, 1 if enable else 0), "profile")

@@ -585,6 +585,10 @@ int cfdmx_download_rhs(cfdmx_ctx *ctx, double *b)
 {
     return ctx ? copy_d2h(ctx, b, ctx->b, ctx->g.N) : CFDMX_ERR_ARG;
 }
+int cfdmx_upload_rhs(cfdmx_ctx *ctx, const double *b)
+{
+    return ctx ? copy_h2d(ctx, ctx->b, b, ctx->g.N) : CFDMX_ERR_ARG;
+}
 int cfdmx_upload_eta(cfdmx_ctx *ctx, const double *eta)
 {
     return ctx ? copy_h2d(ctx, ctx->eta, eta, ctx->g.G) : CFDMX_ERR_ARG;
@@ -596,6 +600,11 @@ int cfdmx_download_eta(cfdmx_ctx *ctx, double *eta)
 int cfdmx_download_force(cfdmx_ctx *ctx, double *f)
 {
     return ctx ? copy_d2h(ctx, f, ctx->f, 3 * (size_t)ctx->g.G) : CFDMX_ERR_ARG;
+}
+
+int cfdmx_upload_force(cfdmx_ctx *ctx, const double *f)
+{
+    return ctx ? copy_h2d(ctx, ctx->f, f, 3 * (size_t)ctx->g.G) : CFDMX_ERR_ARG;
 }
 
 int cfdmx_set_inflow(cfdmx_ctx *ctx, const double *profile)
@@ -679,6 +688,30 @@ int cfdmx_set_eta_smoothed(cfdmx_ctx *ctx, double cx, double cy, double R, doubl
             for (int k = 1; k < g.nz - 1; ++k)
                 eta[((size_t)i * g.ny + j) * g.nz + k] = eta[((size_t)i * g.ny + j) * g.nz];
     return copy_h2d(ctx, ctx->eta, eta.data(), g.G);
+}
+
+int cfdmx_set_physics(cfdmx_ctx *ctx, double re, double dt)
+{
+    if (!ctx) return CFDMX_ERR_ARG;
+    if (!(dt > 0.0) || !(re > 0.0)) FAIL(CFDMX_ERR_ARG, "dt and re must be positive");
+    ctx->cfg.re = re;
+    ctx->cfg.dt = dt;
+    ctx->bc.conv_coef = ctx->cfg.convective_speed * dt / ctx->Dx[ctx->g.nx - 3];
+    return CFDMX_OK;
+}
+
+int cfdmx_set_tolerance(cfdmx_ctx *ctx, double tol, int maxit)
+{
+    if (!ctx) return CFDMX_ERR_ARG;
+    if (!(tol > 0.0) || maxit < 0) FAIL(CFDMX_ERR_ARG, "tol must be positive and maxit >= 0");
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(ctx->kst_h, ctx->kst, sizeof(KState), hipMemcpyDeviceToHost));
+    ctx->kst_h->tol = tol;
+    ctx->kst_h->maxit = maxit;
+    HIPCHK(hipMemcpy(ctx->kst, ctx->kst_h, sizeof(KState), hipMemcpyHostToDevice));
+    ctx->cfg.tol = tol;
+    ctx->cfg.maxit = maxit;
+    return CFDMX_OK;
 }
 
 int cfdmx_convdiff(cfdmx_ctx *ctx)

@@ -111,9 +111,11 @@ int cfdmx_download_pressure(cfdmx_ctx *ctx, double *p);
 int cfdmx_upload_xresult(cfdmx_ctx *ctx, const double *x);
 int cfdmx_download_xresult(cfdmx_ctx *ctx, double *x);
 int cfdmx_download_rhs(cfdmx_ctx *ctx, double *b);
+int cfdmx_upload_rhs(cfdmx_ctx *ctx, const double *b); /* MxClass::matB, N doubles */
 int cfdmx_upload_eta(cfdmx_ctx *ctx, const double *eta);
 int cfdmx_download_eta(cfdmx_ctx *ctx, double *eta);
 int cfdmx_download_force(cfdmx_ctx *ctx, double *f); /* DfibArray::f, 3*G */
+int cfdmx_upload_force(cfdmx_ctx *ctx, const double *f);
 int cfdmx_set_inflow(cfdmx_ctx *ctx, const double *profile); /* u inflow per ghosted j, ny */
 
 /* Seeded initial field on the device: u = u0 + amp*(2U-1) on interior cells with U
@@ -129,6 +131,13 @@ int cfdmx_start(cfdmx_ctx *ctx);
 int cfdmx_set_eta_cylinder_z(cfdmx_ctx *ctx, double cx, double cy, double radius, int nsub);
 /* Smoothed mask of half-width eps for differentiation (no reference counterpart). */
 int cfdmx_set_eta_smoothed(cfdmx_ctx *ctx, double cx, double cy, double radius, double eps);
+
+/* simpulationVariable::set_Re / set_dt (0_1_SimulationVariable.hpp:26-31,95-106) after
+ * creation */
+int cfdmx_set_physics(cfdmx_ctx *ctx, double re, double dt);
+/* bicgstabRe2::setTolerance (bicgstabRe2.hpp:38) and the iteration limit
+ * (iters_max_, :47); also the SOR tolerance/limit (simu.p_criteria, p_sor_iter_max) */
+int cfdmx_set_tolerance(cfdmx_ctx *ctx, double tol, int maxit);
 
 /* ---- stages of one time step (RUN_CPU.cpp:257-436) ---- */
 /* ConvectionDifussion, Convection_Difussion.hpp:174-450: T0 -> T1, QUICK, explicit Euler */
