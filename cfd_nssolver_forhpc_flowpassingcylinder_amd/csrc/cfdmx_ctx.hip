@@ -27,6 +27,7 @@ struct cfdmx_ctx {
     double *x = nullptr, *b = nullptr, *px1 = nullptr, *px2 = nullptr;
     BicgBuffers B = {};
     double *part = nullptr, *cfl_part = nullptr, *flux_part = nullptr;
+    double *bpart[3] = {}; // per-phase BiCGSTAB partials (A, B, C)
     double *scal = nullptr; // [0] cfl, [1] cd, [2] cl
     double *inflow = nullptr;
     KState *kst = nullptr;
@@ -332,6 +333,7 @@ static int setup_device(cfdmx_ctx *ctx)
     ALLOC(ctx->B.ap[1], N);
     ALLOC(ctx->part, 4 * (size_t)(g.nblocks > g.nmarch ? g.nblocks : g.nmarch) + 64);
     ALLOC(ctx->cfl_part, (size_t)g.nblocks + 8);
+    for (int q = 0; q < 3; ++q) ALLOC(ctx->bpart[q], 2 * (size_t)g.nmarch + 64);
     ALLOC(ctx->flux_part, 3 * ((size_t)g.cyz / kPerBlock + 2));
     ALLOC(ctx->scal, 16);
     ALLOC(ctx->inflow, (size_t)g.ny);
@@ -760,18 +762,16 @@ static int solve_bicgstab_impl(cfdmx_ctx *ctx, int *iters, double *res)
     std::swap(ctx->px1, ctx->px2); // px2 <- px1, then px1 <- x (bicgstabRe2.hpp:94-95)
     PROF(CFDMX_K_BICG_INIT, launch_bicg_init(g, s, pre, ctx->b, ctx->x, ctx->B.r0, ctx->B.r,
                                              ctx->px1, ctx->part, ctx->kst));
-    PROF(CFDMX_K_BICG_FIN, launch_bicg_fin(g, s, ctx->part, ctx->kst, 0));
+    PROF(CFDMX_K_BICG_FIN, launch_bicg_fin0(g, s, ctx->part, ctx->kst));
     LAUNCHCHK();
-    int batch = 4;
+    int batch = 4, m = 0;
     for (;;) {
-        for (int q = 0; q < batch; ++q) {
-            PROF(CFDMX_K_BICG_A, launch_bicg_A(g, s, pre, ctx->B, ctx->part, ctx->kst));
-            PROF(CFDMX_K_BICG_FIN, launch_bicg_fin(g, s, ctx->part, ctx->kst, 1));
-            PROF(CFDMX_K_BICG_B, launch_bicg_B(g, s, pre, ctx->B, ctx->part, ctx->kst));
-            PROF(CFDMX_K_BICG_FIN, launch_bicg_fin(g, s, ctx->part, ctx->kst, 2));
-            PROF(CFDMX_K_BICG_C, launch_bicg_C(g, s, ctx->B, ctx->x, ctx->part, ctx->kst));
-            PROF(CFDMX_K_BICG_FIN, launch_bicg_fin(g, s, ctx->part, ctx->kst, 3));
+        for (int q = 0; q < batch; ++q, ++m) {
+            PROF(CFDMX_K_BICG_A, launch_bicg_A(g, s, pre, ctx->B, ctx->bpart, ctx->kst, m));
+            PROF(CFDMX_K_BICG_B, launch_bicg_B(g, s, pre, ctx->B, ctx->bpart, ctx->kst, m));
+            PROF(CFDMX_K_BICG_C, launch_bicg_C(g, s, ctx->B, ctx->x, ctx->bpart, ctx->kst, m));
         }
+        PROF(CFDMX_K_BICG_FIN, launch_bicg_check(g, s, ctx->bpart, ctx->kst, m - 1));
         LAUNCHCHK();
         HIPCHK(hipMemcpyAsync(ctx->kst_h, ctx->kst, sizeof(KState), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));

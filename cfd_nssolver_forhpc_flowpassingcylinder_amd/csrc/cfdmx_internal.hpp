@@ -16,7 +16,7 @@ struct BCParams {
 
 // BiCGSTAB scalars kept on the device (bicgstabRe2.hpp:89-176)
 struct KState {
-    double rho;   // r1r0
+    double rho[2]; // r1r0 of iteration m in slot m & 1
     double alpha; // a
     double omega; // w
     double beta;  // b
@@ -60,13 +60,16 @@ void launch_rhs(const Geo &g, hipStream_t s, int pre, const double *us, const do
 void launch_apply(const Geo &g, hipStream_t s, int pre, const double *x, double *r);
 void launch_bicg_init(const Geo &g, hipStream_t s, int pre, const double *b, const double *x,
                       double *r0, double *r, double *px1, double *part, KState *st);
-void launch_bicg_A(const Geo &g, hipStream_t s, int pre, BicgBuffers &B, double *part,
-                   KState *st);
-void launch_bicg_B(const Geo &g, hipStream_t s, int pre, BicgBuffers &B, double *part,
-                   KState *st);
-void launch_bicg_fin(const Geo &g, hipStream_t s, double *part, KState *st, int phase);
-void launch_bicg_C(const Geo &g, hipStream_t s, BicgBuffers &B, double *x, double *part,
-                   KState *st);
+// one BiCGSTAB iteration m = three kernels; each folds the previous phase's partials
+// (part[0] <- A, part[1] <- B, part[2] <- C)
+void launch_bicg_A(const Geo &g, hipStream_t s, int pre, BicgBuffers &B, double *const part[3],
+                   KState *st, int m);
+void launch_bicg_B(const Geo &g, hipStream_t s, int pre, BicgBuffers &B, double *const part[3],
+                   KState *st, int m);
+void launch_bicg_C(const Geo &g, hipStream_t s, BicgBuffers &B, double *x,
+                   double *const part[3], KState *st, int m);
+void launch_bicg_check(const Geo &g, hipStream_t s, double *const part[3], KState *st, int m);
+void launch_bicg_fin0(const Geo &g, hipStream_t s, double *part, KState *st);
 void launch_nanfix(hipStream_t s, int n, double *x, const double *px1, int *count);
 void launch_copy(hipStream_t s, int n, const double *a, double *b);
 void launch_fill(hipStream_t s, int n, double val, double *a);

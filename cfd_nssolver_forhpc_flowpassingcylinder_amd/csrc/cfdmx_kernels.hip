@@ -594,7 +594,9 @@ k_rhs(Geo g, const double *__restrict__ us, const double *__restrict__ vs,
 //   B: s = r - alpha*Ap, As = A s, partials (As, s), (As, As)       -> omega
 //   C: x += alpha*p + omega*s, r = s - omega*As, partials (r,r), (r,r0)
 //      -> convergence test, beta, restart decision
-// One-block kernels (k_bicg_fin) fold the partials and update the scalars.
+// Each phase starts by folding the previous phase's per-block partials (the same
+// fixed order in every block, so every block holds identical scalars); block 0 stores
+// them in KState for the later phases.  k_bicg_check ends a batch of iterations.
 // In A and B the new vector (p or s) of the block's own 1024 cells is formed once
 // into LDS; stencil neighbours outside that range are formed from global memory
 // with the same expression, so every value is bit-identical wherever it is formed.
@@ -685,9 +687,47 @@ __device__ __forceinline__ void march_row(const Geo &g, int i, int j, int k, dou
     }
 }
 
+// Start of iteration m in phase A: the end of iteration m-1 (bicgstabRe2.hpp:129-174)
+// from phase C's partials (r,r), (r,r0).  Returns false when the solve has ended.
+__device__ __forceinline__ bool bicg_iteration_start(const Geo &g, KState *st, int m,
+                                                     const double *partC, bool writer,
+                                                     int &mode, double &beta)
+{
+    if (m == 0) {
+        mode = 2; // p = r0 (bicgstabRe2.hpp:101-102)
+        beta = 0.0;
+        return true;
+    }
+    double t[2];
+    fold_partials<2>(partC, g.nmarch, g.nmarch, t);
+    const double res = sqrt(t[0]) / st->norm0; // L2Norm(r1) / norm0
+    const int it = m - 1;                      // pIter of the iteration that just ended
+    if (res < st->tol || it >= st->maxit) {
+        if (writer) {
+            st->res = res;
+            st->iter = it;
+            st->done = res < st->tol ? 1 : 2;
+        }
+        return false;
+    }
+    const double rho_l = st->rho[(m - 1) & 1];
+    const double rho = t[1];
+    beta = (st->alpha / st->omega) * (rho / rho_l);
+    mode = (st->timestep < 10 || fabs(rho_l) > st->tol * st->restart_factor) ? 0 : 1;
+    if (writer) {
+        st->rho[m & 1] = rho;
+        st->res = res;
+        st->iter = m;
+        st->beta = beta;
+        st->mode = mode;
+        if (mode == 1) st->restarts += 1;
+    }
+    return true;
+}
+
 template <int PHASE, int PRE>
 __global__ void __launch_bounds__(kThreads)
-k_march(Geo g, MarchIO io, KState *st)
+k_march(Geo g, MarchIO io, KState *st, int m, const double *part_in)
 {
     if (st->done) return;
     int b = blockIdx.x;
@@ -698,8 +738,18 @@ k_march(Geo g, MarchIO io, KState *st)
     const int i0 = chunk * kChunk, i1 = min(i0 + kChunk, g.nxc);
     const int q0 = tile * kTile, qn = min(kTile, g.cyz - q0);
     const int H = g.nzc, W = qn + 2 * H, WS = kTile + 2 * H;
-    const int mode = st->mode;
-    const double beta = st->beta, omega = st->omega, alpha = st->alpha;
+    const bool writer = (b == 0 && threadIdx.x == 0);
+    int mode = 0;
+    double beta = 0.0, alpha = 0.0;
+    const double omega = st->omega;
+    if (PHASE == 0) {
+        if (!bicg_iteration_start(g, st, m, part_in, writer, mode, beta)) return;
+    } else {
+        double t[1];
+        fold_partials<1>(part_in, g.nmarch, g.nmarch, t);
+        alpha = st->rho[m & 1] / t[0]; // a = r1r0 / dot(ap, r0)
+        if (writer) st->alpha = alpha;
+    }
     auto f = [&](int n) -> double {
         if (PHASE == 0) return mode == 0 ? io.r[n] + beta * (io.p[n] - omega * io.ap[n]) : io.r[n];
         return io.r[n] - alpha * io.ap[n];
@@ -780,79 +830,77 @@ k_march(Geo g, MarchIO io, KState *st)
     }
 }
 
+// phase C in the march mapping (elementwise)
 __global__ void __launch_bounds__(kThreads)
-k_bicg_C(Geo g, double *__restrict__ r, const double *__restrict__ ap,
-         const double *__restrict__ p, const double *__restrict__ as,
-         const double *__restrict__ r0, double *__restrict__ x, double *__restrict__ part,
-         KState *st)
+k_march_C(Geo g, double *__restrict__ r, const double *__restrict__ ap,
+          const double *__restrict__ p, const double *__restrict__ as,
+          const double *__restrict__ r0, double *__restrict__ x, const double *part_in,
+          double *__restrict__ part_out, KState *st)
 {
     if (st->done) return;
-    const int L = logical_block(g);
-    if (L >= g.nblocks) return;
-    const double alpha = st->alpha, omega = st->omega;
+    int b = blockIdx.x;
+    if (g.xcd_remap) b = (b & 7) * ((gridDim.x + 7) >> 3) + (b >> 3);
+    if (b >= g.nmarch) return;
+    double t[2];
+    fold_partials<2>(part_in, g.nmarch, g.nmarch, t);
+    const double omega = t[0] / t[1]; // w = dot(as, s) / dot(as, as)
+    if (b == 0 && threadIdx.x == 0) st->omega = omega;
+    const double alpha = st->alpha;
+    const int tile = b % g.ntile, chunk = b / g.ntile;
+    const int i0 = chunk * kChunk, i1 = min(i0 + kChunk, g.nxc);
+    const int q0 = tile * kTile, qn = min(kTile, g.cyz - q0);
     double acc[2] = {0.0, 0.0};
+    for (int i = i0; i < i1; ++i) {
 #pragma unroll
-    for (int c = 0; c < kCells; ++c) {
-        const int n = L * kPerBlock + c * kThreads + threadIdx.x;
-        if (n < g.N) {
-            const double sn = r[n] - alpha * ap[n];
-            x[n] += alpha * p[n] + omega * sn;
-            const double rn = sn - omega * as[n];
-            r[n] = rn;
-            acc[0] += rn * rn;
-            acc[1] += rn * r0[n];
-        }
-    }
-    block_sums<2>(acc, part, g.nblocks, L);
-}
-
-// scalar updates between the phases (one block)
-__global__ void __launch_bounds__(kThreads)
-k_bicg_fin(const double *part, int nb, KState *st, int phase)
-{
-    if (phase != 0 && st->done) return;
-    if (phase == 0) {
-        double t[2];
-        fold_partials<2>(part, nb, nb, t);
-        if (threadIdx.x == 0) {
-            st->timestep += 1;
-            st->norm0 = sqrt(t[0]); // L2Norm(rhs)
-            st->rho = t[1];         // dot(r1, r0)
-            st->iter = 0;
-            st->done = 0;
-            st->mode = 2;
-        }
-    } else if (phase == 1) {
-        double t[1];
-        fold_partials<1>(part, nb, nb, t);
-        if (threadIdx.x == 0) st->alpha = st->rho / t[0]; // a = r1r0 / dot(ap, r0)
-    } else if (phase == 2) {
-        double t[2];
-        fold_partials<2>(part, nb, nb, t);
-        if (threadIdx.x == 0) st->omega = t[0] / t[1]; // w = dot(as, s) / dot(as, as)
-    } else {
-        double t[2];
-        fold_partials<2>(part, nb, nb, t);
-        if (threadIdx.x == 0) {
-            const double res = sqrt(t[0]) / st->norm0; // L2Norm(r1) / norm0
-            st->res = res;
-            if (res < st->tol) {
-                st->done = 1;
-            } else if (st->iter >= st->maxit) {
-                st->done = 2;
-            } else {
-                const double rho_l = st->rho;
-                st->rho = t[1];
-                st->beta = (st->alpha / st->omega) * (st->rho / rho_l);
-                if (st->timestep < 10 || fabs(rho_l) > st->tol * st->restart_factor) {
-                    st->mode = 0;
-                } else {
-                    st->mode = 1;
-                    st->restarts += 1;
-                }
-                st->iter += 1;
+        for (int c = 0; c < kCells; ++c) {
+            const int tq = c * kThreads + threadIdx.x;
+            if (tq < qn) {
+                const int n = i * g.cyz + q0 + tq;
+                const double sn = r[n] - alpha * ap[n];
+                x[n] += alpha * p[n] + omega * sn;
+                const double rn = sn - omega * as[n];
+                r[n] = rn;
+                acc[0] += rn * rn;
+                acc[1] += rn * r0[n];
             }
         }
+    }
+    block_sums<2>(acc, part_out, g.nmarch, b);
+}
+
+// end of a batch: the convergence test of the last iteration m (bicgstabRe2.hpp:129-153)
+__global__ void __launch_bounds__(kThreads)
+k_bicg_check(Geo g, const double *partC, KState *st, int m)
+{
+    if (st->done) return;
+    double t[2];
+    fold_partials<2>(partC, g.nmarch, g.nmarch, t);
+    if (threadIdx.x == 0) {
+        const double res = sqrt(t[0]) / st->norm0;
+        st->res = res;
+        if (res < st->tol) {
+            st->done = 1;
+            st->iter = m;
+        } else if (m >= st->maxit) {
+            st->done = 2;
+            st->iter = m;
+        }
+    }
+}
+
+// scalars after the initial residual (bicgstabRe2.hpp:88-105)
+__global__ void __launch_bounds__(kThreads)
+k_bicg_fin0(const double *part, int nb, KState *st)
+{
+    double t[2];
+    fold_partials<2>(part, nb, nb, t);
+    if (threadIdx.x == 0) {
+        st->timestep += 1;
+        st->norm0 = sqrt(t[0]); // L2Norm(rhs)
+        st->rho[0] = t[1];      // dot(r1, r0)
+        st->iter = 0;
+        st->done = 0;
+        st->mode = 2;
     }
 }
 
@@ -1261,8 +1309,8 @@ static inline size_t march_lds(const Geo &g)
     return (size_t)3 * (kTile + 2 * g.nzc) * sizeof(double);
 }
 
-void launch_bicg_A(const Geo &g, hipStream_t s, int pre, BicgBuffers &B, double *part,
-                   KState *st)
+void launch_bicg_A(const Geo &g, hipStream_t s, int pre, BicgBuffers &B, double *const part[3],
+                   KState *st, int m)
 {
     MarchIO io = {};
     io.r = B.r;
@@ -1271,45 +1319,51 @@ void launch_bicg_A(const Geo &g, hipStream_t s, int pre, BicgBuffers &B, double 
     io.r0 = B.r0;
     io.out_p = B.p[1 - B.cur];
     io.out_ap = B.ap[1 - B.cur];
-    io.part = part;
+    io.part = part[0];
     if (pre)
         hipLaunchKernelGGL((k_march<0, 1>), dim3(march_blocks(g)), dim3(kThreads), march_lds(g),
-                           s, g, io, st);
+                           s, g, io, st, m, (const double *)part[2]);
     else
         hipLaunchKernelGGL((k_march<0, 0>), dim3(march_blocks(g)), dim3(kThreads), march_lds(g),
-                           s, g, io, st);
+                           s, g, io, st, m, (const double *)part[2]);
     B.cur = 1 - B.cur;
 }
 
-void launch_bicg_B(const Geo &g, hipStream_t s, int pre, BicgBuffers &B, double *part,
-                   KState *st)
+void launch_bicg_B(const Geo &g, hipStream_t s, int pre, BicgBuffers &B, double *const part[3],
+                   KState *st, int m)
 {
     MarchIO io = {};
     io.r = B.r;
     io.ap = B.ap[B.cur];
     io.out_as = B.as;
-    io.part = part;
+    io.part = part[1];
     if (pre)
         hipLaunchKernelGGL((k_march<1, 1>), dim3(march_blocks(g)), dim3(kThreads), march_lds(g),
-                           s, g, io, st);
+                           s, g, io, st, m, (const double *)part[0]);
     else
         hipLaunchKernelGGL((k_march<1, 0>), dim3(march_blocks(g)), dim3(kThreads), march_lds(g),
-                           s, g, io, st);
+                           s, g, io, st, m, (const double *)part[0]);
 }
 
-void launch_bicg_fin(const Geo &g, hipStream_t s, double *part, KState *st, int phase)
+void launch_bicg_C(const Geo &g, hipStream_t s, BicgBuffers &B, double *x,
+                   double *const part[3], KState *st, int m)
 {
-    const int nb = (phase == 1 || phase == 2) ? g.nmarch : g.nblocks;
-    hipLaunchKernelGGL(k_bicg_fin, dim3(1), dim3(kThreads), 0, s, part, nb, st, phase);
+    (void)m;
+    hipLaunchKernelGGL(k_march_C, dim3(march_blocks(g)), dim3(kThreads), 0, s, g, B.r,
+                       B.ap[B.cur], B.p[B.cur], B.as, B.r0, x, (const double *)part[1], part[2],
+                       st);
 }
 
-void launch_bicg_C(const Geo &g, hipStream_t s, BicgBuffers &B, double *x, double *part,
-                   KState *st)
+void launch_bicg_check(const Geo &g, hipStream_t s, double *const part[3], KState *st, int m)
 {
-    hipLaunchKernelGGL(k_bicg_C, dim3(grid_blocks(g)), dim3(kThreads), 0, s, g, B.r,
-                       B.ap[B.cur], B.p[B.cur], B.as, B.r0, x, part, st);
+    hipLaunchKernelGGL(k_bicg_check, dim3(1), dim3(kThreads), 0, s, g, (const double *)part[2],
+                       st, m);
 }
 
+void launch_bicg_fin0(const Geo &g, hipStream_t s, double *part, KState *st)
+{
+    hipLaunchKernelGGL(k_bicg_fin0, dim3(1), dim3(kThreads), 0, s, part, g.nblocks, st);
+}
 
 void launch_nanfix(hipStream_t s, int n, double *x, const double *px1, int *count)
 {

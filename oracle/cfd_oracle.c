@@ -825,7 +825,7 @@ int orc_bicg_solve(orc_bicg *S, const orc_grid *g, int precond, int zp, int mode
             x[i] += a * S->p1[i] + w * S->s1[i];
             S->r1[i] = S->s1[i] - w * S->as[i];
         }
-        if ((l2norm_mode(mode, S->r1, n, S->tmp) / norm0) < tol)
+        if ((sqrt(dot_march(mode, g, S->r1, S->r1, n, S->tmp)) / norm0) < tol)
             break;
         if (pIter >= maxit) {
             /* NaN/Inf reset, bicgstabRe2.hpp:133-153 */
@@ -847,7 +847,7 @@ int orc_bicg_solve(orc_bicg *S, const orc_grid *g, int precond, int zp, int mode
             break;
         }
         r1r0l = r1r0;
-        r1r0 = dot_mode(mode, S->r1, S->r0, n, S->tmp);
+        r1r0 = dot_march(mode, g, S->r1, S->r0, n, S->tmp);
         b = (a / w) * (r1r0 / r1r0l);
         if (S->timestep < 10 || fabs(r1r0l) > tol * restart_factor) {
             for (long i = 0; i < n; ++i)
@@ -863,7 +863,7 @@ int orc_bicg_solve(orc_bicg *S, const orc_grid *g, int precond, int zp, int mode
         ++S->iter_total;
     }
     if (res_out)
-        *res_out = l2norm_mode(mode, S->r1, n, S->tmp) / norm0;
+        *res_out = sqrt(dot_march(mode, g, S->r1, S->r1, n, S->tmp)) / norm0;
     return pIter;
 }
 

@@ -15,8 +15,9 @@
  *   ORC_RED_SERIAL  -- index order, exactly math::dot / L2Norm at one OpenMP
  *                      thread (math.hpp:13-36) and virtualF_Int (virtualForceIntergrator.hpp:79-86);
  *   ORC_RED_BLOCKED -- the device's canonical orders (blocks of 256 threads x 4 cells
- *                      over the linear cell index, or the plane-marching order of the
- *                      stencil kernels for the BiCGSTAB (Ap,r0), (As,s), (As,As) dots;
+ *                      over the linear cell index for the initial residual, or the
+ *                      plane-marching order of the BiCGSTAB phase kernels for the
+ *                      (Ap,r0), (As,s), (As,As), (r,r), (r,r0) sums;
  *                      per-wave tree, fixed two-level combine), so that the HIP
  *                      library can be checked bit-for-bit.
  */
