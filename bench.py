@@ -31,14 +31,22 @@ HBM_PEAK_GBS = 8000.0
 # algorithmic HBM bytes per interior cell and launch (FP64; each array counted once,
 # ghosts and 1-D metric tables excluded)
 BYTES_PER_CELL = {
-    "bicg_A": 48,    # r r, p, Ap, r0;  w p', Ap'
+    "bicg_A": 40,    # r r, q, r0;  w p, Ap
     "bicg_B": 24,    # r r, Ap;         w As
-    "bicg_C": 64,    # r r, Ap, p, As, r0, x;  w x, r
+    "bicg_C": 72,    # r r, Ap, p, As, r0, x;  w x, r, q
     "bicg_init": 48, # r b, x;  w r0, r, px1
     "convdiff": 48,  # r u, v, w;  w u*, v*, w*
     "rhs": 32,       # r u*, v*, w*;  w b
     "project": 96,   # r u*, v*, w*, X, eta;  w u, v, w, p, f(3)
     "sor_half": 24,  # r p (all), rhs (half);  w p (half)
+}
+
+MIXED_INNER_TOL = 1e-3
+
+# the same phases on float Krylov vectors (mixed-precision inner solve); the FP64
+# residual/update passes of the refinement are counted under bicg_init
+BYTES_PER_CELL_FP32 = {
+    "bicg_A": 20, "bicg_B": 12, "bicg_C": 36, "convdiff": 48, "rhs": 32, "project": 96,
 }
 
 WORKLOADS = {
@@ -48,7 +56,7 @@ WORKLOADS = {
 }
 
 
-def make_solver(wl, precond, solver, seed):
+def make_solver(wl, precond, solver, seed, maxit=5000, precision=0):
     import cfd_nssolver_forhpc_flowpassingcylinder_amd as pkg
     from cfd_nssolver_forhpc_flowpassingcylinder_amd.api import (PRECOND_JACOBI, PRECOND_NONE,
                                                                   SOLVER_BICGSTAB, SOLVER_RBSOR)
@@ -56,15 +64,45 @@ def make_solver(wl, precond, solver, seed):
     slip_dir = [[1, 0, 0], [0, 0, 0], [0, 0, 0], [0, 0, 0], [0, 0, 0], [0, 0, 0]]
     cfg = pkg.default_config(
         nx=wl["nx"], ny=wl["ny"], nz=wl["nz"], lx=20.0, ly=10.0, lz=0.5, re=wl["re"],
-        dt=wl["dt"], tol=wl["tol"], maxit=5000,
+        dt=wl["dt"], tol=wl["tol"], maxit=maxit,
         precond=PRECOND_JACOBI if precond == "jacobi" else PRECOND_NONE,
         solver=SOLVER_RBSOR if solver == "rbsor" else SOLVER_BICGSTAB, sor_omega=1.8,
         bc_num=slip_num, bc_dir=slip_dir, z_periodic=1, outflow_convective=1,
-        convective_speed=1.0, flux_correction=1, skip_first_pressure=1)
+        convective_speed=1.0, flux_correction=1, skip_first_pressure=1,
+        precision=precision, inner_tol=MIXED_INNER_TOL)
     s = pkg.FlowSolver(cfg)
     s.init_flow(1.0, 1e-3, seed)
     s.set_eta_cylinder_z(5.0, 5.0, 0.5, 100)
     return s
+
+
+def copy_probe(mib):
+    import torch
+    n = mib * (1 << 20) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8,
+                      "launches": 3}), flush=True)
+
+
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary for this
+    workload (FETCH_SIZE + WRITE_SIZE per dispatch, calibrated on a copy kernel), or
+    None when no summary covers it."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{workload}.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("#traffic "):
+                rec = json.loads(line[len("#traffic "):])
+                if rec.get("kernel") == kernel:
+                    return rec.get("bytes_per_launch")
+    return None
 
 
 def cpu_baseline(wl, iters_per_step, budget_s):
@@ -129,6 +167,44 @@ def cpu_baseline(wl, iters_per_step, budget_s):
     }
 
 
+def run_mode(wl, args, precision):
+    """W warm-up steps, K timed steps (wall clock), then profiled steps (HIP events)."""
+    tag = "FP64" if precision == 0 else "mixed"
+    s = make_solver(wl, args.precond, args.solver, args.seed, args.maxit, precision)
+    warm = []
+    for q in range(args.warmup):
+        warm.append(s.step(1))
+        print(f"{tag} warmup step {q + 1}: {warm[-1]['iters']} iterations", file=sys.stderr,
+              flush=True)
+    s.synchronize()
+    infos = []
+    t0 = time.perf_counter()
+    for q in range(args.steps):
+        infos.append(s.step(1))
+        print(f"{tag} timed step {q + 1}: {infos[-1]['iters']} iterations, residual "
+              f"{infos[-1]['err']:.3e}", file=sys.stderr, flush=True)
+    s.synchronize()
+    wall = time.perf_counter() - t0
+    # per-kernel device time from HIP events on the library's stream, over further
+    # steps of the same run (events are kept out of the wall-clock region)
+    s.profile(True)
+    for q in range(args.profile_steps):
+        s.step(1)
+        print(f"{tag} profiled step {q + 1}", file=sys.stderr, flush=True)
+    s.synchronize()
+    prof = s.profile_read()
+    s.profile(False)
+    return s, warm, infos, wall, prof
+
+
+def kernel_table(prof, n_cells, bytes_per_cell):
+    return {k: {"ms_total": round(v[0], 3), "launches": v[1],
+                "avg_us": round(v[0] * 1e3 / v[1], 2) if v[1] else None,
+                "GBps": (round(bytes_per_cell[k] * n_cells / (v[0] / 1e3 / v[1]) / 1e9, 1)
+                         if v[1] and k in bytes_per_cell else None)}
+            for k, v in prof.items() if v[1] > 0}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -139,9 +215,21 @@ def main():
     ap.add_argument("--solver", default="bicgstab", choices=["bicgstab", "rbsor"])
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="skip the mixed-precision run (FP32 inner BiCGSTAB, FP64 refinement)")
     ap.add_argument("--cpu-budget", type=float, default=25.0)
     ap.add_argument("--profile-steps", type=int, default=1)
+    ap.add_argument("--maxit", type=int, default=5000,
+                    help="BiCGSTAB iteration cap (5000, the reference default; lower only for "
+                         "counter-collection passes, which need few launches)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB three times (the "
+                         "calibration kernel for HBM byte counters) and exit")
     args = ap.parse_args()
+
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
 
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not run "
@@ -149,37 +237,13 @@ def main():
         sys.exit(2)
 
     wl = WORKLOADS[args.workload]
-    s = make_solver(wl, args.precond, args.solver, args.seed)
     n_cells = wl["nx"] * wl["ny"] * wl["nz"]
-    warm = []
-    for q in range(args.warmup):
-        warm.append(s.step(1))
-        print(f"warmup step {q + 1}: {warm[-1]['iters']} iterations", file=sys.stderr,
-              flush=True)
-    s.synchronize()
-    infos = []
-    t0 = time.perf_counter()
-    for q in range(args.steps):
-        infos.append(s.step(1))
-        print(f"timed step {q + 1}: {infos[-1]['iters']} iterations", file=sys.stderr,
-              flush=True)
-    s.synchronize()
-    t1 = time.perf_counter()
-    wall = t1 - t0
-    # per-kernel device time from HIP events on the library's stream, over further
-    # steps of the same run (events are kept out of the wall-clock region)
-    s.profile(True)
-    for q in range(args.profile_steps):
-        s.step(1)
-        print(f"profiled step {q + 1}", file=sys.stderr, flush=True)
-    s.synchronize()
-    prof = s.profile_read()
-    s.profile(False)
+    s, warm, infos, wall, prof = run_mode(wl, args, 0)
     value = n_cells * args.steps / wall / 1e6
     iters = [i["iters"] for i in infos]
     mean_iters = sum(iters) / max(1, len(iters))
 
-    # dominant kernel by device time during the timed steps
+    # dominant kernel by device time during the profiled steps
     cands = {k: v for k, v in prof.items() if k in BYTES_PER_CELL and v[1] > 0}
     dom = max(cands, key=lambda k: cands[k][0])
     ms, launches = cands[dom]
@@ -188,13 +252,49 @@ def main():
     achieved = alg_bytes / avg_s / 1e9
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": None, "avg_launch_us": round(avg_s * 1e6, 2),
+                "traffic": measured_traffic(args.workload, dom),
+                "avg_launch_us": round(avg_s * 1e6, 2),
                 "bytes_per_launch": alg_bytes}
-    kernels = {k: {"ms_total": round(v[0], 3), "launches": v[1],
-                   "avg_us": round(v[0] * 1e3 / v[1], 2) if v[1] else None,
-                   "GBps": (round(BYTES_PER_CELL[k] * n_cells / (v[0] / 1e3 / v[1]) / 1e9, 1)
-                            if v[1] and k in BYTES_PER_CELL else None)}
-               for k, v in prof.items() if v[1] > 0}
+    kernels = kernel_table(prof, n_cells, BYTES_PER_CELL)
+
+    mixed = None
+    if not args.no_mixed and args.solver == "bicgstab":
+        import numpy as np
+        ref = [s.download_velocity(0), s.pressure()]
+        ref_last = infos[-1]
+        s.close()
+        sm, mwarm, minfos, mwall, mprof = run_mode(wl, args, 1)
+        mvalue = n_cells * args.steps / mwall / 1e6
+
+        def rel(a, b):
+            return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+        got = sm.download_velocity(0)
+        unorm = float(np.linalg.norm(ref[0][0]))
+        errs = {"u": rel(got[0], ref[0][0]),
+                "v": float(np.linalg.norm(got[1] - ref[0][1])) / unorm,
+                "w": float(np.linalg.norm(got[2] - ref[0][2])) / unorm,
+                "p": None}
+        # pressure is defined up to a constant: interior fields with the mean removed
+        shp = (wl["nx"] + 4, wl["ny"] + 4, wl["nz"] + 4)
+        pr = ref[1].reshape(shp)[2:-2, 2:-2, 2:-2]
+        pm = sm.pressure().reshape(shp)[2:-2, 2:-2, 2:-2]
+        errs["p"] = rel(pm - pm.mean(), pr - pr.mean())
+        mixed = {
+            "value": round(mvalue, 4), "unit": "Mcell-steps/s",
+            "ms_per_step": round(mwall / args.steps * 1e3, 3),
+            "speedup_vs_fp64": round(mvalue / value, 3),
+            "inner_tol": MIXED_INNER_TOL, "tol": wl["tol"],
+            "iters_per_step": [i["iters"] for i in minfos],
+            "residual_per_step": [i["err"] for i in minfos],
+            "errors_vs_fp64": errs,
+            "errors_note": ("relative L2 after the same warm-up + timed + profiled steps from "
+                            "the same initial field; v and w relative to |u|; the FP64 "
+                            "run is the library's FP64 path, bitwise equal to the FP64 oracle"),
+            "kernels": kernel_table(mprof, n_cells, BYTES_PER_CELL_FP32),
+        }
+        sm.close()
+        s = None
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -220,7 +320,9 @@ def main():
                          f"{args.solver} ({args.precond}) tol {wl['tol']:g}, FP64"),
             "grid": [wl["nx"], wl["ny"], wl["nz"]],
             "solver": args.solver, "precond": args.precond, "tol": wl["tol"],
+            "maxit": args.maxit,
             "iters_per_step": iters, "mean_iters": mean_iters,
+            "residual_per_step": [i["err"] for i in infos],
             "warmup_iters": [i["iters"] for i in warm],
             "cd_last": infos[-1]["cd"], "cl_last": infos[-1]["cl"],
         },
@@ -228,9 +330,11 @@ def main():
         "profile_steps": args.profile_steps,
         "cpu_baseline": cpu,
         "kernels": kernels,
+        "mixed": mixed,
     }
     print(json.dumps(line), flush=True)
-    s.close()
+    if s is not None:
+        s.close()
 
 
 if __name__ == "__main__":

@@ -26,6 +26,13 @@ struct cfdmx_ctx {
     double *p = nullptr, *eta = nullptr, *f = nullptr;
     double *x = nullptr, *b = nullptr, *px1 = nullptr, *px2 = nullptr;
     BicgBuffers B = {};
+    // mixed precision: FP32 Krylov vectors and correction, inner-solve state, the
+    // three folded sums of the FP64 residual
+    BicgBuf<float> B32 = {};
+    float *e32 = nullptr;
+    KState *kst32 = nullptr, *kst32_h = nullptr;
+    double *mix = nullptr, *mix_h = nullptr;
+    int mix_outer = 0;
     double *part = nullptr, *cfl_part = nullptr, *flux_part = nullptr;
     double *bpart[3] = {}; // per-phase BiCGSTAB partials (A, B, C)
     double *scal = nullptr; // [0] cfl, [1] cd, [2] cl
@@ -344,6 +351,18 @@ static int setup_device(cfdmx_ctx *ctx)
     HIPCHK(hipHostMalloc((void **)&ctx->sst_h, sizeof(SorState), hipHostMallocDefault));
     HIPCHK(hipHostMalloc((void **)&ctx->scal_h, 16 * sizeof(double), hipHostMallocDefault));
     HIPCHK(hipHostMalloc((void **)&ctx->count_h, sizeof(int), hipHostMallocDefault));
+    if (c.precision == CFDMX_PRECISION_MIXED) {
+        BicgBuf<float> &F = ctx->B32;
+        float **fz[] = {&F.r0, &F.r, &F.as, &F.p[0], &F.p[1], &F.ap[0], &F.ap[1], &ctx->e32};
+        for (float **z : fz) {
+            HIPCHK(hipMalloc((void **)z, N * sizeof(float)));
+            ctx->allocs.push_back(*z);
+        }
+        ALLOC(ctx->kst32, 1);
+        ALLOC(ctx->mix, 4);
+        HIPCHK(hipHostMalloc((void **)&ctx->kst32_h, sizeof(KState), hipHostMallocDefault));
+        HIPCHK(hipHostMalloc((void **)&ctx->mix_h, 4 * sizeof(double), hipHostMallocDefault));
+    }
 
     // reference initial state: u = 1, v = w = 0 in T0/T1/T3 (RUN_CPU.cpp:66-78),
     // p = 0, eta = 0, f = 0 (resize_variable), X_result = 0 (RUN_CPU.cpp:120)
@@ -399,9 +418,18 @@ static int validate(cfdmx_ctx *ctx)
     if (c.precond != CFDMX_PRECOND_NONE && c.precond != CFDMX_PRECOND_JACOBI)
         FAIL(CFDMX_ERR_ARG, "unknown precond %d", c.precond);
     if (c.maxit < 0) FAIL(CFDMX_ERR_ARG, "maxit must be >= 0");
-    if (c.precision != CFDMX_PRECISION_FP64)
-        FAIL(CFDMX_ERR_ARG, "precision %d is not available", c.precision);
-    if (c.nz > 640) FAIL(CFDMX_ERR_ARG, "nz above 640 is not supported (LDS ring of the stencil)");
+    if (c.precision != CFDMX_PRECISION_FP64 && c.precision != CFDMX_PRECISION_MIXED)
+        FAIL(CFDMX_ERR_ARG, "unknown precision %d", c.precision);
+    if (c.precision == CFDMX_PRECISION_MIXED) {
+        if (c.solver != CFDMX_SOLVER_BICGSTAB)
+            FAIL(CFDMX_ERR_ARG, "mixed precision needs the BiCGSTAB solver");
+        if (!(c.inner_tol > 0.0 && c.inner_tol < 1.0))
+            FAIL(CFDMX_ERR_ARG, "inner_tol must be in (0, 1)");
+        if (c.grid_kind != CFDMX_GRID_UNIFORM)
+            FAIL(CFDMX_ERR_ARG, "mixed precision needs the uniform grid (on the stretched SML "
+                                "grid the FP32 inner solve stalls)");
+    }
+    if (c.nz > 600) FAIL(CFDMX_ERR_ARG, "nz above 600 is not supported (LDS ring of the stencil)");
     if (c.z_periodic && c.solver == CFDMX_SOLVER_RBSOR && (c.nz % 2) != 0)
         FAIL(CFDMX_ERR_ARG, "red-black SOR with periodic z needs an even nz");
     if (c.flux_correction) {
@@ -447,7 +475,7 @@ int cfdmx_config_default(cfdmx_config *cfg)
     cfg->sor_omega = 1.8; // p_sor_omega (0_1_SimulationVariable.hpp:65)
     cfg->skip_first_pressure = 1;
     cfg->precision = CFDMX_PRECISION_FP64;
-    cfg->inner_tol = 1e-5;
+    cfg->inner_tol = 1e-3;
     cfg->xcd_remap = 1;
     return CFDMX_OK;
 }
@@ -465,6 +493,8 @@ int cfdmx_destroy(cfdmx_ctx *ctx)
     prof_harvest(ctx);
     for (hipEvent_t e : ctx->prof_pool) (void)hipEventDestroy(e);
     if (ctx->kst_h) (void)hipHostFree(ctx->kst_h);
+    if (ctx->kst32_h) (void)hipHostFree(ctx->kst32_h);
+    if (ctx->mix_h) (void)hipHostFree(ctx->mix_h);
     if (ctx->sst_h) (void)hipHostFree(ctx->sst_h);
     if (ctx->scal_h) (void)hipHostFree(ctx->scal_h);
     if (ctx->count_h) (void)hipHostFree(ctx->count_h);
@@ -762,7 +792,7 @@ static int solve_bicgstab_impl(cfdmx_ctx *ctx, int *iters, double *res)
     std::swap(ctx->px1, ctx->px2); // px2 <- px1, then px1 <- x (bicgstabRe2.hpp:94-95)
     PROF(CFDMX_K_BICG_INIT, launch_bicg_init(g, s, pre, ctx->b, ctx->x, ctx->B.r0, ctx->B.r,
                                              ctx->px1, ctx->part, ctx->kst));
-    PROF(CFDMX_K_BICG_FIN, launch_bicg_fin0(g, s, ctx->part, ctx->kst));
+    PROF(CFDMX_K_BICG_CHECK, launch_bicg_fin0(g, s, ctx->part, ctx->kst));
     LAUNCHCHK();
     int batch = 4, m = 0;
     for (;;) {
@@ -771,7 +801,7 @@ static int solve_bicgstab_impl(cfdmx_ctx *ctx, int *iters, double *res)
             PROF(CFDMX_K_BICG_B, launch_bicg_B(g, s, pre, ctx->B, ctx->bpart, ctx->kst, m));
             PROF(CFDMX_K_BICG_C, launch_bicg_C(g, s, ctx->B, ctx->x, ctx->bpart, ctx->kst, m));
         }
-        PROF(CFDMX_K_BICG_FIN, launch_bicg_check(g, s, ctx->bpart, ctx->kst, m - 1));
+        PROF(CFDMX_K_BICG_CHECK, launch_bicg_check(g, s, ctx->bpart, ctx->kst, m - 1));
         LAUNCHCHK();
         HIPCHK(hipMemcpyAsync(ctx->kst_h, ctx->kst, sizeof(KState), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
@@ -800,9 +830,89 @@ static int solve_bicgstab_impl(cfdmx_ctx *ctx, int *iters, double *res)
     return CFDMX_OK;
 }
 
+// Mixed precision (no reference counterpart): FP64 iterative refinement.  Each outer
+// pass forms r = b - A x in FP64 and stops when ||r|| / ||b|| < tol; otherwise an FP32
+// BiCGSTAB (the same phase kernels on float vectors, FP64 sums) solves A e = r to the
+// relative tolerance max(inner_tol, 0.3 tol ||b|| / ||r||) and x += e in FP64.  The
+// iteration limit bounds the inner iterations summed over the outer passes.  A pass
+// that raises the FP64 residual (a stalled FP32 Krylov process) is taken back and
+// ends the solve with the previous x.
+static constexpr int kMaxOuter = 40;
+
+static int solve_mixed_impl(cfdmx_ctx *ctx, int *iters, double *res)
+{
+    hipStream_t s = ctx->stream;
+    const Geo &g = ctx->g;
+    const cfdmx_config &c = ctx->cfg;
+    const int pre = c.precond;
+    int total = 0, outer = 0;
+    double res64 = 0.0, res_prev = 0.0;
+    for (;;) {
+        if (outer > 0) PROF(CFDMX_K_BICG_INIT, launch_mixed_update(g, s, 1.0, ctx->x, ctx->e32));
+        PROF(CFDMX_K_BICG_INIT, launch_mixed_resid(g, s, pre, ctx->b, ctx->x, ctx->B32.r,
+                                                   ctx->B32.r0, ctx->e32, ctx->part));
+        launch_mixed_fold(g, s, ctx->part, ctx->mix);
+        LAUNCHCHK();
+        HIPCHK(hipMemcpyAsync(ctx->mix_h, ctx->mix, 3 * sizeof(double), hipMemcpyDeviceToHost,
+                              s));
+        HIPCHK(hipStreamSynchronize(s));
+        prof_harvest(ctx);
+        const double bnorm = std::sqrt(ctx->mix_h[0]);
+        res64 = std::sqrt(ctx->mix_h[1]) / bnorm;
+        if (outer > 0 && !(res64 <= res_prev)) {
+            PROF(CFDMX_K_BICG_INIT, launch_mixed_update(g, s, -1.0, ctx->x, ctx->e32));
+            LAUNCHCHK();
+            res64 = res_prev;
+            break;
+        }
+        res_prev = res64;
+        if (!(res64 >= c.tol)) break; // converged (or NaN: nothing left to refine)
+        if (total >= c.maxit || outer >= kMaxOuter) break;
+        KState ks = {};
+        ks.rho[0] = ctx->mix_h[2]; // (r32, r0) with r0 = r32
+        ks.norm0 = std::sqrt(ctx->mix_h[2]);
+        ks.tol = std::fmax(c.inner_tol, 0.3 * c.tol / res64);
+        ks.restart_factor = 1e-7;
+        ks.maxit = c.maxit - total;
+        ks.mode = 2;
+        ks.timestep = 0; // no restarts in the inner solve
+        ks.done = ks.norm0 > 0.0 ? 0 : 1;
+        *ctx->kst32_h = ks;
+        HIPCHK(hipMemcpyAsync(ctx->kst32, ctx->kst32_h, sizeof ks, hipMemcpyHostToDevice, s));
+        ctx->B32.cur = 0;
+        int batch = 4, m = 0;
+        for (;;) {
+            for (int q = 0; q < batch; ++q, ++m) {
+                PROF(CFDMX_K_BICG_A,
+                     launch_bicg_A(g, s, pre, ctx->B32, ctx->bpart, ctx->kst32, m));
+                PROF(CFDMX_K_BICG_B,
+                     launch_bicg_B(g, s, pre, ctx->B32, ctx->bpart, ctx->kst32, m));
+                PROF(CFDMX_K_BICG_C,
+                     launch_bicg_C(g, s, ctx->B32, ctx->e32, ctx->bpart, ctx->kst32, m));
+            }
+            PROF(CFDMX_K_BICG_CHECK, launch_bicg_check(g, s, ctx->bpart, ctx->kst32, m - 1));
+            LAUNCHCHK();
+            HIPCHK(hipMemcpyAsync(ctx->kst32_h, ctx->kst32, sizeof(KState),
+                                  hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            prof_harvest(ctx);
+            if (ctx->kst32_h->done) break;
+            batch = batch < 64 ? batch * 2 : 64;
+        }
+        total += ctx->kst32_h->iter + 1;
+        ++outer;
+        if (!std::isfinite(ctx->kst32_h->res)) break; // diverged: keep x, report res64
+    }
+    ctx->mix_outer = outer;
+    if (iters) *iters = total;
+    if (res) *res = res64;
+    return CFDMX_OK;
+}
+
 int cfdmx_solve_bicgstab(cfdmx_ctx *ctx, int *iters, double *residual)
 {
     if (!ctx) return CFDMX_ERR_ARG;
+    if (ctx->cfg.precision == CFDMX_PRECISION_MIXED) return solve_mixed_impl(ctx, iters, residual);
     return solve_bicgstab_impl(ctx, iters, residual);
 }
 
@@ -926,7 +1036,8 @@ static int step_once(cfdmx_ctx *ctx, int *iters, double *err, int *solved)
         *solved = 1;
         if (c.solver == CFDMX_SOLVER_BICGSTAB) {
             rc = cfdmx_pressure_rhs(ctx);
-            if (!rc) rc = solve_bicgstab_impl(ctx, iters, err);
+            if (!rc) rc = c.precision == CFDMX_PRECISION_MIXED ? solve_mixed_impl(ctx, iters, err)
+                                                               : solve_bicgstab_impl(ctx, iters, err);
         } else {
             rc = solve_rbsor_impl(ctx, iters, err);
         }

@@ -41,11 +41,15 @@ struct SorState {
     int pad;
 };
 
-struct BicgBuffers {
-    double *r0, *r, *as;
-    double *p[2], *ap[2];
+// Krylov vectors of one BiCGSTAB solve (double; float for the mixed-precision inner
+// solve).  p[cur] holds q = p - omega*Ap between phase C and the next phase A.
+template <typename T>
+struct BicgBuf {
+    T *r0, *r, *as;
+    T *p[2], *ap[2];
     int cur;
 };
+using BicgBuffers = BicgBuf<double>;
 
 void launch_convdiff(const Geo &g, hipStream_t s, const double *u, const double *v,
                      const double *w, double *us, double *vs, double *ws, double dt,
@@ -62,13 +66,22 @@ void launch_bicg_init(const Geo &g, hipStream_t s, int pre, const double *b, con
                       double *r0, double *r, double *px1, double *part, KState *st);
 // one BiCGSTAB iteration m = three kernels; each folds the previous phase's partials
 // (part[0] <- A, part[1] <- B, part[2] <- C)
-void launch_bicg_A(const Geo &g, hipStream_t s, int pre, BicgBuffers &B, double *const part[3],
+template <typename T>
+void launch_bicg_A(const Geo &g, hipStream_t s, int pre, BicgBuf<T> &B, double *const part[3],
                    KState *st, int m);
-void launch_bicg_B(const Geo &g, hipStream_t s, int pre, BicgBuffers &B, double *const part[3],
+template <typename T>
+void launch_bicg_B(const Geo &g, hipStream_t s, int pre, BicgBuf<T> &B, double *const part[3],
                    KState *st, int m);
-void launch_bicg_C(const Geo &g, hipStream_t s, BicgBuffers &B, double *x,
-                   double *const part[3], KState *st, int m);
+template <typename T>
+void launch_bicg_C(const Geo &g, hipStream_t s, BicgBuf<T> &B, T *x, double *const part[3],
+                   KState *st, int m);
 void launch_bicg_check(const Geo &g, hipStream_t s, double *const part[3], KState *st, int m);
+// mixed precision: FP64 residual into float r, r0 (e = 0), x += e, fold of the three sums
+void launch_mixed_resid(const Geo &g, hipStream_t s, int pre, const double *b, const double *x,
+                        float *r32, float *r0_32, float *e32, double *part);
+void launch_mixed_update(const Geo &g, hipStream_t s, double sign, double *x, const float *e);
+void launch_mixed_fold(const Geo &g, hipStream_t s, const double *part, double *out);
+
 void launch_bicg_fin0(const Geo &g, hipStream_t s, double *part, KState *st);
 void launch_nanfix(hipStream_t s, int n, double *x, const double *px1, int *count);
 void launch_copy(hipStream_t s, int n, const double *a, double *b);

@@ -630,18 +630,39 @@ k_bicg_init(Geo g, int pre, const double *__restrict__ b, const double *__restri
 }
 
 // Plane-marching stencil phases.  A block owns the plane-index tile
-// q in [q0, q0+1024) (q = j*nzc + k) for the planes i in [i0, i0+8); an LDS ring
-// holds the new vector (p for A, s for B) on planes i-1, i, i+1 over the tile plus
-// one z-row (nzc cells) on each side, so every stencil neighbour is read from LDS.
-// Each ring value is formed once per block from r, p, Ap with the reference's
-// expression; cells of neighbouring tiles/chunks are formed again there, identically.
+// q in [q0, q0+1024) (q = j*nzc + k) for the planes i in [i0, i0+8); even chunks march
+// down (i = i0+7 .. i0), odd chunks up, so that two neighbouring chunks read their
+// shared halo planes at the same time (the second read hits L2).  An LDS ring holds
+// the new vector (p for A, s for B) on the planes behind, at and ahead of the current
+// one over the tile plus one z-row (nzc cells) on each side, so every stencil
+// neighbour is read from LDS.  Each ring value is formed once per block with the
+// reference's expression; cells of neighbouring tiles/chunks are formed again there,
+// identically.  Phase A forms p = r + beta*q from q = p - omega*Ap, which phase C of
+// the previous iteration stores in place of p (the same two roundings as
+// r + beta*(p - omega*Ap), bicgstabRe2.hpp:161-163).
+// T is the storage and arithmetic type of the Krylov vectors (double; float for the
+// inner solve of the mixed-precision mode); sums are always accumulated in double.
+template <typename T>
 struct MarchIO {
-    const double *r, *p, *ap; // inputs
-    double *r0;               // A: restart target
-    double *out_p, *out_ap;   // A outputs
-    double *out_as;           // B output
+    const T *r, *q, *ap; // inputs (A: r, q; B: r, Ap)
+    T *r0;               // A: restart target
+    T *out_p, *out_ap;   // A outputs
+    T *out_as;           // B output
     double *part;
 };
+
+// planes of march block `chunk`: first plane and direction
+__device__ __forceinline__ void march_span(const Geo &g, int chunk, int &first, int &dir, int &len)
+{
+    const int i0 = chunk * kChunk, i1 = min(i0 + kChunk, g.nxc);
+    len = i1 - i0;
+    dir = (chunk & 1) ? 1 : -1;
+    first = dir > 0 ? i0 : i1 - 1;
+}
+
+// LDS of a march block: y coefficients of the tile's rows, z coefficients (double),
+// then the three ring planes (T)
+__host__ __device__ inline int march_rows(int nzc) { return kTile / nzc + 2; }
 
 template <int PRE>
 __device__ __forceinline__ void march_row(const Geo &g, int i, int j, int k, double xm, double xp,
@@ -725,23 +746,28 @@ __device__ __forceinline__ bool bicg_iteration_start(const Geo &g, KState *st, i
     return true;
 }
 
-template <int PHASE, int PRE>
-__global__ void __launch_bounds__(kThreads)
-k_march(Geo g, MarchIO io, KState *st, int m, const double *part_in)
+// NF = 5: at most 128 VGPRs, so that four blocks (one march launch at 512x256x64)
+// fit each CU at once
+template <int PHASE, int PRE, int NF, typename T>
+__global__ void __launch_bounds__(kThreads, NF <= 5 ? 4 : 1)
+k_march(Geo g, MarchIO<T> io, KState *st, int m, const double *part_in)
 {
     if (st->done) return;
     int b = blockIdx.x;
     if (g.xcd_remap) b = (b & 7) * ((gridDim.x + 7) >> 3) + (b >> 3);
     if (b >= g.nmarch) return;
-    extern __shared__ double ring[];
+    extern __shared__ double lds[];
     const int tile = b % g.ntile, chunk = b / g.ntile;
-    const int i0 = chunk * kChunk, i1 = min(i0 + kChunk, g.nxc);
+    int ib, d, len;
+    march_span(g, chunk, ib, d, len);
     const int q0 = tile * kTile, qn = min(kTile, g.cyz - q0);
     const int H = g.nzc, W = qn + 2 * H, WS = kTile + 2 * H;
+    const int RY = march_rows(g.nzc);
+    double *cy = lds, *cz = lds + 2 * RY;
+    T *ring = reinterpret_cast<T *>(cz + 2 * g.nzc);
     const bool writer = (b == 0 && threadIdx.x == 0);
     int mode = 0;
     double beta = 0.0, alpha = 0.0;
-    const double omega = st->omega;
     if (PHASE == 0) {
         if (!bicg_iteration_start(g, st, m, part_in, writer, mode, beta)) return;
     } else {
@@ -750,92 +776,137 @@ k_march(Geo g, MarchIO io, KState *st, int m, const double *part_in)
         alpha = st->rho[m & 1] / t[0]; // a = r1r0 / dot(ap, r0)
         if (writer) st->alpha = alpha;
     }
-    auto f = [&](int n) -> double {
-        if (PHASE == 0) return mode == 0 ? io.r[n] + beta * (io.p[n] - omega * io.ap[n]) : io.r[n];
-        return io.r[n] - alpha * io.ap[n];
+    // 1-D operator coefficients of the tile's rows j in [jlo, jlo + rows) and of all k
+    const int jlo = (int)fdiv((unsigned)q0, g.dz);
+    const int rows = (int)fdiv((unsigned)(q0 + qn - 1), g.dz) - jlo + 1;
+    for (int t = threadIdx.x; t < rows; t += kThreads) {
+        cy[t] = g.pa[1].m[jlo + t];
+        cy[RY + t] = g.pa[1].p[jlo + t];
+    }
+    for (int t = threadIdx.x; t < g.nzc; t += kThreads) {
+        cz[t] = g.pa[2].m[t];
+        cz[g.nzc + t] = g.pa[2].p[t];
+    }
+    // Ring planes are staged in two steps so that a plane's loads are in flight while
+    // the previous plane is computed: load() issues the raw r and q (A) or Ap (B) of
+    // ring plane i (NF cells per thread, ring cell t = threadIdx.x + 256u), put() forms
+    // the ring value and writes it to its LDS slot.
+    auto inside = [&](int i, int t) {
+        const int q = q0 - H + t;
+        return i >= 0 && i < g.nxc && t < W && q >= 0 && q < g.cyz;
     };
-    auto fill = [&](int slot, int i) {
-        double *R = ring + slot * WS;
-        const bool in = (i >= 0 && i < g.nxc);
-        for (int t = threadIdx.x; t < W; t += kThreads) {
-            const int q = q0 - H + t;
-            R[t] = (in && q >= 0 && q < g.cyz) ? f(i * g.cyz + q) : 0.0;
+    const T betaT = (T)beta, alphaT = (T)alpha;
+    auto load = [&](int i, T (&a)[NF], T (&c)[NF]) {
+#pragma unroll
+        for (int u = 0; u < NF; ++u) {
+            const int t = threadIdx.x + u * kThreads;
+            const bool ok = inside(i, t);
+            const int n = i * g.cyz + q0 - H + t;
+            a[u] = ok ? io.r[n] : (T)0;
+            if (PHASE == 0) c[u] = (ok && mode == 0) ? io.q[n] : (T)0;
+            else c[u] = ok ? io.ap[n] : (T)0;
+        }
+    };
+    auto put = [&](int slot, int i, const T (&a)[NF], const T (&c)[NF]) {
+        T *R = ring + slot * WS;
+#pragma unroll
+        for (int u = 0; u < NF; ++u) {
+            const int t = threadIdx.x + u * kThreads;
+            if (t < W) {
+                T v;
+                if (PHASE == 0) v = mode == 0 ? a[u] + betaT * c[u] : a[u];
+                else v = a[u] - alphaT * c[u];
+                R[t] = inside(i, t) ? v : (T)0;
+            }
         }
     };
     int jc[kCells], kc[kCells];
-    double ymv[kCells], ypv[kCells], zmv[kCells], zpv[kCells];
 #pragma unroll
     for (int c = 0; c < kCells; ++c) {
         const int q = q0 + c * kThreads + threadIdx.x;
         const int j = (int)fdiv((unsigned)q, g.dz);
-        jc[c] = j;
+        jc[c] = j - jlo;
         kc[c] = q - j * g.nzc;
-        const bool ok = (c * kThreads + (int)threadIdx.x) < qn;
-        ymv[c] = ok ? g.pa[1].m[j] : 0.0;
-        ypv[c] = ok ? g.pa[1].p[j] : 0.0;
-        zmv[c] = ok ? g.pa[2].m[kc[c]] : 0.0;
-        zpv[c] = ok ? g.pa[2].p[kc[c]] : 0.0;
     }
-    fill(0, i0 - 1);
-    fill(1, i0);
+    T a0[NF], c0[NF], a1[NF], c1[NF];
+    load(ib - d, a0, c0);
+    load(ib, a1, c1);
+    put(0, ib - d, a0, c0);
+    load(ib + d, a0, c0);
+    put(1, ib, a1, c1);
     double acc[2] = {0.0, 0.0};
-    for (int i = i0; i < i1; ++i) {
-        const int sm = (i - i0) % 3, sc = (i - i0 + 1) % 3, sp = (i - i0 + 2) % 3;
-        fill(sp, i + 1);
+    for (int s = 0; s < len; ++s) {
+        const int i = ib + s * d;
+        const int sb = s % 3, sc = (s + 1) % 3, sa = (s + 2) % 3; // behind, at, ahead
+        put(sa, i + d, a0, c0);
+        // the second operand of phase A's dot (r0, or r on a restart) for this plane,
+        // then the next ring plane, both in flight across the barrier
+        T r0v[kCells];
+        if (PHASE == 0) {
+#pragma unroll
+            for (int c = 0; c < kCells; ++c) {
+                const int tq = c * kThreads + threadIdx.x;
+                const int n = i * g.cyz + q0 + tq;
+                r0v[c] = tq < qn ? (mode == 1 ? io.r[n] : io.r0[n]) : (T)0;
+            }
+        }
+        if (s + 1 < len) load(i + 2 * d, a0, c0);
         __syncthreads();
-        const double *Rm = ring + sm * WS, *Rc = ring + sc * WS, *Rp = ring + sp * WS;
+        const T *Rc = ring + sc * WS;
+        const T *Rm = ring + (d > 0 ? sb : sa) * WS, *Rp = ring + (d > 0 ? sa : sb) * WS;
         const double xm = g.pa[0].m[i], xp = g.pa[0].p[i];
 #pragma unroll
         for (int c = 0; c < kCells; ++c) {
             const int tq = c * kThreads + threadIdx.x;
             if (tq >= qn) break;
-            const int j = jc[c], k = kc[c];
+            const int jj = jc[c], k = kc[c], j = jlo + jj;
             double v[7];
-            march_row<PRE>(g, i, j, k, xm, xp, ymv[c], ypv[c], zmv[c], zpv[c], v);
+            march_row<PRE>(g, i, j, k, xm, xp, cy[jj], cy[RY + jj], cz[k], cz[g.nzc + k], v);
+            // the 7-point sum accumulates in double for every T: in float the
+            // Laplacian's cancellation on smooth vectors stalls the Krylov process
             const int tc = tq + H;
-            double s = 0.0;
-            if (i != 0) s += v[0] * Rm[tc];
-            if (j != 0) s += v[1] * Rc[tc - g.nzc];
-            if (k != 0 || g.zper) s += v[2] * Rc[k == 0 ? tc + g.nzc - 1 : tc - 1];
-            s += v[3] * Rc[tc];
-            if (k != g.nzc - 1 || g.zper) s += v[4] * Rc[k == g.nzc - 1 ? tc - (g.nzc - 1) : tc + 1];
-            if (j != g.nyc - 1) s += v[5] * Rc[tc + g.nzc];
-            if (i != g.nxc - 1) s += v[6] * Rp[tc];
+            double acc7 = 0.0;
+            if (i != 0) acc7 += v[0] * (double)Rm[tc];
+            if (j != 0) acc7 += v[1] * (double)Rc[tc - g.nzc];
+            if (k != 0 || g.zper) acc7 += v[2] * (double)Rc[k == 0 ? tc + g.nzc - 1 : tc - 1];
+            acc7 += v[3] * (double)Rc[tc];
+            if (k != g.nzc - 1 || g.zper)
+                acc7 += v[4] * (double)Rc[k == g.nzc - 1 ? tc - (g.nzc - 1) : tc + 1];
+            if (j != g.nyc - 1) acc7 += v[5] * (double)Rc[tc + g.nzc];
+            if (i != g.nxc - 1) acc7 += v[6] * (double)Rp[tc];
+            const T sum = (T)acc7;
             const int n = i * g.cyz + q0 + tq;
-            const double own = Rc[tc];
+            const T own = Rc[tc];
+            const double sd = (double)sum;
             if (PHASE == 0) {
                 io.out_p[n] = own;
-                io.out_ap[n] = s;
-                double r0n;
-                if (mode == 1) {
-                    r0n = io.r[n];
-                    io.r0[n] = r0n;
-                } else {
-                    r0n = io.r0[n];
-                }
-                acc[0] += s * r0n;
+                io.out_ap[n] = sum;
+                const T r0n = r0v[c];
+                if (mode == 1) io.r0[n] = r0n;
+                acc[0] += sd * (double)r0n;
             } else {
-                io.out_as[n] = s;
-                acc[0] += s * own;
-                acc[1] += s * s;
+                io.out_as[n] = sum;
+                acc[0] += sd * (double)own;
+                acc[1] += sd * sd;
             }
         }
         __syncthreads();
     }
     if (PHASE == 0) {
-        double a1[1] = {acc[0]};
-        block_sums<1>(a1, io.part, g.nmarch, b);
+        double a1s[1] = {acc[0]};
+        block_sums<1>(a1s, io.part, g.nmarch, b);
     } else {
         block_sums<2>(acc, io.part, g.nmarch, b);
     }
 }
 
-// phase C in the march mapping (elementwise)
+// phase C in the march mapping (elementwise): x, r update (bicgstabRe2.hpp:123-127),
+// the partials of (r,r), (r,r0), and q = p - omega*Ap in place of p for the next phase A
+template <typename T>
 __global__ void __launch_bounds__(kThreads)
-k_march_C(Geo g, double *__restrict__ r, const double *__restrict__ ap,
-          const double *__restrict__ p, const double *__restrict__ as,
-          const double *__restrict__ r0, double *__restrict__ x, const double *part_in,
-          double *__restrict__ part_out, KState *st)
+k_march_C(Geo g, T *__restrict__ r, const T *__restrict__ ap, T *__restrict__ p,
+          const T *__restrict__ as, const T *__restrict__ r0, T *__restrict__ x,
+          const double *part_in, double *__restrict__ part_out, KState *st)
 {
     if (st->done) return;
     int b = blockIdx.x;
@@ -845,27 +916,86 @@ k_march_C(Geo g, double *__restrict__ r, const double *__restrict__ ap,
     fold_partials<2>(part_in, g.nmarch, g.nmarch, t);
     const double omega = t[0] / t[1]; // w = dot(as, s) / dot(as, as)
     if (b == 0 && threadIdx.x == 0) st->omega = omega;
-    const double alpha = st->alpha;
+    const T alpha = (T)st->alpha, om = (T)omega;
     const int tile = b % g.ntile, chunk = b / g.ntile;
-    const int i0 = chunk * kChunk, i1 = min(i0 + kChunk, g.nxc);
+    int ib, d, len;
+    march_span(g, chunk, ib, d, len);
     const int q0 = tile * kTile, qn = min(kTile, g.cyz - q0);
     double acc[2] = {0.0, 0.0};
-    for (int i = i0; i < i1; ++i) {
+    for (int s = 0; s < len; ++s) {
+        const int i = ib + s * d;
 #pragma unroll
         for (int c = 0; c < kCells; ++c) {
             const int tq = c * kThreads + threadIdx.x;
             if (tq < qn) {
                 const int n = i * g.cyz + q0 + tq;
-                const double sn = r[n] - alpha * ap[n];
-                x[n] += alpha * p[n] + omega * sn;
-                const double rn = sn - omega * as[n];
+                const T apn = ap[n], pn = p[n];
+                const T sn = r[n] - alpha * apn;
+                x[n] += alpha * pn + om * sn;
+                const T rn = sn - om * as[n];
                 r[n] = rn;
-                acc[0] += rn * rn;
-                acc[1] += rn * r0[n];
+                p[n] = pn - om * apn;
+                const double rd = (double)rn;
+                acc[0] += rd * rd;
+                acc[1] += rd * (double)r0[n];
             }
         }
     }
     block_sums<2>(acc, part_out, g.nmarch, b);
+}
+
+// ---- mixed precision (no reference counterpart): FP64 iterative refinement around
+// an FP32 BiCGSTAB.  The outer residual r = b - A x is formed in FP64 on the same
+// operator rows as k_bicg_init; the inner solve A e = r starts from e = 0 with
+// r0 = p = float(r).  Partials: (b,b), (r,r) in FP64 and (r32,r32).
+__global__ void __launch_bounds__(kThreads)
+k_mixed_resid(Geo g, int pre, const double *__restrict__ b, const double *__restrict__ x,
+              float *__restrict__ r32, float *__restrict__ r0_32, float *__restrict__ e32,
+              double *__restrict__ part)
+{
+    const int L = logical_block(g);
+    if (L >= g.nblocks) return;
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int c = 0; c < kCells; ++c) {
+        const int n = L * kPerBlock + c * kThreads + threadIdx.x;
+        if (n >= g.N) break;
+        int i, j, k;
+        decode(g, n, i, j, k);
+        RowCoef rc;
+        if (pre) row_coef<1>(g, i, j, k, rc);
+        else row_coef<0>(g, i, j, k, rc);
+        const double ax = row_apply(rc, n, [&](int m) { return x[m]; });
+        const double bn = b[n];
+        const double rr = bn - ax;
+        const float rf = (float)rr;
+        r32[n] = rf;
+        r0_32[n] = rf;
+        e32[n] = 0.0f;
+        acc[0] += bn * bn;
+        acc[1] += rr * rr;
+        acc[2] += (double)rf * (double)rf;
+    }
+    block_sums<3>(acc, part, g.nblocks, L);
+}
+
+// x += sign * e (FP64 accumulation of the FP32 correction; sign -1 takes it back)
+__global__ void k_mixed_update(int n, double sign, double *__restrict__ x,
+                               const float *__restrict__ e)
+{
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x)
+        x[q] += sign * (double)e[q];
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_mixed_fold(const double *part, int nb, double *out)
+{
+    double t[3];
+    fold_partials<3>(part, nb, nb, t);
+    if (threadIdx.x == 0) {
+        out[0] = t[0];
+        out[1] = t[1];
+        out[2] = t[2];
+    }
 }
 
 // end of a batch: the convergence test of the last iteration m (bicgstabRe2.hpp:129-153)
@@ -1304,55 +1434,80 @@ static inline int march_blocks(const Geo &g)
 {
     return g.xcd_remap ? ((g.nmarch + 7) / 8) * 8 : g.nmarch;
 }
+template <typename T>
 static inline size_t march_lds(const Geo &g)
 {
-    return (size_t)3 * (kTile + 2 * g.nzc) * sizeof(double);
+    return ((size_t)2 * march_rows(g.nzc) + 2 * g.nzc) * sizeof(double) +
+           (size_t)3 * (kTile + 2 * g.nzc) * sizeof(T);
 }
 
-void launch_bicg_A(const Geo &g, hipStream_t s, int pre, BicgBuffers &B, double *const part[3],
+// ring cells per thread: ceil((1024 + 2 nzc) / 256); 5 covers nzc <= 128, 9 covers the
+// largest supported nzc (600)
+template <int PHASE, int NF, typename T>
+static void launch_march_nf(const Geo &g, hipStream_t s, int pre, const MarchIO<T> &io,
+                            KState *st, int m, const double *part_in)
+{
+    if (pre)
+        hipLaunchKernelGGL((k_march<PHASE, 1, NF, T>), dim3(march_blocks(g)), dim3(kThreads),
+                           march_lds<T>(g), s, g, io, st, m, part_in);
+    else
+        hipLaunchKernelGGL((k_march<PHASE, 0, NF, T>), dim3(march_blocks(g)), dim3(kThreads),
+                           march_lds<T>(g), s, g, io, st, m, part_in);
+}
+template <int PHASE, typename T>
+static void launch_march(const Geo &g, hipStream_t s, int pre, const MarchIO<T> &io, KState *st,
+                         int m, const double *part_in)
+{
+    const int nf = (kTile + 2 * g.nzc + kThreads - 1) / kThreads;
+    if (nf <= 5)
+        launch_march_nf<PHASE, 5>(g, s, pre, io, st, m, part_in);
+    else
+        launch_march_nf<PHASE, 9>(g, s, pre, io, st, m, part_in);
+}
+template <typename T>
+void launch_bicg_A(const Geo &g, hipStream_t s, int pre, BicgBuf<T> &B, double *const part[3],
                    KState *st, int m)
 {
-    MarchIO io = {};
+    MarchIO<T> io = {};
     io.r = B.r;
-    io.p = B.p[B.cur];
-    io.ap = B.ap[B.cur];
+    io.q = B.p[B.cur]; // q = p - omega*Ap, stored by phase C
     io.r0 = B.r0;
     io.out_p = B.p[1 - B.cur];
     io.out_ap = B.ap[1 - B.cur];
     io.part = part[0];
-    if (pre)
-        hipLaunchKernelGGL((k_march<0, 1>), dim3(march_blocks(g)), dim3(kThreads), march_lds(g),
-                           s, g, io, st, m, (const double *)part[2]);
-    else
-        hipLaunchKernelGGL((k_march<0, 0>), dim3(march_blocks(g)), dim3(kThreads), march_lds(g),
-                           s, g, io, st, m, (const double *)part[2]);
+    launch_march<0>(g, s, pre, io, st, m, part[2]);
     B.cur = 1 - B.cur;
 }
-
-void launch_bicg_B(const Geo &g, hipStream_t s, int pre, BicgBuffers &B, double *const part[3],
+template <typename T>
+void launch_bicg_B(const Geo &g, hipStream_t s, int pre, BicgBuf<T> &B, double *const part[3],
                    KState *st, int m)
 {
-    MarchIO io = {};
+    MarchIO<T> io = {};
     io.r = B.r;
     io.ap = B.ap[B.cur];
     io.out_as = B.as;
     io.part = part[1];
-    if (pre)
-        hipLaunchKernelGGL((k_march<1, 1>), dim3(march_blocks(g)), dim3(kThreads), march_lds(g),
-                           s, g, io, st, m, (const double *)part[0]);
-    else
-        hipLaunchKernelGGL((k_march<1, 0>), dim3(march_blocks(g)), dim3(kThreads), march_lds(g),
-                           s, g, io, st, m, (const double *)part[0]);
+    launch_march<1>(g, s, pre, io, st, m, part[0]);
 }
-
-void launch_bicg_C(const Geo &g, hipStream_t s, BicgBuffers &B, double *x,
-                   double *const part[3], KState *st, int m)
+template <typename T>
+void launch_bicg_C(const Geo &g, hipStream_t s, BicgBuf<T> &B, T *x, double *const part[3],
+                   KState *st, int m)
 {
     (void)m;
-    hipLaunchKernelGGL(k_march_C, dim3(march_blocks(g)), dim3(kThreads), 0, s, g, B.r,
+    hipLaunchKernelGGL(k_march_C<T>, dim3(march_blocks(g)), dim3(kThreads), 0, s, g, B.r,
                        B.ap[B.cur], B.p[B.cur], B.as, B.r0, x, (const double *)part[1], part[2],
                        st);
 }
+#define CFDMX_BICG_INSTANCES(T)                                                               \
+    template void launch_bicg_A<T>(const Geo &, hipStream_t, int, BicgBuf<T> &,               \
+                                   double *const[3], KState *, int);                          \
+    template void launch_bicg_B<T>(const Geo &, hipStream_t, int, BicgBuf<T> &,               \
+                                   double *const[3], KState *, int);                          \
+    template void launch_bicg_C<T>(const Geo &, hipStream_t, BicgBuf<T> &, T *,               \
+                                   double *const[3], KState *, int);
+CFDMX_BICG_INSTANCES(double)
+CFDMX_BICG_INSTANCES(float)
+#undef CFDMX_BICG_INSTANCES
 
 void launch_bicg_check(const Geo &g, hipStream_t s, double *const part[3], KState *st, int m)
 {
@@ -1360,6 +1515,20 @@ void launch_bicg_check(const Geo &g, hipStream_t s, double *const part[3], KStat
                        st, m);
 }
 
+void launch_mixed_resid(const Geo &g, hipStream_t s, int pre, const double *b, const double *x,
+                        float *r32, float *r0_32, float *e32, double *part)
+{
+    hipLaunchKernelGGL(k_mixed_resid, dim3(grid_blocks(g)), dim3(kThreads), 0, s, g, pre, b, x,
+                       r32, r0_32, e32, part);
+}
+void launch_mixed_update(const Geo &g, hipStream_t s, double sign, double *x, const float *e)
+{
+    hipLaunchKernelGGL(k_mixed_update, dim3(2048), dim3(kThreads), 0, s, g.N, sign, x, e);
+}
+void launch_mixed_fold(const Geo &g, hipStream_t s, const double *part, double *out)
+{
+    hipLaunchKernelGGL(k_mixed_fold, dim3(1), dim3(kThreads), 0, s, part, g.nblocks, out);
+}
 void launch_bicg_fin0(const Geo &g, hipStream_t s, double *part, KState *st)
 {
     hipLaunchKernelGGL(k_bicg_fin0, dim3(1), dim3(kThreads), 0, s, part, g.nblocks, st);

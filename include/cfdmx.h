@@ -67,7 +67,8 @@ typedef struct cfdmx_config {
     double sor_omega;
     int skip_first_pressure; /* RUN_CPU.cpp:303: no pressure solve on step 1 */
     int precision;
-    double inner_tol; /* mixed precision: FP32 inner tolerance */
+    double inner_tol; /* mixed precision: relative tolerance of each FP32 inner solve
+                         (default 1e-3; 1e-5 can stall the FP32 Krylov process) */
     /* launch shaping */
     int xcd_remap; /* map consecutive logical blocks to one XCD (0 = hardware order) */
 } cfdmx_config;
@@ -184,7 +185,7 @@ int cfdmx_synchronize(cfdmx_ctx *ctx);
 #define CFDMX_K_BICG_A 4
 #define CFDMX_K_BICG_B 5
 #define CFDMX_K_BICG_C 6
-#define CFDMX_K_BICG_FIN 7
+#define CFDMX_K_BICG_CHECK 7 /* initial fold and end-of-batch convergence check */
 #define CFDMX_K_SOR 8
 #define CFDMX_K_PROJECT 9
 #define CFDMX_K_COUNT 10

@@ -177,8 +177,9 @@ double orc_blocked_sum(const double *v, long n)
 
 /* march order of the device's plane-marching stencil kernels: block b = chunk*ntile +
  * tile covers planes [8*chunk, 8*chunk+8) of the plane-index tile [1024*tile,
- * 1024*tile+1024); thread t accumulates plane by plane the cells q = 1024*tile +
- * 256c + t, c = 0..3; then the same block fold and partial fold as orc_blocked_sum. */
+ * 1024*tile+1024), downwards for even chunks and upwards for odd ones; thread t
+ * accumulates plane by plane the cells q = 1024*tile + 256c + t, c = 0..3; then the
+ * same block fold and partial fold as orc_blocked_sum. */
 double orc_march_sum(const orc_grid *g, const double *v)
 {
     const long cyz = (long)g->nyc * g->nzc;
@@ -194,12 +195,14 @@ double orc_march_sum(const orc_grid *g, const double *v)
         const long q0 = (long)tile * ORC_MARCH_TILE;
         for (int t = 0; t < ORC_RB_THREADS; ++t) {
             double s = 0.0;
-            for (int i = i0; i < i1; ++i)
+            for (int step = 0; step < i1 - i0; ++step) {
+                const int i = (chunk & 1) ? i0 + step : i1 - 1 - step;
                 for (int c = 0; c < ORC_RB_CELLS; ++c) {
                     const long q = q0 + (long)c * ORC_RB_THREADS + t;
                     if (q < cyz)
                         s += v[(long)i * cyz + q];
                 }
+            }
             acc[t] = s;
         }
         part[b] = tree256(acc);

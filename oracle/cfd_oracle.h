@@ -136,6 +136,25 @@ void orc_sim_start(orc_sim *s); /* initial BC_main + BC_copy, T1 = T3 = T0 */
 int orc_sim_step(orc_sim *s, double out[8]);
 /* out: cd, cl, time, iters, err, cfl, restarts, solve_flag */
 
+/* ---- adjoint of the time-averaged drag (cfd_adjoint.c; no reference counterpart) ----
+ * Uniform grid, SPE operator.  J = (1/nsteps) sum_n Cd^n from U^0 = BC_start(U_init);
+ * the mask is orc_eta_smoothed(cx, cy, R, eps); bc->inflow (ghosted j) is the inflow
+ * profile.  grad_inflow (length ny, ghosted j) and grad_mask (d/dcx, d/dcy, d/dR) may be
+ * NULL.  The pressure adjoint is solved to adj_tol. */
+double orc_adjoint_drag(const orc_grid *g, const orc_bc *bc, const orc_params *prm,
+                        const double *u_init, const double *v_init, const double *w_init,
+                        double cx, double cy, double R, double eps, int nsteps,
+                        double adj_tol, double *grad_inflow, double grad_mask[3]);
+double orc_drag_objective(const orc_grid *g, const orc_bc *bc, const orc_params *prm,
+                          const double *u_init, const double *v_init, const double *w_init,
+                          double cx, double cy, double R, double eps, int nsteps);
+/* stage transposes, for dot-product tests */
+void orc_adj_convdiff(const orc_grid *g, double dt, double nu, int zp, const double *u,
+                      const double *v, const double *w, const double *lus, const double *lvs,
+                      const double *lws, double *lu, double *lv, double *lw);
+void orc_adj_bc_main(const orc_grid *g, const orc_bc *bc, int advance_outflow, double dt,
+                     double *lu, double *lv, double *lw, double *l_inflow);
+
 #ifdef __cplusplus
 }
 #endif

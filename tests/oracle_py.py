@@ -145,6 +145,41 @@ class Oracle:
         L.orc_sim_field.argtypes = [C.c_void_p, C.c_int]
         L.orc_sim_start.argtypes = [C.c_void_p]
         L.orc_sim_step.argtypes = [C.c_void_p, DP]
+        L.orc_adjoint_drag.restype = C.c_double
+        L.orc_adjoint_drag.argtypes = [G, C.POINTER(OrcBC), C.POINTER(OrcParams), DP, DP, DP,
+                                       C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
+                                       C.c_double, DP, DP]
+        L.orc_drag_objective.restype = C.c_double
+        L.orc_drag_objective.argtypes = [G, C.POINTER(OrcBC), C.POINTER(OrcParams), DP, DP, DP,
+                                         C.c_double, C.c_double, C.c_double, C.c_double, C.c_int]
+        L.orc_adj_convdiff.argtypes = [G, C.c_double, C.c_double, C.c_int] + [DP] * 9
+        L.orc_adj_bc_main.argtypes = [G, C.POINTER(OrcBC), C.c_int, C.c_double] + [DP] * 4
+
+    # ---- adjoint of the time-averaged drag (cfd_adjoint.c) ----
+    def adjoint_drag(self, g, bc, prm, u, v, w, mask, nsteps, adj_tol=1e-12):
+        """(J, grad_inflow[ny], grad_mask[3]) with mask = (cx, cy, R, eps)."""
+        gi = np.zeros(g.ny)
+        gm = np.zeros(3)
+        J = self.lib.orc_adjoint_drag(C.byref(g.c), C.byref(bc), C.byref(prm), ptr(u), ptr(v),
+                                      ptr(w), *mask, nsteps, adj_tol, ptr(gi), ptr(gm))
+        return J, gi, gm
+
+    def drag_objective(self, g, bc, prm, u, v, w, mask, nsteps):
+        return self.lib.orc_drag_objective(C.byref(g.c), C.byref(bc), C.byref(prm), ptr(u),
+                                           ptr(v), ptr(w), *mask, nsteps)
+
+    def adj_convdiff(self, g, dt, nu, u, v, w, lus, lvs, lws, z_periodic=0):
+        lu, lv, lw = np.zeros(g.G), np.zeros(g.G), np.zeros(g.G)
+        self.lib.orc_adj_convdiff(C.byref(g.c), dt, nu, z_periodic,
+                                  *map(ptr, (u, v, w, lus, lvs, lws, lu, lv, lw)))
+        return lu, lv, lw
+
+    def adj_bc_main(self, g, bc, lu, lv, lw, advance=0, dt=0.0):
+        """In place on (lu, lv, lw); returns the inflow adjoint (ghosted j)."""
+        li = np.zeros(g.ny)
+        self.lib.orc_adj_bc_main(C.byref(g.c), C.byref(bc), advance, dt,
+                                 *map(ptr, (lu, lv, lw, li)))
+        return li
 
     def grid(self, nxc, nyc, nzc, lx=20.0, ly=10.0, lz=0.5, kind=GRID_UNIFORM):
         return Grid(self.lib, nxc, nyc, nzc, lx, ly, lz, kind)

@@ -60,8 +60,9 @@ def test_blocked_sum_order_brute_force(o):
 
 def test_march_sum_order_brute_force(o):
     """March order of the plane-marching kernels, written out independently: block
-    b = chunk*ntile + tile, thread t sums plane by plane (8 planes per chunk) the
-    plane cells q = 1024*tile + 256c + t; block and partial folds as above."""
+    b = chunk*ntile + tile, thread t sums plane by plane (8 planes per chunk, downwards
+    in even chunks, upwards in odd ones) the plane cells q = 1024*tile + 256c + t;
+    block and partial folds as above."""
     g = o.grid(20, 13, 7)  # cyz = 91 < 1024: one tile; 3 chunks (8, 8, 4 planes)
     g2 = o.grid(9, 40, 60)  # cyz = 2400: three tiles, the last partial; 2 chunks
     rng = np.random.default_rng(2)
@@ -86,7 +87,8 @@ def test_march_sum_order_brute_force(o):
             acc = []
             for t in range(256):
                 s = 0.0
-                for i in range(chunk * 8, min(chunk * 8 + 8, gg.nxc)):
+                planes = list(range(chunk * 8, min(chunk * 8 + 8, gg.nxc)))
+                for i in (planes if chunk % 2 else planes[::-1]):
                     for c in range(4):
                         q = tile * 1024 + c * 256 + t
                         if q < cyz:
