@@ -33,11 +33,21 @@ EXPORTS = [
     "cfdmx_transform", "cfdmx_project_ib", "cfdmx_swap_t0_t3", "cfdmx_cd_cl", "cfdmx_cfl",
     "cfdmx_apply_operator", "cfdmx_step", "cfdmx_synchronize", "cfdmx_profile",
     "cfdmx_profile_read", "cfdmx_set_tolerance", "cfdmx_set_physics", "cfdmx_upload_force",
+    "cfdmx_drag_objective", "cfdmx_adjoint_drag",
 ]
 
 # kernel classes of cfdmx_profile_read
 KERNELS = {"convdiff": 0, "bc": 1, "rhs": 2, "bicg_init": 3, "bicg_A": 4, "bicg_B": 5,
            "bicg_C": 6, "bicg_check": 7, "sor_half": 8, "project": 9}
+
+
+class AdjointInfo(C.Structure):
+    _fields_ = [("forward_ms", C.c_double), ("reverse_ms", C.c_double),
+                ("forward_iters", C.c_int), ("adjoint_iters", C.c_int), ("nsteps", C.c_int),
+                ("checkpoints", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class Config(C.Structure):
@@ -109,6 +119,8 @@ def load_library(path=LIB_PATH):
     L.cfdmx_set_tolerance.argtypes = [vp, C.c_double, C.c_int]
     L.cfdmx_set_physics.argtypes = [vp, C.c_double, C.c_double]
     L.cfdmx_profile_read.argtypes = [vp, C.c_int, DP, C.POINTER(C.c_longlong)]
+    L.cfdmx_drag_objective.argtypes = [vp, C.c_int, DP]
+    L.cfdmx_adjoint_drag.argtypes = [vp, C.c_int, C.c_int, DP, DP, DP, C.POINTER(AdjointInfo)]
     for n in EXPORTS:
         if n != "cfdmx_last_error":
             getattr(L, n).restype = C.c_int
@@ -312,6 +324,25 @@ class FlowSolver:
 
     def set_tolerance(self, tol, maxit):
         self._chk(self.L.cfdmx_set_tolerance(self.h, tol, maxit), "set_tolerance")
+
+    # ---- adjoint of the time-averaged drag ----
+    def drag_objective(self, nsteps):
+        """J = mean Cd over nsteps steps from the current state (state restored)."""
+        J = C.c_double()
+        self._chk(self.L.cfdmx_drag_objective(self.h, nsteps, C.byref(J)), "drag_objective")
+        return J.value
+
+    def adjoint_drag(self, nsteps, checkpoint_every=10, mask=True, inflow=True):
+        """(J, dJ/d inflow profile (ghosted j), dJ/d (cx, cy, R), info dict)."""
+        J = C.c_double()
+        gi = np.zeros(self.ny) if inflow else None
+        gm = np.zeros(3) if mask else None
+        info = AdjointInfo()
+        self._chk(self.L.cfdmx_adjoint_drag(self.h, nsteps, checkpoint_every, C.byref(J),
+                                            _p(gi) if inflow else None,
+                                            _p(gm) if mask else None, C.byref(info)),
+                  "adjoint_drag")
+        return J.value, gi, gm, info.as_dict()
 
     def profile(self, enable):
         self._chk(self.L.cfdmx_profile(self.h, 1 if enable else 0), "profile")

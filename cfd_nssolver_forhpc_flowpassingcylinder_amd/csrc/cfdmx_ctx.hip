@@ -2,6 +2,8 @@
 #include "cfdmx.h"
 #include "cfdmx_internal.hpp"
 
+#include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -33,6 +35,26 @@ struct cfdmx_ctx {
     KState *kst32 = nullptr, *kst32_h = nullptr;
     double *mix = nullptr, *mix_h = nullptr;
     int mix_outer = 0;
+    // smoothed immersed-boundary mask (cfdmx_set_eta_smoothed), the adjoint's parameters
+    int mask_smoothed = 0;
+    double mask_cx = 0.0, mask_cy = 0.0, mask_r = 0.0, mask_eps = 0.0;
+    // adjoint of the drag: tape, checkpoints and adjoint fields (allocated on first use)
+    struct Adj {
+        int K = 0, ncp = 0;
+        std::vector<void *> mem;
+        double *tapeU = nullptr, *tapeP = nullptr, *tapeX = nullptr;
+        std::vector<int> solved;
+        double *cpU = nullptr, *cpP = nullptr, *cpX = nullptr, *cpPx1 = nullptr,
+               *cpPx2 = nullptr;
+        KState *cpK = nullptr;
+        std::vector<int> cp_loop, cp_resets;
+        std::vector<double> cp_time;
+        double *lu[3] = {}, *ls[3] = {}, *le[3] = {}, *us[3] = {};
+        double *lx = nullptr, *mu = nullptr, *mpx1 = nullptr, *mpx2 = nullptr;
+        KState *kadj = nullptr, *kadj_h = nullptr;
+        double *leta = nullptr, *Jr = nullptr, *lin = nullptr, *tin = nullptr;
+        double *apart = nullptr, *out = nullptr, *out_h = nullptr;
+    } adj;
     double *part = nullptr, *cfl_part = nullptr, *flux_part = nullptr;
     double *bpart[3] = {}; // per-phase BiCGSTAB partials (A, B, C)
     double *scal = nullptr; // [0] cfl, [1] cd, [2] cl
@@ -494,6 +516,9 @@ int cfdmx_destroy(cfdmx_ctx *ctx)
     for (hipEvent_t e : ctx->prof_pool) (void)hipEventDestroy(e);
     if (ctx->kst_h) (void)hipHostFree(ctx->kst_h);
     if (ctx->kst32_h) (void)hipHostFree(ctx->kst32_h);
+    for (void *q : ctx->adj.mem) (void)hipFree(q);
+    if (ctx->adj.kadj_h) (void)hipHostFree(ctx->adj.kadj_h);
+    if (ctx->adj.out_h) (void)hipHostFree(ctx->adj.out_h);
     if (ctx->mix_h) (void)hipHostFree(ctx->mix_h);
     if (ctx->sst_h) (void)hipHostFree(ctx->sst_h);
     if (ctx->scal_h) (void)hipHostFree(ctx->scal_h);
@@ -687,6 +712,7 @@ int cfdmx_set_eta_cylinder_z(cfdmx_ctx *ctx, double cx, double cy, double radius
 {
     if (!ctx) return CFDMX_ERR_ARG;
     if (nsub <= 0) FAIL(CFDMX_ERR_ARG, "nsub must be positive");
+    ctx->mask_smoothed = 0;
     launch_fill(ctx->stream, ctx->g.G, 0.0, ctx->eta); // RUN_CPU.cpp:228
     launch_eta_cylz(ctx->g, ctx->stream, ctx->Xn, ctx->Yn, cx, cy, radius, nsub,
                     std::pow(nsub, -2), ctx->eta);
@@ -719,6 +745,11 @@ int cfdmx_set_eta_smoothed(cfdmx_ctx *ctx, double cx, double cy, double R, doubl
         for (int j = 0; j < g.ny - 1; ++j)
             for (int k = 1; k < g.nz - 1; ++k)
                 eta[((size_t)i * g.ny + j) * g.nz + k] = eta[((size_t)i * g.ny + j) * g.nz];
+    ctx->mask_smoothed = 1;
+    ctx->mask_cx = cx;
+    ctx->mask_cy = cy;
+    ctx->mask_r = R;
+    ctx->mask_eps = eps;
     return copy_h2d(ctx, ctx->eta, eta.data(), g.G);
 }
 
@@ -784,50 +815,65 @@ int cfdmx_pressure_rhs(cfdmx_ctx *ctx)
     return CFDMX_OK;
 }
 
-static int solve_bicgstab_impl(cfdmx_ctx *ctx, int *iters, double *res)
+// One FP64 BiCGSTAB solve A x = b (bicgstabRe2.hpp:82-180) on the given vectors and
+// solver state; the Krylov work vectors and partials are shared scratch.
+struct SolveTarget {
+    const double *b;
+    double *x;
+    double **px1, **px2; // history copies (bicgstabRe2.hpp:94-95), swapped in place
+    KState *kst, *kst_h;
+};
+
+static int solve_bicgstab_core(cfdmx_ctx *ctx, const SolveTarget &T, int *iters, double *res)
 {
     hipStream_t s = ctx->stream;
     const Geo &g = ctx->g;
     const int pre = ctx->cfg.precond;
-    std::swap(ctx->px1, ctx->px2); // px2 <- px1, then px1 <- x (bicgstabRe2.hpp:94-95)
-    PROF(CFDMX_K_BICG_INIT, launch_bicg_init(g, s, pre, ctx->b, ctx->x, ctx->B.r0, ctx->B.r,
-                                             ctx->px1, ctx->part, ctx->kst));
-    PROF(CFDMX_K_BICG_CHECK, launch_bicg_fin0(g, s, ctx->part, ctx->kst));
+    std::swap(*T.px1, *T.px2); // px2 <- px1, then px1 <- x (bicgstabRe2.hpp:94-95)
+    PROF(CFDMX_K_BICG_INIT, launch_bicg_init(g, s, pre, T.b, T.x, ctx->B.r0, ctx->B.r, *T.px1,
+                                             ctx->part, T.kst));
+    PROF(CFDMX_K_BICG_CHECK, launch_bicg_fin0(g, s, ctx->part, T.kst));
     LAUNCHCHK();
     int batch = 4, m = 0;
     for (;;) {
         for (int q = 0; q < batch; ++q, ++m) {
-            PROF(CFDMX_K_BICG_A, launch_bicg_A(g, s, pre, ctx->B, ctx->bpart, ctx->kst, m));
-            PROF(CFDMX_K_BICG_B, launch_bicg_B(g, s, pre, ctx->B, ctx->bpart, ctx->kst, m));
-            PROF(CFDMX_K_BICG_C, launch_bicg_C(g, s, ctx->B, ctx->x, ctx->bpart, ctx->kst, m));
+            PROF(CFDMX_K_BICG_A, launch_bicg_A(g, s, pre, ctx->B, ctx->bpart, T.kst, m));
+            PROF(CFDMX_K_BICG_B, launch_bicg_B(g, s, pre, ctx->B, ctx->bpart, T.kst, m));
+            PROF(CFDMX_K_BICG_C, launch_bicg_C(g, s, ctx->B, T.x, ctx->bpart, T.kst, m));
         }
-        PROF(CFDMX_K_BICG_CHECK, launch_bicg_check(g, s, ctx->bpart, ctx->kst, m - 1));
+        PROF(CFDMX_K_BICG_CHECK, launch_bicg_check(g, s, ctx->bpart, T.kst, m - 1));
         LAUNCHCHK();
-        HIPCHK(hipMemcpyAsync(ctx->kst_h, ctx->kst, sizeof(KState), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(T.kst_h, T.kst, sizeof(KState), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
         prof_harvest(ctx);
-        if (ctx->kst_h->done) break;
+        if (T.kst_h->done) break;
         batch = batch < 64 ? batch * 2 : 64;
     }
-    if (ctx->kst_h->done == 2) {
+    if (T.kst_h->done == 2) {
         // NaN/Inf reset at the iteration limit (bicgstabRe2.hpp:133-153)
         launch_set_int(s, ctx->count, 0);
-        launch_nanfix(s, g.N, ctx->x, ctx->px1, ctx->count);
+        launch_nanfix(s, g.N, T.x, *T.px1, ctx->count);
         LAUNCHCHK();
         HIPCHK(hipMemcpyAsync(ctx->count_h, ctx->count, sizeof(int), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
         const int check = *ctx->count_h;
         if (check > g.N * 0.1) {
-            launch_copy(s, g.N, ctx->px2, ctx->x);
+            launch_copy(s, g.N, *T.px2, T.x);
             LAUNCHCHK();
             ++ctx->resets;
         } else if (check != 0) {
             ++ctx->resets;
         }
     }
-    if (iters) *iters = ctx->kst_h->iter;
-    if (res) *res = ctx->kst_h->res;
+    if (iters) *iters = T.kst_h->iter;
+    if (res) *res = T.kst_h->res;
     return CFDMX_OK;
+}
+
+static int solve_bicgstab_impl(cfdmx_ctx *ctx, int *iters, double *res)
+{
+    SolveTarget T = {ctx->b, ctx->x, &ctx->px1, &ctx->px2, ctx->kst, ctx->kst_h};
+    return solve_bicgstab_core(ctx, T, iters, res);
 }
 
 // Mixed precision (no reference counterpart): FP64 iterative refinement.  Each outer
@@ -1120,3 +1166,301 @@ int cfdmx_profile_read(cfdmx_ctx *ctx, int kernel, double *total_ms, long long *
 }
 
 } // extern "C"
+
+// ---------------------------------------------------------------------------
+// Adjoint of the time-averaged drag coefficient (no reference counterpart)
+// ---------------------------------------------------------------------------
+// J = (1/n) sum of Cd over n steps from the current state, with U^0 = BC_start(U).
+// The forward pass keeps a checkpoint (velocity, pressure, X_result, history vectors,
+// solver state) every K steps; the reverse pass restores each checkpoint, recomputes
+// its K steps onto a tape, and runs the transposed step kernels back through them.
+// The pressure adjoint reuses the FP64 BiCGSTAB (A is symmetric on the uniform grid)
+// with its own solution, history and state.  The context is returned to the state
+// at entry (start boundary conditions applied).
+static int adj_alloc(cfdmx_ctx *ctx, int K, int ncp)
+{
+    cfdmx_ctx::Adj &A = ctx->adj;
+    if (A.K == K && A.ncp >= ncp && A.kadj) return CFDMX_OK;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (void *q : A.mem) (void)hipFree(q);
+    A.mem.clear();
+    const size_t G = ctx->g.G, N = ctx->g.N;
+    auto al = [&](double **p, size_t n) -> int {
+        void *q = nullptr;
+        HIPCHK(hipMalloc(&q, (n > 0 ? n : 1) * sizeof(double)));
+        A.mem.push_back(q);
+        *p = static_cast<double *>(q);
+        return CFDMX_OK;
+    };
+    int rc = 0;
+    rc |= al(&A.tapeU, (size_t)K * 3 * G);
+    rc |= al(&A.tapeP, (size_t)K * G);
+    rc |= al(&A.tapeX, (size_t)K * N);
+    rc |= al(&A.cpU, (size_t)ncp * 3 * G);
+    rc |= al(&A.cpP, (size_t)ncp * G);
+    rc |= al(&A.cpX, (size_t)ncp * N);
+    rc |= al(&A.cpPx1, (size_t)ncp * N);
+    rc |= al(&A.cpPx2, (size_t)ncp * N);
+    for (int q = 0; q < 3; ++q) {
+        rc |= al(&A.lu[q], G);
+        rc |= al(&A.ls[q], G);
+        rc |= al(&A.le[q], G);
+        rc |= al(&A.us[q], G);
+    }
+    rc |= al(&A.lx, N);
+    rc |= al(&A.mu, N);
+    rc |= al(&A.mpx1, N);
+    rc |= al(&A.mpx2, N);
+    rc |= al(&A.leta, G);
+    rc |= al(&A.Jr, (size_t)63 * G);
+    rc |= al(&A.lin, (size_t)ctx->g.ny);
+    rc |= al(&A.tin, (size_t)ctx->g.cyz);
+    rc |= al(&A.apart, 4 * ((size_t)ctx->g.nblocks + 64) + 3 * ((size_t)ctx->g.cyz / kPerBlock + 2));
+    rc |= al(&A.out, 4);
+    if (rc) return CFDMX_ERR_HIP;
+    double *kp = nullptr, *kc = nullptr;
+    rc |= al(&kp, (sizeof(KState) + 7) / 8);
+    rc |= al(&kc, (size_t)ncp * ((sizeof(KState) + 7) / 8));
+    if (rc) return CFDMX_ERR_HIP;
+    A.kadj = reinterpret_cast<KState *>(kp);
+    A.cpK = reinterpret_cast<KState *>(kc);
+    if (!A.kadj_h) HIPCHK(hipHostMalloc((void **)&A.kadj_h, sizeof(KState), hipHostMallocDefault));
+    if (!A.out_h) HIPCHK(hipHostMalloc((void **)&A.out_h, 4 * sizeof(double), hipHostMallocDefault));
+    A.K = K;
+    A.ncp = ncp;
+    A.solved.assign((size_t)K, 0);
+    A.cp_loop.assign((size_t)ncp, 0);
+    A.cp_resets.assign((size_t)ncp, 0);
+    A.cp_time.assign((size_t)ncp, 0.0);
+    return CFDMX_OK;
+}
+
+static int d2d(cfdmx_ctx *ctx, double *dst, const double *src, size_t n)
+{
+    HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    return CFDMX_OK;
+}
+
+static int adj_save(cfdmx_ctx *ctx, int c)
+{
+    cfdmx_ctx::Adj &A = ctx->adj;
+    const size_t G = ctx->g.G, N = ctx->g.N;
+    int rc = 0;
+    for (int q = 0; q < 3; ++q) rc |= d2d(ctx, A.cpU + ((size_t)c * 3 + q) * G, ctx->t0[q], G);
+    rc |= d2d(ctx, A.cpP + (size_t)c * G, ctx->p, G);
+    rc |= d2d(ctx, A.cpX + (size_t)c * N, ctx->x, N);
+    rc |= d2d(ctx, A.cpPx1 + (size_t)c * N, ctx->px1, N);
+    rc |= d2d(ctx, A.cpPx2 + (size_t)c * N, ctx->px2, N);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(A.cpK + c, ctx->kst, sizeof(KState), hipMemcpyDeviceToDevice,
+                          ctx->stream));
+    A.cp_loop[c] = ctx->loop;
+    A.cp_time[c] = ctx->time;
+    A.cp_resets[c] = ctx->resets;
+    return CFDMX_OK;
+}
+
+static int adj_restore(cfdmx_ctx *ctx, int c)
+{
+    cfdmx_ctx::Adj &A = ctx->adj;
+    const size_t G = ctx->g.G, N = ctx->g.N;
+    int rc = 0;
+    for (int q = 0; q < 3; ++q) rc |= d2d(ctx, ctx->t0[q], A.cpU + ((size_t)c * 3 + q) * G, G);
+    rc |= d2d(ctx, ctx->p, A.cpP + (size_t)c * G, G);
+    rc |= d2d(ctx, ctx->x, A.cpX + (size_t)c * N, N);
+    rc |= d2d(ctx, ctx->px1, A.cpPx1 + (size_t)c * N, N);
+    rc |= d2d(ctx, ctx->px2, A.cpPx2 + (size_t)c * N, N);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(ctx->kst, A.cpK + c, sizeof(KState), hipMemcpyDeviceToDevice,
+                          ctx->stream));
+    // the boundary copies into T1 that the step after the checkpoint reads
+    launch_bc_copy(ctx->g, ctx->stream, ctx->t0[0], ctx->t0[1], ctx->t0[2], ctx->t1[0],
+                   ctx->t1[1], ctx->t1[2]);
+    LAUNCHCHK();
+    ctx->loop = A.cp_loop[c];
+    ctx->time = A.cp_time[c];
+    ctx->resets = A.cp_resets[c];
+    return CFDMX_OK;
+}
+
+static int adj_check(cfdmx_ctx *ctx, int nsteps, int K)
+{
+    const cfdmx_config &c = ctx->cfg;
+    if (nsteps < 1 || K < 1) FAIL(CFDMX_ERR_ARG, "nsteps and checkpoint_every must be >= 1");
+    if (c.grid_kind != CFDMX_GRID_UNIFORM || c.precond != CFDMX_PRECOND_NONE ||
+        c.solver != CFDMX_SOLVER_BICGSTAB || c.precision != CFDMX_PRECISION_FP64)
+        FAIL(CFDMX_ERR_ARG, "the drag adjoint needs the uniform grid, the SPE operator "
+                            "(precond none), BiCGSTAB and FP64");
+    if (!ctx->started) FAIL(CFDMX_ERR_STATE, "call cfdmx_init_flow or cfdmx_start first");
+    return CFDMX_OK;
+}
+
+// the forward path of the objective: BC_start, then n steps; Cd summed in step order
+static int adj_forward(cfdmx_ctx *ctx, int nsteps, int K, int keep_checkpoints, double *J,
+                       int *iters_total)
+{
+    hipStream_t s = ctx->stream;
+    launch_bc(ctx->g, s, ctx->bc, 0, ctx->t0[0], ctx->t0[1], ctx->t0[2], ctx->p, ctx->t1[0],
+              ctx->t1[1], ctx->t1[2], ctx->flux_part);
+    LAUNCHCHK();
+    double sum = 0.0;
+    int it_tot = 0;
+    for (int n = 0; n < nsteps; ++n) {
+        if (keep_checkpoints && n % K == 0) {
+            int rc = adj_save(ctx, n / K);
+            if (rc) return rc;
+        }
+        int it = 0, solved = 0;
+        double err = 0.0;
+        int rc = step_once(ctx, &it, &err, &solved);
+        if (!rc) rc = read_scalars(ctx);
+        if (rc) return rc;
+        sum += ctx->scal_h[1];
+        it_tot += it;
+    }
+    *J = sum / nsteps;
+    if (iters_total) *iters_total = it_tot;
+    return CFDMX_OK;
+}
+
+int cfdmx_drag_objective(cfdmx_ctx *ctx, int nsteps, double *J)
+{
+    if (!ctx || !J) return CFDMX_ERR_ARG;
+    int rc = adj_check(ctx, nsteps, 1);
+    if (!rc) rc = adj_alloc(ctx, 1, 1);
+    if (!rc) rc = adj_save(ctx, 0);
+    if (!rc) rc = adj_forward(ctx, nsteps, 1, 0, J, nullptr);
+    if (!rc) rc = adj_restore(ctx, 0);
+    if (!rc) HIPCHK(hipStreamSynchronize(ctx->stream));
+    return rc;
+}
+
+int cfdmx_adjoint_drag(cfdmx_ctx *ctx, int nsteps, int checkpoint_every, double *J,
+                       double *grad_inflow, double *grad_mask, cfdmx_adjoint_info *info)
+{
+    if (!ctx || !J) return CFDMX_ERR_ARG;
+    const int K = checkpoint_every;
+    int rc = adj_check(ctx, nsteps, K);
+    if (rc) return rc;
+    if (grad_mask && !ctx->mask_smoothed)
+        FAIL(CFDMX_ERR_STATE, "mask gradients need the smoothed mask (cfdmx_set_eta_smoothed)");
+    const int ncp = (nsteps + K - 1) / K;
+    rc = adj_alloc(ctx, K, ncp + 1);
+    if (rc) return rc;
+    cfdmx_ctx::Adj &A = ctx->adj;
+    hipStream_t s = ctx->stream;
+    const Geo &g = ctx->g;
+    const cfdmx_config &c = ctx->cfg;
+    const size_t G = g.G, N = g.N;
+    HIPCHK(hipStreamSynchronize(s));
+    auto t0 = std::chrono::steady_clock::now();
+    // entry state in the extra checkpoint slot ncp
+    rc = adj_save(ctx, ncp);
+    if (rc) return rc;
+    int fwd_iters = 0;
+    rc = adj_forward(ctx, nsteps, K, 1, J, &fwd_iters);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(s));
+    auto t1 = std::chrono::steady_clock::now();
+
+    // reverse pass
+    for (int q = 0; q < 3; ++q) launch_fill(s, (int)G, 0.0, A.lu[q]);
+    launch_fill(s, (int)G, 0.0, A.leta);
+    launch_fill(s, g.ny, 0.0, A.lin);
+    launch_fill(s, (int)N, 0.0, A.mu);
+    launch_fill(s, (int)N, 0.0, A.mpx1);
+    launch_fill(s, (int)N, 0.0, A.mpx2);
+    LAUNCHCHK();
+    KState ks = {};
+    ks.tol = c.tol;
+    ks.restart_factor = 1e-7;
+    ks.maxit = c.maxit;
+    ks.done = 1;
+    *A.kadj_h = ks;
+    HIPCHK(hipMemcpyAsync(A.kadj, A.kadj_h, sizeof ks, hipMemcpyHostToDevice, s));
+    double area = 0.0;
+    for (int j = kGC; j < g.ny - kGC; ++j)
+        for (int k = kGC; k < g.nz - kGC; ++k) area += ctx->Dy[j] * ctx->Dz[k];
+    const double wf = (-2.0 / c.lz) / nsteps;
+    const double nu = 1.0 / c.re;
+    int adj_iters = 0;
+    for (int seg = ncp - 1; seg >= 0; --seg) {
+        rc = adj_restore(ctx, seg);
+        if (rc) return rc;
+        const int n0 = seg * K, n1 = std::min(n0 + K, nsteps);
+        for (int n = n0; n < n1; ++n) {
+            const int slot = n - n0;
+            for (int q = 0; q < 3; ++q)
+                if ((rc = d2d(ctx, A.tapeU + ((size_t)slot * 3 + q) * G, ctx->t0[q], G))) return rc;
+            if ((rc = d2d(ctx, A.tapeP + (size_t)slot * G, ctx->p, G))) return rc;
+            int it = 0, solved = 0;
+            double err = 0.0;
+            if ((rc = step_once(ctx, &it, &err, &solved))) return rc;
+            if ((rc = d2d(ctx, A.tapeX + (size_t)slot * N, ctx->x, N))) return rc;
+            A.solved[slot] = solved;
+        }
+        for (int n = n1 - 1; n >= n0; --n) {
+            const int slot = n - n0;
+            double *U0[3];
+            for (int q = 0; q < 3; ++q) U0[q] = A.tapeU + ((size_t)slot * 3 + q) * G;
+            // boundary conditions at the end of step n
+            launch_adj_bc(g, s, ctx->bc, 1, A.lu[0], A.lu[1], A.lu[2], A.tin, A.lin, A.apart,
+                          1.0 / area);
+            // U* of step n, recomputed from U^{n-1}
+            for (int q = 0; q < 3; ++q)
+                if ((rc = d2d(ctx, A.us[q], U0[q], G))) return rc;
+            launch_convdiff(g, s, U0[0], U0[1], U0[2], A.us[0], A.us[1], A.us[2], c.dt, nu,
+                            ctx->cfl_part);
+            // projection and forcing
+            launch_adj_project(g, s, c.dt, A.solved[slot], c.outflow_convective, wf,
+                               A.tapeX + (size_t)slot * N, A.tapeP + (size_t)slot * G, ctx->eta,
+                               A.us, A.lu, A.ls, A.le, A.lx, A.leta);
+            LAUNCHCHK();
+            if (A.solved[slot]) {
+                // pressure: A mu = lx, mean-free, then the right-hand side's transpose
+                SolveTarget T = {A.lx, A.mu, &A.mpx1, &A.mpx2, A.kadj, A.kadj_h};
+                int it = 0;
+                double res = 0.0;
+                if ((rc = solve_bicgstab_core(ctx, T, &it, &res))) return rc;
+                adj_iters += it;
+                launch_adj_remove_mean(g, s, A.mu, A.apart);
+                launch_adj_rhs(g, s, c.dt, A.mu, A.ls);
+            }
+            if (g.zper) launch_adj_periodic_interior(g, s, A.ls);
+            const double *Uc[3] = {U0[0], U0[1], U0[2]};
+            launch_adj_convdiff(g, s, c.dt, nu, Uc, A.Jr, A.ls, A.lu);
+            LAUNCHCHK();
+        }
+    }
+    // U^0 = BC_start(U)
+    launch_adj_bc(g, s, ctx->bc, 0, A.lu[0], A.lu[1], A.lu[2], A.tin, A.lin, A.apart,
+                  1.0 / area);
+    if (grad_mask)
+        launch_adj_mask(g, s, ctx->Xn, ctx->Yn, ctx->mask_cx, ctx->mask_cy, ctx->mask_r,
+                        ctx->mask_eps, A.leta, A.apart, A.out);
+    LAUNCHCHK();
+    if (grad_inflow) {
+        if ((rc = copy_d2h(ctx, grad_inflow, A.lin, g.ny))) return rc;
+        if (!ctx->bc.inflow)
+            for (int j = 0; j < g.ny; ++j) grad_inflow[j] = 0.0;
+    }
+    if (grad_mask) {
+        HIPCHK(hipMemcpyAsync(A.out_h, A.out, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        for (int q = 0; q < 3; ++q) grad_mask[q] = A.out_h[q];
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    auto t2 = std::chrono::steady_clock::now();
+    rc = adj_restore(ctx, ncp);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(s));
+    if (info) {
+        info->forward_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        info->reverse_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
+        info->forward_iters = fwd_iters;
+        info->adjoint_iters = adj_iters;
+        info->nsteps = nsteps;
+        info->checkpoints = ncp;
+    }
+    return CFDMX_OK;
+}

@@ -106,4 +106,22 @@ void launch_init_flow(const Geo &g, hipStream_t s, double u0, double amp,
 void launch_transform(const Geo &g, hipStream_t s, const double *x, double *p);
 void launch_set_int(hipStream_t s, int *a, int val);
 
+// adjoint of the drag (cfdmx_adjoint.hip)
+void launch_adj_bc(const Geo &g, hipStream_t s, const BCParams &bc, int advance, double *lu,
+                   double *lv, double *lw, double *tin, double *lin, double *part, double rarea);
+void launch_adj_project(const Geo &g, hipStream_t s, double dt, int use_x, int convective,
+                        double wf, const double *x, const double *p, const double *eta,
+                        double *const us[3], double *const l3[3], double *const ls[3],
+                        double *const le[3], double *lx, double *leta);
+void launch_adj_rhs(const Geo &g, hipStream_t s, double dt, const double *lb,
+                    double *const ls[3]);
+void launch_adj_remove_mean(const Geo &g, hipStream_t s, double *a, double *part);
+void launch_adj_periodic_interior(const Geo &g, hipStream_t s, double *const l[3]);
+void launch_adj_convdiff(const Geo &g, hipStream_t s, double dt, double nu,
+                         const double *const U[3], double *Jr, double *const ls[3],
+                         double *const lu[3]);
+void launch_adj_mask(const Geo &g, hipStream_t s, const double *Xn, const double *Yn, double cx,
+                     double cy, double R, double eps, const double *leta, double *part,
+                     double *out);
+
 } // namespace cfdmx

@@ -1,0 +1,112 @@
+"""GPU adjoint of the time-averaged drag (cfdmx_adjoint_drag) against the CPU reference
+adjoint (oracle/cfd_adjoint.c, itself checked against finite differences of the
+forward oracle), against central finite differences of the GPU forward map, and
+through the PyTorch autograd wrapper.  No reference counterpart exists."""
+import numpy as np
+import pytest
+import torch
+
+import cfd_nssolver_forhpc_flowpassingcylinder_amd as pkg
+import oracle_py as op
+from cfd_nssolver_forhpc_flowpassingcylinder_amd.autograd import drag_objective
+
+pytestmark = pytest.mark.gpu
+
+SEED = 20261202
+
+
+def _pair(nxc, nyc, nzc, nsteps, skip_first=1, tol=1e-13, re=100.0, dt=0.002):
+    rng = np.random.default_rng(7)
+    inflow = np.ones(nyc + 4)
+    inflow[2:-2] += 0.01 * (2 * rng.random(nyc) - 1)
+    eps = 1.5 * 20.0 / nxc
+    mask = (5.0, 5.0, 1.0, eps)
+    cfg = pkg.default_config(nx=nxc, ny=nyc, nz=nzc, lx=20.0, ly=10.0, lz=0.5, re=re, dt=dt,
+                             tol=tol, maxit=5000, bc_num=op.SLIP_NUM, bc_dir=op.SLIP_DIR,
+                             z_periodic=1, outflow_convective=1, convective_speed=1.0,
+                             flux_correction=1, skip_first_pressure=skip_first)
+    gpu = pkg.FlowSolver(cfg)
+    gpu.set_inflow(inflow)
+    gpu.init_flow(1.0, 1e-3, SEED)
+    gpu.set_eta_smoothed(*mask)
+    o = op.Oracle()
+    g = o.grid(nxc, nyc, nzc)
+    bc = op.make_bc(op.SLIP_NUM, op.SLIP_DIR, inflow=inflow, z_periodic=1,
+                    outflow_convective=1, conv_speed=1.0, flux_correction=1)
+    prm = op.OrcParams(dt, 1.0 / re, op.SOLVER_BICGSTAB, op.PRECOND_NONE, tol, 5000, 1.8,
+                       op.RED_BLOCKED, skip_first, 1)
+    U = o.init_velocity(g, 1.0, 1e-3, SEED)
+    return gpu, o, g, bc, prm, U, mask, inflow
+
+
+@pytest.mark.parametrize("skip_first,K", [(1, 2), (0, 3)])
+def test_adjoint_matches_cpu_reference(skip_first, K):
+    nsteps = 4
+    gpu, o, g, bc, prm, U, mask, inflow = _pair(24, 12, 4, nsteps, skip_first)
+    J, gi, gm, info = gpu.adjoint_drag(nsteps, K)
+    Jo, gio, gmo = o.adjoint_drag(g, bc, prm, *U, mask, nsteps)
+    print(f"J gpu {J!r} oracle {Jo!r}; info {info}")
+    assert J == Jo  # the forward passes are bit-identical
+    assert info["checkpoints"] == -(-nsteps // K)
+    for q in range(3):
+        print(f"mask[{q}]: gpu {gm[q]:.12e} cpu {gmo[q]:.12e}")
+        assert abs(gm[q] - gmo[q]) <= 1e-8 * max(abs(gmo[q]), 1e-3 * abs(gmo[2]))
+    scale = np.max(np.abs(gio))
+    assert np.max(np.abs(gi - gio)) <= 1e-8 * scale, np.max(np.abs(gi - gio)) / scale
+    gpu.close()
+
+
+def test_adjoint_against_gpu_finite_differences_and_state():
+    nsteps, K = 6, 4
+    gpu, o, g, bc, prm, U, mask, inflow = _pair(48, 24, 8, nsteps)
+    u_before = gpu.download_velocity(0)
+    p_before = gpu.pressure()
+    J, gi, gm, info = gpu.adjoint_drag(nsteps, K)
+    print("adjoint info", info)
+    # the context is back at its entry state
+    for a, b in zip(gpu.download_velocity(0), u_before):
+        assert np.array_equal(a, b)
+    assert np.array_equal(gpu.pressure(), p_before)
+    assert gpu.drag_objective(nsteps) == J
+    h = 1e-6
+    cx, cy, r, eps = mask
+    gpu.set_eta_smoothed(cx, cy, r + h, eps)
+    jp = gpu.drag_objective(nsteps)
+    gpu.set_eta_smoothed(cx, cy, r - h, eps)
+    jm = gpu.drag_objective(nsteps)
+    gpu.set_eta_smoothed(*mask)
+    fd = (jp - jm) / (2 * h)
+    print(f"dJ/dR adjoint {gm[2]:.10e} FD {fd:.10e}")
+    assert abs(gm[2] - fd) <= 1e-5 * abs(fd)
+    for j in (2, 12, 25):
+        prof = inflow.copy()
+        prof[j] += h
+        gpu.set_inflow(prof)
+        jp = gpu.drag_objective(nsteps)
+        prof[j] -= 2 * h
+        gpu.set_inflow(prof)
+        jm = gpu.drag_objective(nsteps)
+        gpu.set_inflow(inflow)
+        fd = (jp - jm) / (2 * h)
+        print(f"dJ/du_in[{j}] adjoint {gi[j]:.10e} FD {fd:.10e}")
+        assert abs(gi[j] - fd) <= 1e-5 * np.max(np.abs(gi))
+    gpu.close()
+
+
+def test_torch_autograd_wrapper():
+    nsteps, K = 3, 2
+    gpu, o, g, bc, prm, U, mask, inflow = _pair(24, 12, 4, nsteps)
+    eps = mask[3]
+    inf_t = torch.tensor(inflow[2:-2], dtype=torch.float64, requires_grad=True)
+    m_t = torch.tensor(mask[:3], dtype=torch.float64, requires_grad=True)
+    J = drag_objective(inf_t, m_t, gpu, nsteps, K, eps)
+    J.backward()
+    Jd, gi, gm, _ = gpu.adjoint_drag(nsteps, K)
+    assert float(J) == Jd
+    assert np.array_equal(inf_t.grad.numpy(), gi[2:-2])
+    assert np.array_equal(m_t.grad.numpy(), gm)
+    # torch's own central differences on the mask parameters
+    f = lambda m: drag_objective(inf_t.detach(), m, gpu, nsteps, K, eps)
+    assert torch.autograd.gradcheck(f, (m_t.detach().clone().requires_grad_(True),),
+                                    eps=1e-6, atol=1e-5, rtol=1e-5)
+    gpu.close()
