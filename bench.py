@@ -41,12 +41,12 @@ BYTES_PER_CELL = {
     "sor_half": 24,  # r p (all), rhs (half);  w p (half)
 }
 
-MIXED_INNER_TOL = 1e-3
+MIXED_INNER_TOL = 1e-5
 
 # the same phases on float Krylov vectors (mixed-precision inner solve); the FP64
 # residual/update passes of the refinement are counted under bicg_init
 BYTES_PER_CELL_FP32 = {
-    "bicg_A": 20, "bicg_B": 12, "bicg_C": 36, "convdiff": 48, "rhs": 32, "project": 96,
+    "convdiff": 48, "rhs": 32, "project": 96,
 }
 
 WORKLOADS = {
@@ -197,6 +197,57 @@ def run_mode(wl, args, precision):
     return s, warm, infos, wall, prof
 
 
+# gradients of the time-averaged drag: 256 x 128 x 32, Re 100, 50 steps, checkpoints
+# every 10 steps, 128-point inflow profile 1 +- 0.01 (seeded), smoothed mask of width
+# 1.5 cells
+ADJOINT_CASE = dict(nx=256, ny=128, nz=32, re=100.0, dt=0.002, tol=1e-10, nsteps=50, K=10)
+
+
+def adjoint_run(seed):
+    import numpy as np
+    import cfd_nssolver_forhpc_flowpassingcylinder_amd as pkg
+    a = ADJOINT_CASE
+    slip_num = [[0, 0, 0], [1, 1, 1], [1, 0, 1], [1, 0, 1], [1, 1, 1], [1, 1, 1]]
+    slip_dir = [[1, 0, 0], [0, 0, 0], [0, 0, 0], [0, 0, 0], [0, 0, 0], [0, 0, 0]]
+    cfg = pkg.default_config(
+        nx=a["nx"], ny=a["ny"], nz=a["nz"], lx=20.0, ly=10.0, lz=0.5, re=a["re"], dt=a["dt"],
+        tol=a["tol"], maxit=5000, bc_num=slip_num, bc_dir=slip_dir, z_periodic=1,
+        outflow_convective=1, convective_speed=1.0, flux_correction=1, skip_first_pressure=1)
+    s = pkg.FlowSolver(cfg)
+    rng = np.random.default_rng(seed)
+    prof = np.ones(a["ny"] + 4)
+    prof[2:-2] += 0.01 * (2.0 * rng.random(a["ny"]) - 1.0)
+    s.set_inflow(prof)
+    s.init_flow(1.0, 1e-3, seed)
+    eps = 1.5 * 20.0 / a["nx"]
+    s.set_eta_smoothed(5.0, 5.0, 0.5, eps)
+    t = time.perf_counter()
+    J0 = s.drag_objective(a["nsteps"])
+    plain = time.perf_counter() - t
+    print(f"adjoint case: plain forward {plain:.2f} s", file=sys.stderr, flush=True)
+    J, gi, gm, info = s.adjoint_drag(a["nsteps"], a["K"])
+    print(f"adjoint case: checkpointed forward {info['forward_ms'] / 1e3:.2f} s, reverse "
+          f"{info['reverse_ms'] / 1e3:.2f} s", file=sys.stderr, flush=True)
+    s.close()
+    n = a["nsteps"]
+    return {
+        "config": (f"{a['nx']}x{a['ny']}x{a['nz']}, Re={a['re']:g}, dt={a['dt']:g}, "
+                   f"{n} steps, checkpoint every {a['K']}, tol {a['tol']:g}, FP64; objective "
+                   "mean Cd; parameters: 128-point inflow profile, cylinder (cx, cy, R) of a "
+                   "smoothed mask 1.5 cells wide"),
+        "J": J, "J_plain_forward": J0,
+        "grad_mask": {"cx": gm[0], "cy": gm[1], "R": gm[2]},
+        "grad_inflow_l2": float(np.linalg.norm(gi)),
+        "forward_ms_per_step": round(plain * 1e3 / n, 3),
+        "checkpointed_forward_ms_per_step": round(info["forward_ms"] / n, 3),
+        "reverse_ms_per_step": round(info["reverse_ms"] / n, 3),
+        "adjoint_step_cost": round(info["reverse_ms"] / (plain * 1e3), 3),
+        "adjoint_step_cost_note": ("wall(recompute + adjoint step) / wall(forward step); "
+                                   "the reverse pass re-runs each checkpointed segment"),
+        "forward_iters": info["forward_iters"], "adjoint_iters": info["adjoint_iters"],
+    }
+
+
 def kernel_table(prof, n_cells, bytes_per_cell):
     return {k: {"ms_total": round(v[0], 3), "launches": v[1],
                 "avg_us": round(v[0] * 1e3 / v[1], 2) if v[1] else None,
@@ -215,6 +266,8 @@ def main():
     ap.add_argument("--solver", default="bicgstab", choices=["bicgstab", "rbsor"])
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the drag-adjoint run (256x128x32, 50 steps)")
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the mixed-precision run (FP32 inner BiCGSTAB, FP64 refinement)")
     ap.add_argument("--cpu-budget", type=float, default=25.0)
@@ -296,6 +349,13 @@ def main():
         sm.close()
         s = None
 
+    adjoint = None
+    if not args.no_adjoint and args.workload == "cylinder512":
+        if s is not None:
+            s.close()
+            s = None
+        adjoint = adjoint_run(args.seed)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(wl, mean_iters, args.cpu_budget)
@@ -331,6 +391,7 @@ def main():
         "cpu_baseline": cpu,
         "kernels": kernels,
         "mixed": mixed,
+        "adjoint": adjoint,
     }
     print(json.dumps(line), flush=True)
     if s is not None:

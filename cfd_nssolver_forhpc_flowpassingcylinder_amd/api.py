@@ -38,7 +38,7 @@ EXPORTS = [
 
 # kernel classes of cfdmx_profile_read
 KERNELS = {"convdiff": 0, "bc": 1, "rhs": 2, "bicg_init": 3, "bicg_A": 4, "bicg_B": 5,
-           "bicg_C": 6, "bicg_check": 7, "sor_half": 8, "project": 9}
+           "bicg_C": 6, "bicg_check": 7, "sor_half": 8, "project": 9, "mg": 10, "pb": 11}
 
 
 class AdjointInfo(C.Structure):

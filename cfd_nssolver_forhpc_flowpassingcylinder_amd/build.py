@@ -12,7 +12,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libcfdmx.so")
-SOURCES = ["cfdmx_kernels.hip", "cfdmx_adjoint.hip", "cfdmx_ctx.hip"]
+SOURCES = ["cfdmx_kernels.hip", "cfdmx_adjoint.hip", "cfdmx_mg.hip", "cfdmx_ctx.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("CFDMX_ARCH", "gfx950")
 

@@ -28,11 +28,14 @@ struct cfdmx_ctx {
     double *p = nullptr, *eta = nullptr, *f = nullptr;
     double *x = nullptr, *b = nullptr, *px1 = nullptr, *px2 = nullptr;
     BicgBuffers B = {};
-    // mixed precision: FP32 Krylov vectors and correction, inner-solve state, the
-    // three folded sums of the FP64 residual
-    BicgBuf<float> B32 = {};
+    // mixed precision: FP32 vectors of the preconditioned inner BiCGSTAB (r, r0, p, v,
+    // s, t, p^, s^) and the correction e, the multigrid levels, the inner-solve state,
+    // the three folded sums of the FP64 residual
+    float *pb[8] = {};
     float *e32 = nullptr;
-    KState *kst32 = nullptr, *kst32_h = nullptr;
+    std::vector<MGLevel> mg;
+    PBState *pbs = nullptr, *pbs_h = nullptr;
+    double *pb_part = nullptr;
     double *mix = nullptr, *mix_h = nullptr;
     int mix_outer = 0;
     // smoothed immersed-boundary mask (cfdmx_set_eta_smoothed), the adjoint's parameters
@@ -374,15 +377,64 @@ static int setup_device(cfdmx_ctx *ctx)
     HIPCHK(hipHostMalloc((void **)&ctx->scal_h, 16 * sizeof(double), hipHostMallocDefault));
     HIPCHK(hipHostMalloc((void **)&ctx->count_h, sizeof(int), hipHostMallocDefault));
     if (c.precision == CFDMX_PRECISION_MIXED) {
-        BicgBuf<float> &F = ctx->B32;
-        float **fz[] = {&F.r0, &F.r, &F.as, &F.p[0], &F.p[1], &F.ap[0], &F.ap[1], &ctx->e32};
-        for (float **z : fz) {
-            HIPCHK(hipMalloc((void **)z, N * sizeof(float)));
+        auto falloc = [&](float **z, size_t n) -> int {
+            HIPCHK(hipMalloc((void **)z, (n > 0 ? n : 1) * sizeof(float)));
             ctx->allocs.push_back(*z);
+            return CFDMX_OK;
+        };
+        for (int q = 0; q < 8; ++q)
+            if (int rc = falloc(&ctx->pb[q], N)) return rc;
+        if (int rc = falloc(&ctx->e32, N)) return rc;
+        // multigrid levels: level 0 is the SPE operator of the uniform grid
+        MGLevel L = {};
+        L.nx = g.nxc;
+        L.ny = g.nyc;
+        L.nz = g.nzc;
+        L.n = (int)N;
+        double h[3] = {ctx->Dx[kGC], ctx->Dy[kGC], ctx->Dz[kGC]};
+        L.cx = (float)(1.0 / (ctx->Dx[kGC] * ctx->Dxs[kGC]));
+        L.cy = (float)(1.0 / (ctx->Dy[kGC] * ctx->Dys[kGC]));
+        L.cz = (float)(1.0 / (ctx->Dz[kGC] * ctx->Dzs[kGC]));
+        L.fx = L.fy = L.fz = 1;
+        L.zper = g.zper;
+        ctx->mg.clear();
+        ctx->mg.push_back(L);
+        for (;;) {
+            const MGLevel &P = ctx->mg.back();
+            if (P.n <= 512) break;
+            const double hmin = std::min(h[0], std::min(h[1], h[2]));
+            const int fx = (P.nx % 2 == 0 && P.nx >= 4 && h[0] <= 2.0 * hmin) ? 2 : 1;
+            const int fy = (P.ny % 2 == 0 && P.ny >= 4 && h[1] <= 2.0 * hmin) ? 2 : 1;
+            const int fz = (P.nz % 2 == 0 && P.nz >= 2 && h[2] <= 2.0 * hmin) ? 2 : 1;
+            if (fx * fy * fz == 1) break;
+            MGLevel C = {};
+            C.nx = P.nx / fx;
+            C.ny = P.ny / fy;
+            C.nz = P.nz / fz;
+            C.n = C.nx * C.ny * C.nz;
+            h[0] *= fx;
+            h[1] *= fy;
+            h[2] *= fz;
+            C.cx = (float)(1.0 / (h[0] * h[0]));
+            C.cy = (float)(1.0 / (h[1] * h[1]));
+            C.cz = (float)(1.0 / (h[2] * h[2]));
+            C.fx = fx;
+            C.fy = fy;
+            C.fz = fz;
+            C.zper = g.zper;
+            if (int rc = falloc(&C.u, (size_t)C.n)) return rc;
+            if (int rc = falloc(&C.f, (size_t)C.n)) return rc;
+            ctx->mg.push_back(C);
         }
-        ALLOC(ctx->kst32, 1);
+        if (ctx->mg.back().n > 512)
+            FAIL(CFDMX_ERR_ARG, "mixed precision: the multigrid hierarchy of %dx%dx%d stops at "
+                                "%d cells (the coarse solve takes at most 512); use sizes "
+                                "with more factors of two", g.nxc, g.nyc, g.nzc,
+                 ctx->mg.back().n);
+        ALLOC(ctx->pbs, 1);
+        ALLOC(ctx->pb_part, 2 * (size_t)g.nblocks + 64);
         ALLOC(ctx->mix, 4);
-        HIPCHK(hipHostMalloc((void **)&ctx->kst32_h, sizeof(KState), hipHostMallocDefault));
+        HIPCHK(hipHostMalloc((void **)&ctx->pbs_h, sizeof(PBState), hipHostMallocDefault));
         HIPCHK(hipHostMalloc((void **)&ctx->mix_h, 4 * sizeof(double), hipHostMallocDefault));
     }
 
@@ -447,9 +499,9 @@ static int validate(cfdmx_ctx *ctx)
             FAIL(CFDMX_ERR_ARG, "mixed precision needs the BiCGSTAB solver");
         if (!(c.inner_tol > 0.0 && c.inner_tol < 1.0))
             FAIL(CFDMX_ERR_ARG, "inner_tol must be in (0, 1)");
-        if (c.grid_kind != CFDMX_GRID_UNIFORM)
-            FAIL(CFDMX_ERR_ARG, "mixed precision needs the uniform grid (on the stretched SML "
-                                "grid the FP32 inner solve stalls)");
+        if (c.grid_kind != CFDMX_GRID_UNIFORM || c.precond != CFDMX_PRECOND_NONE)
+            FAIL(CFDMX_ERR_ARG, "mixed precision needs the uniform grid and the SPE operator "
+                                "(precond none): its multigrid levels are uniform Laplacians");
     }
     if (c.nz > 600) FAIL(CFDMX_ERR_ARG, "nz above 600 is not supported (LDS ring of the stencil)");
     if (c.z_periodic && c.solver == CFDMX_SOLVER_RBSOR && (c.nz % 2) != 0)
@@ -497,7 +549,7 @@ int cfdmx_config_default(cfdmx_config *cfg)
     cfg->sor_omega = 1.8; // p_sor_omega (0_1_SimulationVariable.hpp:65)
     cfg->skip_first_pressure = 1;
     cfg->precision = CFDMX_PRECISION_FP64;
-    cfg->inner_tol = 1e-3;
+    cfg->inner_tol = 1e-5;
     cfg->xcd_remap = 1;
     return CFDMX_OK;
 }
@@ -515,7 +567,7 @@ int cfdmx_destroy(cfdmx_ctx *ctx)
     prof_harvest(ctx);
     for (hipEvent_t e : ctx->prof_pool) (void)hipEventDestroy(e);
     if (ctx->kst_h) (void)hipHostFree(ctx->kst_h);
-    if (ctx->kst32_h) (void)hipHostFree(ctx->kst32_h);
+    if (ctx->pbs_h) (void)hipHostFree(ctx->pbs_h);
     for (void *q : ctx->adj.mem) (void)hipFree(q);
     if (ctx->adj.kadj_h) (void)hipHostFree(ctx->adj.kadj_h);
     if (ctx->adj.out_h) (void)hipHostFree(ctx->adj.out_h);
@@ -878,25 +930,67 @@ static int solve_bicgstab_impl(cfdmx_ctx *ctx, int *iters, double *res)
 
 // Mixed precision (no reference counterpart): FP64 iterative refinement.  Each outer
 // pass forms r = b - A x in FP64 and stops when ||r|| / ||b|| < tol; otherwise an FP32
-// BiCGSTAB (the same phase kernels on float vectors, FP64 sums) solves A e = r to the
-// relative tolerance max(inner_tol, 0.3 tol ||b|| / ||r||) and x += e in FP64.  The
+// BiCGSTAB preconditioned by an FP32 multigrid V-cycle (FP64 sums) solves A e = r to
+// the relative tolerance max(inner_tol, 0.3 tol ||b|| / ||r||) and x += e in FP64.  The
 // iteration limit bounds the inner iterations summed over the outer passes.  A pass
 // that raises the FP64 residual (a stalled FP32 Krylov process) is taken back and
 // ends the solve with the previous x.
 static constexpr int kMaxOuter = 40;
+
+// FP32 BiCGSTAB with the multigrid V-cycle as right preconditioner on A e = r:
+// vectors r, r0, p, v, s, t, p^, s^ (pb[0..7]), solution e; returns the iterations
+static int solve_pb_inner(cfdmx_ctx *ctx, double rr, double tol, int maxit, int *iters,
+                          double *res)
+{
+    hipStream_t s = ctx->stream;
+    const Geo &g = ctx->g;
+    float *r = ctx->pb[0], *r0 = ctx->pb[1], *p = ctx->pb[2], *v = ctx->pb[3],
+          *sv = ctx->pb[4], *t = ctx->pb[5], *ph = ctx->pb[6], *sh = ctx->pb[7];
+    PBState ps = {};
+    ps.rho = rr; // (r, r0) with r0 = r
+    ps.norm0 = std::sqrt(rr);
+    ps.tol = tol;
+    ps.maxit = maxit;
+    ps.done = (ps.norm0 > 0.0 && maxit > 0) ? 0 : 1;
+    *ctx->pbs_h = ps;
+    HIPCHK(hipMemcpyAsync(ctx->pbs, ctx->pbs_h, sizeof ps, hipMemcpyHostToDevice, s));
+    std::vector<MGLevel> &L = ctx->mg;
+    const int nlev = (int)L.size();
+    for (int it = 0; !ctx->pbs_h->done; ++it) {
+        PROF(CFDMX_K_PB, launch_pb_update_p(g, s, it == 0, r, v, p, ctx->pbs));
+        L[0].f = p;
+        L[0].u = ph;
+        PROF(CFDMX_K_MG, launch_mg_vcycle(s, L.data(), nlev, 0));
+        PROF(CFDMX_K_PB, launch_pb_spmv(g, s, L[0], 1, ph, v, r0, ctx->part, ctx->pbs));
+        PROF(CFDMX_K_PB, launch_pb_update_s(g, s, ctx->part, r, v, sv, ctx->pbs));
+        L[0].f = sv;
+        L[0].u = sh;
+        PROF(CFDMX_K_MG, launch_mg_vcycle(s, L.data(), nlev, 0));
+        PROF(CFDMX_K_PB, launch_pb_spmv(g, s, L[0], 2, sh, t, sv, ctx->part, ctx->pbs));
+        PROF(CFDMX_K_PB, launch_pb_update_xr(g, s, ctx->part, ph, sh, sv, t, r0, ctx->e32, r,
+                                             ctx->pb_part, ctx->pbs));
+        launch_pb_end(g, s, ctx->pb_part, ctx->pbs);
+        LAUNCHCHK();
+        HIPCHK(hipMemcpyAsync(ctx->pbs_h, ctx->pbs, sizeof(PBState), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        prof_harvest(ctx);
+    }
+    *iters = ctx->pbs_h->iter;
+    *res = ctx->pbs_h->res;
+    return CFDMX_OK;
+}
 
 static int solve_mixed_impl(cfdmx_ctx *ctx, int *iters, double *res)
 {
     hipStream_t s = ctx->stream;
     const Geo &g = ctx->g;
     const cfdmx_config &c = ctx->cfg;
-    const int pre = c.precond;
     int total = 0, outer = 0;
     double res64 = 0.0, res_prev = 0.0;
     for (;;) {
         if (outer > 0) PROF(CFDMX_K_BICG_INIT, launch_mixed_update(g, s, 1.0, ctx->x, ctx->e32));
-        PROF(CFDMX_K_BICG_INIT, launch_mixed_resid(g, s, pre, ctx->b, ctx->x, ctx->B32.r,
-                                                   ctx->B32.r0, ctx->e32, ctx->part));
+        PROF(CFDMX_K_BICG_INIT, launch_mixed_resid(g, s, 0, ctx->b, ctx->x, ctx->pb[0],
+                                                   ctx->pb[1], ctx->e32, ctx->part));
         launch_mixed_fold(g, s, ctx->part, ctx->mix);
         LAUNCHCHK();
         HIPCHK(hipMemcpyAsync(ctx->mix_h, ctx->mix, 3 * sizeof(double), hipMemcpyDeviceToHost,
@@ -914,40 +1008,14 @@ static int solve_mixed_impl(cfdmx_ctx *ctx, int *iters, double *res)
         res_prev = res64;
         if (!(res64 >= c.tol)) break; // converged (or NaN: nothing left to refine)
         if (total >= c.maxit || outer >= kMaxOuter) break;
-        KState ks = {};
-        ks.rho[0] = ctx->mix_h[2]; // (r32, r0) with r0 = r32
-        ks.norm0 = std::sqrt(ctx->mix_h[2]);
-        ks.tol = std::fmax(c.inner_tol, 0.3 * c.tol / res64);
-        ks.restart_factor = 1e-7;
-        ks.maxit = c.maxit - total;
-        ks.mode = 2;
-        ks.timestep = 0; // no restarts in the inner solve
-        ks.done = ks.norm0 > 0.0 ? 0 : 1;
-        *ctx->kst32_h = ks;
-        HIPCHK(hipMemcpyAsync(ctx->kst32, ctx->kst32_h, sizeof ks, hipMemcpyHostToDevice, s));
-        ctx->B32.cur = 0;
-        int batch = 4, m = 0;
-        for (;;) {
-            for (int q = 0; q < batch; ++q, ++m) {
-                PROF(CFDMX_K_BICG_A,
-                     launch_bicg_A(g, s, pre, ctx->B32, ctx->bpart, ctx->kst32, m));
-                PROF(CFDMX_K_BICG_B,
-                     launch_bicg_B(g, s, pre, ctx->B32, ctx->bpart, ctx->kst32, m));
-                PROF(CFDMX_K_BICG_C,
-                     launch_bicg_C(g, s, ctx->B32, ctx->e32, ctx->bpart, ctx->kst32, m));
-            }
-            PROF(CFDMX_K_BICG_CHECK, launch_bicg_check(g, s, ctx->bpart, ctx->kst32, m - 1));
-            LAUNCHCHK();
-            HIPCHK(hipMemcpyAsync(ctx->kst32_h, ctx->kst32, sizeof(KState),
-                                  hipMemcpyDeviceToHost, s));
-            HIPCHK(hipStreamSynchronize(s));
-            prof_harvest(ctx);
-            if (ctx->kst32_h->done) break;
-            batch = batch < 64 ? batch * 2 : 64;
-        }
-        total += ctx->kst32_h->iter + 1;
+        int it = 0;
+        double ires = 0.0;
+        int rc = solve_pb_inner(ctx, ctx->mix_h[2], std::fmax(c.inner_tol, 0.3 * c.tol / res64),
+                                c.maxit - total, &it, &ires);
+        if (rc) return rc;
+        total += it;
         ++outer;
-        if (!std::isfinite(ctx->kst32_h->res)) break; // diverged: keep x, report res64
+        if (!std::isfinite(ires)) break; // diverged: keep x, report res64
     }
     ctx->mix_outer = outer;
     if (iters) *iters = total;

@@ -1506,7 +1506,6 @@ void launch_bicg_C(const Geo &g, hipStream_t s, BicgBuf<T> &B, T *x, double *con
     template void launch_bicg_C<T>(const Geo &, hipStream_t, BicgBuf<T> &, T *,               \
                                    double *const[3], KState *, int);
 CFDMX_BICG_INSTANCES(double)
-CFDMX_BICG_INSTANCES(float)
 #undef CFDMX_BICG_INSTANCES
 
 void launch_bicg_check(const Geo &g, hipStream_t s, double *const part[3], KState *st, int m)

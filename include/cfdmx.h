@@ -68,7 +68,7 @@ typedef struct cfdmx_config {
     int skip_first_pressure; /* RUN_CPU.cpp:303: no pressure solve on step 1 */
     int precision;
     double inner_tol; /* mixed precision: relative tolerance of each FP32 inner solve
-                         (default 1e-3; 1e-5 can stall the FP32 Krylov process) */
+                         (default 1e-5) */
     /* launch shaping */
     int xcd_remap; /* map consecutive logical blocks to one XCD (0 = hardware order) */
 } cfdmx_config;
@@ -212,7 +212,9 @@ int cfdmx_adjoint_drag(cfdmx_ctx *ctx, int nsteps, int checkpoint_every, double 
 #define CFDMX_K_BICG_CHECK 7 /* initial fold and end-of-batch convergence check */
 #define CFDMX_K_SOR 8
 #define CFDMX_K_PROJECT 9
-#define CFDMX_K_COUNT 10
+#define CFDMX_K_MG 10 /* mixed precision: multigrid V-cycle kernels */
+#define CFDMX_K_PB 11 /* mixed precision: FP32 BiCGSTAB products and updates */
+#define CFDMX_K_COUNT 12
 /* Enable (1) or disable (0) HIP-event timing of every launch of the kernels above on
  * the context's stream; resets the totals. */
 int cfdmx_profile(cfdmx_ctx *ctx, int enable);

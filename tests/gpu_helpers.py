@@ -17,7 +17,7 @@ BC_PRESETS = {
 def make_pair(nxc, nyc, nzc, *, lengths=(20.0, 10.0, 0.5), kind=0, re=100.0, dt=0.002,
               solver=0, precond=0, tol=1e-6, maxit=5000, omega=1.8, bc="flowpassing",
               seed=20261202, amp=1e-3, cyl=(5.0, 5.0, 0.5), xcd_remap=1, smoothed_eps=None,
-              skip_first=1, inflow=None, precision=0, inner_tol=1e-3):
+              skip_first=1, inflow=None, precision=0, inner_tol=1e-5):
     num, dirr, zper, conv, flux = BC_PRESETS[bc]
     cfg = pkg.default_config(nx=nxc, ny=nyc, nz=nzc, lx=lengths[0], ly=lengths[1],
                              lz=lengths[2], grid_kind=kind, re=re, dt=dt, solver=solver,
