@@ -275,6 +275,9 @@ def main():
     ap.add_argument("--maxit", type=int, default=5000,
                     help="BiCGSTAB iteration cap (5000, the reference default; lower only for "
                          "counter-collection passes, which need few launches)")
+    ap.add_argument("--part", default="all", choices=["all", "mixed", "adjoint"],
+                    help="run only the mixed-precision or only the adjoint measurement and "
+                         "print its JSON object (for profiling one part)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB three times (the "
                          "calibration kernel for HBM byte counters) and exit")
@@ -291,6 +294,18 @@ def main():
 
     wl = WORKLOADS[args.workload]
     n_cells = wl["nx"] * wl["ny"] * wl["nz"]
+    if args.part == "adjoint":
+        print(json.dumps({"adjoint": adjoint_run(args.seed)}), flush=True)
+        return
+    if args.part == "mixed":
+        sm, mwarm, minfos, mwall, mprof = run_mode(wl, args, 1)
+        print(json.dumps({"mixed": {
+            "value": round(n_cells * args.steps / mwall / 1e6, 4), "unit": "Mcell-steps/s",
+            "iters_per_step": [i["iters"] for i in minfos],
+            "residual_per_step": [i["err"] for i in minfos],
+            "kernels": kernel_table(mprof, n_cells, BYTES_PER_CELL_FP32)}}), flush=True)
+        sm.close()
+        return
     s, warm, infos, wall, prof = run_mode(wl, args, 0)
     value = n_cells * args.steps / wall / 1e6
     iters = [i["iters"] for i in infos]
