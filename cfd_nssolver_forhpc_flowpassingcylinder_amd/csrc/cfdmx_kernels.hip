@@ -768,14 +768,6 @@ k_march(Geo g, MarchIO<T> io, KState *st, int m, const double *part_in)
     const bool writer = (b == 0 && threadIdx.x == 0);
     int mode = 0;
     double beta = 0.0, alpha = 0.0;
-    if (PHASE == 0) {
-        if (!bicg_iteration_start(g, st, m, part_in, writer, mode, beta)) return;
-    } else {
-        double t[1];
-        fold_partials<1>(part_in, g.nmarch, g.nmarch, t);
-        alpha = st->rho[m & 1] / t[0]; // a = r1r0 / dot(ap, r0)
-        if (writer) st->alpha = alpha;
-    }
     // 1-D operator coefficients of the tile's rows j in [jlo, jlo + rows) and of all k
     const int jlo = (int)fdiv((unsigned)q0, g.dz);
     const int rows = (int)fdiv((unsigned)(q0 + qn - 1), g.dz) - jlo + 1;
@@ -795,7 +787,9 @@ k_march(Geo g, MarchIO<T> io, KState *st, int m, const double *part_in)
         const int q = q0 - H + t;
         return i >= 0 && i < g.nxc && t < W && q >= 0 && q < g.cyz;
     };
-    const T betaT = (T)beta, alphaT = (T)alpha;
+    T betaT = (T)0, alphaT = (T)0;
+    // q is loaded whatever the mode (put() ignores it on the first iteration and on a
+    // restart), so that loads can be issued before the scalars are known
     auto load = [&](int i, T (&a)[NF], T (&c)[NF]) {
 #pragma unroll
         for (int u = 0; u < NF; ++u) {
@@ -803,8 +797,7 @@ k_march(Geo g, MarchIO<T> io, KState *st, int m, const double *part_in)
             const bool ok = inside(i, t);
             const int n = i * g.cyz + q0 - H + t;
             a[u] = ok ? io.r[n] : (T)0;
-            if (PHASE == 0) c[u] = (ok && mode == 0) ? io.q[n] : (T)0;
-            else c[u] = ok ? io.ap[n] : (T)0;
+            c[u] = ok ? (PHASE == 0 ? io.q[n] : io.ap[n]) : (T)0;
         }
     };
     auto put = [&](int slot, int i, const T (&a)[NF], const T (&c)[NF]) {
@@ -828,29 +821,48 @@ k_march(Geo g, MarchIO<T> io, KState *st, int m, const double *part_in)
         jc[c] = j - jlo;
         kc[c] = q - j * g.nzc;
     }
-    T a0[NF], c0[NF], a1[NF], c1[NF];
-    load(ib - d, a0, c0);
-    load(ib, a1, c1);
-    put(0, ib - d, a0, c0);
-    load(ib + d, a0, c0);
-    put(1, ib, a1, c1);
+    // the second operand of phase A's dot: r0, or r on a restart
+    auto load_r0 = [&](int i, T (&o)[kCells]) {
+#pragma unroll
+        for (int c = 0; c < kCells; ++c) {
+            const int tq = c * kThreads + threadIdx.x;
+            const int n = i * g.cyz + q0 + tq;
+            o[c] = tq < qn ? (mode == 1 ? io.r[n] : io.r0[n]) : (T)0;
+        }
+    };
+    // Two-deep software pipeline: at the start of step s, register set s & 1 holds the
+    // raw inputs of ring plane ib + (s+1)d and, in phase A, the dot operand of plane
+    // ib + s d; both were loaded two steps earlier, so every load has two barriers and
+    // two plane computations to arrive.
+    T ra[2][NF], rc[2][NF], r0v[2][kCells];
+    load(ib - d, ra[0], rc[0]);
+    load(ib, ra[1], rc[1]);
+    // the scalars, from the previous phase's partials, while the first planes load
+    if (PHASE == 0) {
+        if (!bicg_iteration_start(g, st, m, part_in, writer, mode, beta)) return;
+    } else {
+        double t[1];
+        fold_partials<1>(part_in, g.nmarch, g.nmarch, t);
+        alpha = st->rho[m & 1] / t[0]; // a = r1r0 / dot(ap, r0)
+        if (writer) st->alpha = alpha;
+    }
+    betaT = (T)beta;
+    alphaT = (T)alpha;
+    put(0, ib - d, ra[0], rc[0]);
+    load(ib + d, ra[0], rc[0]);
+    put(1, ib, ra[1], rc[1]);
+    if (len > 1) load(ib + 2 * d, ra[1], rc[1]);
+    if (PHASE == 0) {
+        load_r0(ib, r0v[0]);
+        if (len > 1) load_r0(ib + d, r0v[1]);
+    }
     double acc[2] = {0.0, 0.0};
-    for (int s = 0; s < len; ++s) {
+    auto step = [&](int s, auto set) {
+        constexpr int X = decltype(set)::value;
         const int i = ib + s * d;
         const int sb = s % 3, sc = (s + 1) % 3, sa = (s + 2) % 3; // behind, at, ahead
-        put(sa, i + d, a0, c0);
-        // the second operand of phase A's dot (r0, or r on a restart) for this plane,
-        // then the next ring plane, both in flight across the barrier
-        T r0v[kCells];
-        if (PHASE == 0) {
-#pragma unroll
-            for (int c = 0; c < kCells; ++c) {
-                const int tq = c * kThreads + threadIdx.x;
-                const int n = i * g.cyz + q0 + tq;
-                r0v[c] = tq < qn ? (mode == 1 ? io.r[n] : io.r0[n]) : (T)0;
-            }
-        }
-        if (s + 1 < len) load(i + 2 * d, a0, c0);
+        put(sa, i + d, ra[X], rc[X]);
+        if (s + 2 < len) load(i + 3 * d, ra[X], rc[X]);
         __syncthreads();
         const T *Rc = ring + sc * WS;
         const T *Rm = ring + (d > 0 ? sb : sa) * WS, *Rp = ring + (d > 0 ? sa : sb) * WS;
@@ -881,7 +893,7 @@ k_march(Geo g, MarchIO<T> io, KState *st, int m, const double *part_in)
             if (PHASE == 0) {
                 io.out_p[n] = own;
                 io.out_ap[n] = sum;
-                const T r0n = r0v[c];
+                const T r0n = r0v[X][c];
                 if (mode == 1) io.r0[n] = r0n;
                 acc[0] += sd * (double)r0n;
             } else {
@@ -890,7 +902,12 @@ k_march(Geo g, MarchIO<T> io, KState *st, int m, const double *part_in)
                 acc[1] += sd * sd;
             }
         }
+        if (PHASE == 0 && s + 2 < len) load_r0(i + 2 * d, r0v[X]);
         __syncthreads();
+    };
+    for (int s = 0; s < len; s += 2) {
+        step(s, std::integral_constant<int, 0>{});
+        if (s + 1 < len) step(s + 1, std::integral_constant<int, 1>{});
     }
     if (PHASE == 0) {
         double a1s[1] = {acc[0]};
