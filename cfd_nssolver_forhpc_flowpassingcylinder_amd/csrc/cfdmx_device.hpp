@@ -28,9 +28,11 @@ constexpr int kPerBlock = kThreads * kCells;
 constexpr int kGC = 2;
 // plane-marching stencil kernels: a block owns kTile consecutive cells of the (j,k)
 // plane index q = j*nzc + k and marches over kChunk planes i; thread t takes
-// q = q0 + 256c + t, c = 0..3, on every plane, and accumulates plane by plane
+// q = q0 + 256c + t, c = 0..kMarchCells-1, on every plane, and accumulates plane by
+// plane
 constexpr int kTile = 1024;
 constexpr int kChunk = 8;
+constexpr int kMarchCells = kTile / kThreads;
 
 // n / d for n * d < 2^40, with m = ceil(2^40 / d)
 struct FastDiv {

@@ -660,8 +660,8 @@ __device__ __forceinline__ void march_span(const Geo &g, int chunk, int &first, 
     first = dir > 0 ? i0 : i1 - 1;
 }
 
-// LDS of a march block: y coefficients of the tile's rows, z coefficients (double),
-// then the three ring planes (T)
+// LDS of a march block: x coefficients of the chunk's planes, y coefficients of the
+// tile's rows, z coefficients (double), then two plane slots (T)
 __host__ __device__ inline int march_rows(int nzc) { return kTile / nzc + 2; }
 
 template <int PRE>
@@ -746,10 +746,27 @@ __device__ __forceinline__ bool bicg_iteration_start(const Geo &g, KState *st, i
     return true;
 }
 
-// NF = 5: at most 128 VGPRs, so that four blocks (one march launch at 512x256x64)
-// fit each CU at once
-template <int PHASE, int PRE, int NF, typename T>
-__global__ void __launch_bounds__(kThreads, NF <= 5 ? 4 : 1)
+// v when ok is 1, +0 when ok is 0, by masking the bits (a select here becomes a branch
+// around the LDS store, which costs the plane loop its overlap of loads and stores)
+__device__ __forceinline__ double keep_if(double v, int ok)
+{
+    return __longlong_as_double(__double_as_longlong(v) & -(long long)ok);
+}
+__device__ __forceinline__ float keep_if(float v, int ok)
+{
+    return __int_as_float(__float_as_int(v) & -ok);
+}
+
+// A march block keeps the new vector (p for A, s for B) of its own cells on the planes
+// behind, at and ahead of the current one in registers (the x neighbours), and writes
+// the current plane -- its own cells plus one z-row of halo on each side of the tile --
+// to one of two LDS slots for the y and z neighbours.  Two slots need one barrier per
+// plane: the slot written at step s was last read at step s-2.  The raw inputs of the
+// plane two ahead (own cells) and of the next plane's halo are loaded one step before
+// they are formed.  Halo elements per thread: NH = ceil(2 nzc / 256).
+// NH = 1: at most 128 VGPRs, so that four blocks fit each CU at once.
+template <int PHASE, int PRE, int NH, bool FULL, typename T>
+__global__ void __launch_bounds__(kThreads, NH == 1 ? 4 : 1)
 k_march(Geo g, MarchIO<T> io, KState *st, int m, const double *part_in)
 {
     if (st->done) return;
@@ -757,18 +774,22 @@ k_march(Geo g, MarchIO<T> io, KState *st, int m, const double *part_in)
     if (g.xcd_remap) b = (b & 7) * ((gridDim.x + 7) >> 3) + (b >> 3);
     if (b >= g.nmarch) return;
     extern __shared__ double lds[];
+    constexpr int C = kMarchCells;
     const int tile = b % g.ntile, chunk = b / g.ntile;
     int ib, d, len;
     march_span(g, chunk, ib, d, len);
+    const int i0 = chunk * kChunk;
     const int q0 = tile * kTile, qn = min(kTile, g.cyz - q0);
-    const int H = g.nzc, W = qn + 2 * H, WS = kTile + 2 * H;
+    const int H = g.nzc, WP = kTile + 2 * H;
     const int RY = march_rows(g.nzc);
-    double *cy = lds, *cz = lds + 2 * RY;
-    T *ring = reinterpret_cast<T *>(cz + 2 * g.nzc);
+    double *cx = lds, *cy = cx + 2 * kChunk, *cz = cy + 2 * RY;
+    T *slots = reinterpret_cast<T *>(cz + 2 * g.nzc);
     const bool writer = (b == 0 && threadIdx.x == 0);
     int mode = 0;
     double beta = 0.0, alpha = 0.0;
-    // 1-D operator coefficients of the tile's rows j in [jlo, jlo + rows) and of all k
+    T betaT = (T)0, alphaT = (T)0;
+    // 1-D operator coefficients: the chunk's planes, the tile's rows j in
+    // [jlo, jlo + rows), all k
     const int jlo = (int)fdiv((unsigned)q0, g.dz);
     const int rows = (int)fdiv((unsigned)(q0 + qn - 1), g.dz) - jlo + 1;
     for (int t = threadIdx.x; t < rows; t += kThreads) {
@@ -779,135 +800,185 @@ k_march(Geo g, MarchIO<T> io, KState *st, int m, const double *part_in)
         cz[t] = g.pa[2].m[t];
         cz[g.nzc + t] = g.pa[2].p[t];
     }
-    // Ring planes are staged in two steps so that a plane's loads are in flight while
-    // the previous plane is computed: load() issues the raw r and q (A) or Ap (B) of
-    // ring plane i (NF cells per thread, ring cell t = threadIdx.x + 256u), put() forms
-    // the ring value and writes it to its LDS slot.
-    auto inside = [&](int i, int t) {
-        const int q = q0 - H + t;
-        return i >= 0 && i < g.nxc && t < W && q >= 0 && q < g.cyz;
+    if (threadIdx.x < len) {
+        cx[threadIdx.x] = g.pa[0].m[i0 + threadIdx.x];
+        cx[kChunk + threadIdx.x] = g.pa[0].p[i0 + threadIdx.x];
+    }
+    // Loads are branch-free: a lane whose cell is outside the grid, or whose plane is
+    // not wanted, reads element 0, and the value is masked to zero when formed.  With
+    // no load under a branch the compiler waits for exactly the values it needs.  q is
+    // loaded whatever the mode (the first iteration and a restart ignore it), so that
+    // loads can be issued before the scalars are known.
+    const T *src2 = PHASE == 0 ? io.q : io.ap;
+    auto plane_ok = [&](int i, bool want) -> int {
+        return (int)want & (int)(i >= 0) & (int)(i < g.nxc);
     };
-    T betaT = (T)0, alphaT = (T)0;
-    // q is loaded whatever the mode (put() ignores it on the first iteration and on a
-    // restart), so that loads can be issued before the scalars are known
-    auto load = [&](int i, T (&a)[NF], T (&c)[NF]) {
+    auto own_ok = [&](int c) -> int { return FULL ? 1 : (int)(c * kThreads + threadIdx.x < qn); };
+    // halo element h of this thread: slot index t (rows above, then below the tile)
+    auto halo_t = [&](int h) {
+        const int x = threadIdx.x + h * kThreads;
+        return x < H ? x : qn + x;
+    };
+    auto halo_ok = [&](int h) -> int {
+        const int x = threadIdx.x + h * kThreads, q = q0 - H + halo_t(h);
+        return (int)(x < 2 * H) & (int)(q >= 0) & (int)(q < g.cyz);
+    };
+    auto load_own = [&](int i, bool want, T (&a)[C], T (&c2)[C]) {
+        const int pk = plane_ok(i, want);
 #pragma unroll
-        for (int u = 0; u < NF; ++u) {
-            const int t = threadIdx.x + u * kThreads;
-            const bool ok = inside(i, t);
-            const int n = i * g.cyz + q0 - H + t;
-            a[u] = ok ? io.r[n] : (T)0;
-            c[u] = ok ? (PHASE == 0 ? io.q[n] : io.ap[n]) : (T)0;
+        for (int c = 0; c < C; ++c) {
+            const int n = (i * g.cyz + q0 + c * kThreads + (int)threadIdx.x) & -(pk & own_ok(c));
+            a[c] = io.r[n];
+            c2[c] = src2[n];
         }
     };
-    auto put = [&](int slot, int i, const T (&a)[NF], const T (&c)[NF]) {
-        T *R = ring + slot * WS;
+    auto load_halo = [&](int i, bool want, T (&a)[NH], T (&c2)[NH]) {
+        const int pk = plane_ok(i, want);
 #pragma unroll
-        for (int u = 0; u < NF; ++u) {
-            const int t = threadIdx.x + u * kThreads;
-            if (t < W) {
-                T v;
-                if (PHASE == 0) v = mode == 0 ? a[u] + betaT * c[u] : a[u];
-                else v = a[u] - alphaT * c[u];
-                R[t] = inside(i, t) ? v : (T)0;
-            }
+        for (int h = 0; h < NH; ++h) {
+            const int n = (i * g.cyz + q0 - H + halo_t(h)) & -(pk & halo_ok(h));
+            a[h] = io.r[n];
+            c2[h] = src2[n];
         }
     };
-    int jc[kCells], kc[kCells];
+    // the new vector from the raw inputs (PHASE 0: p = r + beta*q, or r; PHASE 1:
+    // s = r - alpha*Ap), zero outside the grid
+    auto form = [&](T a, T c2, int ok) -> T {
+        T v;
+        if (PHASE == 0) v = mode == 0 ? a + betaT * c2 : a;
+        else v = a - alphaT * c2;
+        return keep_if(v, ok);
+    };
+    // the second operand of phase A's dot: r0, or r on a restart
+    auto load_r0 = [&](int i, bool want, T (&o)[C]) {
+        const T *src = mode == 1 ? io.r : io.r0;
+        const int pk = (int)want;
 #pragma unroll
-    for (int c = 0; c < kCells; ++c) {
+        for (int c = 0; c < C; ++c)
+            o[c] = src[(i * g.cyz + q0 + c * kThreads + (int)threadIdx.x) & -(pk & own_ok(c))];
+    };
+    int jc[C], kc[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
         const int q = q0 + c * kThreads + threadIdx.x;
         const int j = (int)fdiv((unsigned)q, g.dz);
         jc[c] = j - jlo;
         kc[c] = q - j * g.nzc;
     }
-    // the second operand of phase A's dot: r0, or r on a restart
-    auto load_r0 = [&](int i, T (&o)[kCells]) {
-#pragma unroll
-        for (int c = 0; c < kCells; ++c) {
-            const int tq = c * kThreads + threadIdx.x;
-            const int n = i * g.cyz + q0 + tq;
-            o[c] = tq < qn ? (mode == 1 ? io.r[n] : io.r0[n]) : (T)0;
+    // own cells of the planes behind (ob), at (oc) and ahead (oa) of the current one
+    T ob[C], oc[C], oa[C];
+    T ra[C], rc[C];   // raw inputs of the plane two ahead (own cells)
+    T ha[NH], hc[NH]; // raw inputs of the current plane's halo
+    T r0c[C];         // phase A: dot operand of the current plane
+    {
+        T pa[C], pc[C], na[C], nc[C];
+        load_own(ib - d, true, ra, rc);
+        load_own(ib, true, pa, pc);
+        load_own(ib + d, true, na, nc);
+        load_halo(ib, true, ha, hc);
+        // the scalars, from the previous phase's partials, while the first planes load
+        if (PHASE == 0) {
+            if (!bicg_iteration_start(g, st, m, part_in, writer, mode, beta)) return;
+        } else {
+            double t[1];
+            fold_partials<1>(part_in, g.nmarch, g.nmarch, t);
+            alpha = st->rho[m & 1] / t[0]; // a = r1r0 / dot(ap, r0)
+            if (writer) st->alpha = alpha;
         }
-    };
-    // Two-deep software pipeline: at the start of step s, register set s & 1 holds the
-    // raw inputs of ring plane ib + (s+1)d and, in phase A, the dot operand of plane
-    // ib + s d; both were loaded two steps earlier, so every load has two barriers and
-    // two plane computations to arrive.
-    T ra[2][NF], rc[2][NF], r0v[2][kCells];
-    load(ib - d, ra[0], rc[0]);
-    load(ib, ra[1], rc[1]);
-    // the scalars, from the previous phase's partials, while the first planes load
-    if (PHASE == 0) {
-        if (!bicg_iteration_start(g, st, m, part_in, writer, mode, beta)) return;
-    } else {
-        double t[1];
-        fold_partials<1>(part_in, g.nmarch, g.nmarch, t);
-        alpha = st->rho[m & 1] / t[0]; // a = r1r0 / dot(ap, r0)
-        if (writer) st->alpha = alpha;
-    }
-    betaT = (T)beta;
-    alphaT = (T)alpha;
-    put(0, ib - d, ra[0], rc[0]);
-    load(ib + d, ra[0], rc[0]);
-    put(1, ib, ra[1], rc[1]);
-    if (len > 1) load(ib + 2 * d, ra[1], rc[1]);
-    if (PHASE == 0) {
-        load_r0(ib, r0v[0]);
-        if (len > 1) load_r0(ib + d, r0v[1]);
-    }
-    double acc[2] = {0.0, 0.0};
-    auto step = [&](int s, auto set) {
-        constexpr int X = decltype(set)::value;
-        const int i = ib + s * d;
-        const int sb = s % 3, sc = (s + 1) % 3, sa = (s + 2) % 3; // behind, at, ahead
-        put(sa, i + d, ra[X], rc[X]);
-        if (s + 2 < len) load(i + 3 * d, ra[X], rc[X]);
-        __syncthreads();
-        const T *Rc = ring + sc * WS;
-        const T *Rm = ring + (d > 0 ? sb : sa) * WS, *Rp = ring + (d > 0 ? sa : sb) * WS;
-        const double xm = g.pa[0].m[i], xp = g.pa[0].p[i];
+        betaT = (T)beta;
+        alphaT = (T)alpha;
+        const int okb = plane_ok(ib - d, true), okc = plane_ok(ib, true),
+                  oka = plane_ok(ib + d, true);
 #pragma unroll
-        for (int c = 0; c < kCells; ++c) {
+        for (int c = 0; c < C; ++c) {
+            const int ow = own_ok(c);
+            ob[c] = form(ra[c], rc[c], okb & ow);
+            oc[c] = form(pa[c], pc[c], okc & ow);
+            oa[c] = form(na[c], nc[c], oka & ow);
+        }
+    }
+    load_own(ib + 2 * d, len > 1, ra, rc);
+    double acc[2] = {0.0, 0.0};
+    for (int s = 0; s < len; ++s) {
+        const int i = ib + s * d;
+        T *S = slots + (s & 1) * WP;
+        // the current plane into its slot: own cells, then the halo
+#pragma unroll
+        for (int c = 0; c < C; ++c)
+            if (FULL || c * kThreads + (int)threadIdx.x < qn) S[H + c * kThreads + threadIdx.x] = oc[c];
+        {
+            const int okp = plane_ok(i, true);
+#pragma unroll
+            for (int h = 0; h < NH; ++h)
+                if (threadIdx.x + h * kThreads < 2 * H)
+                    S[halo_t(h)] = form(ha[h], hc[h], okp & halo_ok(h));
+        }
+        // next loads: the halo of the next plane and phase A's dot operand of this
+        // one (the own cells of the plane three ahead follow this plane's compute)
+        load_halo(i + d, s + 1 < len, ha, hc);
+        if (PHASE == 0) load_r0(i, true, r0c);
+        __syncthreads();
+        const double xm = cx[i - i0], xp = cx[kChunk + i - i0];
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
             const int tq = c * kThreads + threadIdx.x;
-            if (tq >= qn) break;
+            if (!FULL && tq >= qn) break;
             const int jj = jc[c], k = kc[c], j = jlo + jj;
             double v[7];
             march_row<PRE>(g, i, j, k, xm, xp, cy[jj], cy[RY + jj], cz[k], cz[g.nzc + k], v);
-            // the 7-point sum accumulates in double for every T: in float the
-            // Laplacian's cancellation on smooth vectors stalls the Krylov process
+            // All neighbour reads are unconditional (the halo rows and the zeroed
+            // planes outside the grid are in place), and a missing neighbour adds +0.0
+            // in place of its term.  The running sum starts at +0.0 and so is never
+            // -0.0, which makes +0.0 an exact identity: the result equals skipping the
+            // term, as the reference row does.  The 7-point sum accumulates in double
+            // for every T: in float the Laplacian's cancellation on smooth vectors
+            // stalls the Krylov process.
             const int tc = tq + H;
+            const double n0 = (double)(d > 0 ? ob[c] : oa[c]);
+            const double n1 = (double)S[tc - g.nzc];
+            const double n2 = (double)S[k == 0 ? tc + g.nzc - 1 : tc - 1];
+            const double n3 = (double)oc[c];
+            const double n4 = (double)S[k == g.nzc - 1 ? tc - (g.nzc - 1) : tc + 1];
+            const double n5 = (double)S[tc + g.nzc];
+            const double n6 = (double)(d > 0 ? oa[c] : ob[c]);
             double acc7 = 0.0;
-            if (i != 0) acc7 += v[0] * (double)Rm[tc];
-            if (j != 0) acc7 += v[1] * (double)Rc[tc - g.nzc];
-            if (k != 0 || g.zper) acc7 += v[2] * (double)Rc[k == 0 ? tc + g.nzc - 1 : tc - 1];
-            acc7 += v[3] * (double)Rc[tc];
-            if (k != g.nzc - 1 || g.zper)
-                acc7 += v[4] * (double)Rc[k == g.nzc - 1 ? tc - (g.nzc - 1) : tc + 1];
-            if (j != g.nyc - 1) acc7 += v[5] * (double)Rc[tc + g.nzc];
-            if (i != g.nxc - 1) acc7 += v[6] * (double)Rp[tc];
+            acc7 += i != 0 ? v[0] * n0 : 0.0;
+            acc7 += j != 0 ? v[1] * n1 : 0.0;
+            acc7 += (k != 0 || g.zper) ? v[2] * n2 : 0.0;
+            acc7 += v[3] * n3;
+            acc7 += (k != g.nzc - 1 || g.zper) ? v[4] * n4 : 0.0;
+            acc7 += j != g.nyc - 1 ? v[5] * n5 : 0.0;
+            acc7 += i != g.nxc - 1 ? v[6] * n6 : 0.0;
             const T sum = (T)acc7;
             const int n = i * g.cyz + q0 + tq;
-            const T own = Rc[tc];
+            const T own = oc[c];
             const double sd = (double)sum;
             if (PHASE == 0) {
                 io.out_p[n] = own;
                 io.out_ap[n] = sum;
-                const T r0n = r0v[X][c];
-                if (mode == 1) io.r0[n] = r0n;
-                acc[0] += sd * (double)r0n;
+                const T r0v = r0c[c];
+                if (mode == 1) io.r0[n] = r0v;
+                acc[0] += sd * (double)r0v;
             } else {
                 io.out_as[n] = sum;
                 acc[0] += sd * (double)own;
                 acc[1] += sd * sd;
             }
+            // one cell's neighbour reads at a time in flight: keeps the register count
+            // within four blocks per CU
+            __builtin_amdgcn_sched_barrier(0);
         }
-        if (PHASE == 0 && s + 2 < len) load_r0(i + 2 * d, r0v[X]);
-        __syncthreads();
-    };
-    for (int s = 0; s < len; s += 2) {
-        step(s, std::integral_constant<int, 0>{});
-        if (s + 1 < len) step(s + 1, std::integral_constant<int, 1>{});
+        // shift the register queue: the plane two ahead becomes the one ahead
+        {
+            const int ok2 = plane_ok(i + 2 * d, s + 1 < len);
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                ob[c] = oc[c];
+                oc[c] = oa[c];
+                oa[c] = form(ra[c], rc[c], ok2 & own_ok(c));
+            }
+        }
+        load_own(i + 3 * d, s + 2 < len, ra, rc);
     }
     if (PHASE == 0) {
         double a1s[1] = {acc[0]};
@@ -942,7 +1013,7 @@ k_march_C(Geo g, T *__restrict__ r, const T *__restrict__ ap, T *__restrict__ p,
     for (int s = 0; s < len; ++s) {
         const int i = ib + s * d;
 #pragma unroll
-        for (int c = 0; c < kCells; ++c) {
+        for (int c = 0; c < kMarchCells; ++c) {
             const int tq = c * kThreads + threadIdx.x;
             if (tq < qn) {
                 const int n = i * g.cyz + q0 + tq;
@@ -1454,32 +1525,40 @@ static inline int march_blocks(const Geo &g)
 template <typename T>
 static inline size_t march_lds(const Geo &g)
 {
-    return ((size_t)2 * march_rows(g.nzc) + 2 * g.nzc) * sizeof(double) +
-           (size_t)3 * (kTile + 2 * g.nzc) * sizeof(T);
+    return ((size_t)2 * kChunk + 2 * march_rows(g.nzc) + 2 * g.nzc) * sizeof(double) +
+           (size_t)2 * (kTile + 2 * g.nzc) * sizeof(T);
 }
 
-// ring cells per thread: ceil((1024 + 2 nzc) / 256); 5 covers nzc <= 128, 9 covers the
-// largest supported nzc (600)
-template <int PHASE, int NF, typename T>
-static void launch_march_nf(const Geo &g, hipStream_t s, int pre, const MarchIO<T> &io,
-                            KState *st, int m, const double *part_in)
+// halo cells per thread: ceil(2 nzc / 256); 1 covers nzc <= 128, 5 the largest
+// supported nzc (600)
+template <int PHASE, int NH, bool FULL, typename T>
+static void launch_march_full(const Geo &g, hipStream_t s, int pre, const MarchIO<T> &io,
+                              KState *st, int m, const double *part_in)
 {
     if (pre)
-        hipLaunchKernelGGL((k_march<PHASE, 1, NF, T>), dim3(march_blocks(g)), dim3(kThreads),
-                           march_lds<T>(g), s, g, io, st, m, part_in);
+        hipLaunchKernelGGL((k_march<PHASE, 1, NH, FULL, T>), dim3(march_blocks(g)),
+                           dim3(kThreads), march_lds<T>(g), s, g, io, st, m, part_in);
     else
-        hipLaunchKernelGGL((k_march<PHASE, 0, NF, T>), dim3(march_blocks(g)), dim3(kThreads),
-                           march_lds<T>(g), s, g, io, st, m, part_in);
+        hipLaunchKernelGGL((k_march<PHASE, 0, NH, FULL, T>), dim3(march_blocks(g)),
+                           dim3(kThreads), march_lds<T>(g), s, g, io, st, m, part_in);
+}
+template <int PHASE, int NH, typename T>
+static void launch_march_nh(const Geo &g, hipStream_t s, int pre, const MarchIO<T> &io,
+                            KState *st, int m, const double *part_in)
+{
+    if (g.cyz % kTile == 0)
+        launch_march_full<PHASE, NH, true>(g, s, pre, io, st, m, part_in);
+    else
+        launch_march_full<PHASE, NH, false>(g, s, pre, io, st, m, part_in);
 }
 template <int PHASE, typename T>
 static void launch_march(const Geo &g, hipStream_t s, int pre, const MarchIO<T> &io, KState *st,
                          int m, const double *part_in)
 {
-    const int nf = (kTile + 2 * g.nzc + kThreads - 1) / kThreads;
-    if (nf <= 5)
-        launch_march_nf<PHASE, 5>(g, s, pre, io, st, m, part_in);
+    if (2 * g.nzc <= kThreads)
+        launch_march_nh<PHASE, 1>(g, s, pre, io, st, m, part_in);
     else
-        launch_march_nf<PHASE, 9>(g, s, pre, io, st, m, part_in);
+        launch_march_nh<PHASE, 5>(g, s, pre, io, st, m, part_in);
 }
 template <typename T>
 void launch_bicg_A(const Geo &g, hipStream_t s, int pre, BicgBuf<T> &B, double *const part[3],
