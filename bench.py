@@ -169,7 +169,7 @@ def cpu_baseline(wl, iters_per_step, budget_s):
 
 def run_mode(wl, args, precision):
     """W warm-up steps, K timed steps (wall clock), then profiled steps (HIP events)."""
-    tag = "FP64" if precision == 0 else "mixed"
+    tag = {0: "FP64", 1: "mixed", 2: "FP64-MG"}[precision]
     s = make_solver(wl, args.precond, args.solver, args.seed, args.maxit, precision)
     warm = []
     for q in range(args.warmup):
@@ -275,7 +275,7 @@ def main():
     ap.add_argument("--maxit", type=int, default=5000,
                     help="BiCGSTAB iteration cap (5000, the reference default; lower only for "
                          "counter-collection passes, which need few launches)")
-    ap.add_argument("--part", default="all", choices=["all", "mixed", "adjoint"],
+    ap.add_argument("--part", default="all", choices=["all", "mixed", "fp64mg", "adjoint"],
                     help="run only the mixed-precision or only the adjoint measurement and "
                          "print its JSON object (for profiling one part)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -297,9 +297,9 @@ def main():
     if args.part == "adjoint":
         print(json.dumps({"adjoint": adjoint_run(args.seed)}), flush=True)
         return
-    if args.part == "mixed":
-        sm, mwarm, minfos, mwall, mprof = run_mode(wl, args, 1)
-        print(json.dumps({"mixed": {
+    if args.part in ("mixed", "fp64mg"):
+        sm, mwarm, minfos, mwall, mprof = run_mode(wl, args, 1 if args.part == "mixed" else 2)
+        print(json.dumps({args.part: {
             "value": round(n_cells * args.steps / mwall / 1e6, 4), "unit": "Mcell-steps/s",
             "iters_per_step": [i["iters"] for i in minfos],
             "residual_per_step": [i["err"] for i in minfos],

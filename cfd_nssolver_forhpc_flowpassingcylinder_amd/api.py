@@ -16,7 +16,7 @@ OK, ERR_ARG, ERR_HIP, ERR_STATE = 0, -1, -2, -3
 GRID_UNIFORM, GRID_SML = 0, 1
 SOLVER_BICGSTAB, SOLVER_RBSOR = 0, 1
 PRECOND_NONE, PRECOND_JACOBI = 0, 1
-PRECISION_FP64, PRECISION_MIXED = 0, 1
+PRECISION_FP64, PRECISION_MIXED, PRECISION_FP64_MG = 0, 1, 2
 
 DP = C.POINTER(C.c_double)
 

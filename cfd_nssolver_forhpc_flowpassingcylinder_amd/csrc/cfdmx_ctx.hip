@@ -28,12 +28,14 @@ struct cfdmx_ctx {
     double *p = nullptr, *eta = nullptr, *f = nullptr;
     double *x = nullptr, *b = nullptr, *px1 = nullptr, *px2 = nullptr;
     BicgBuffers B = {};
-    // mixed precision: FP32 vectors of the preconditioned inner BiCGSTAB (r, r0, p, v,
-    // s, t, p^, s^) and the correction e, the multigrid levels, the inner-solve state,
-    // the three folded sums of the FP64 residual
-    float *pb[8] = {};
-    float *e32 = nullptr;
-    std::vector<MGLevel> mg;
+    // multigrid-preconditioned solve (mixed precision: float; FP64 multigrid: double):
+    // vectors of the preconditioned inner BiCGSTAB (r, r0, p, v, s, t, p^, s^) and the
+    // correction e, the multigrid levels of the type in use, the inner-solve state, the
+    // three folded sums of the FP64 residual
+    void *pb[8] = {};
+    void *ecorr = nullptr;
+    std::vector<MGLevelT<float>> mg32;
+    std::vector<MGLevelT<double>> mg64;
     PBState *pbs = nullptr, *pbs_h = nullptr;
     double *pb_part = nullptr;
     double *mix = nullptr, *mix_h = nullptr;
@@ -248,6 +250,164 @@ template <class T> static int dalloc(cfdmx_ctx *ctx, T **ptr, size_t count)
         if (rc_) return rc_;                                                                   \
     } while (0)
 
+// ---- multigrid-preconditioned solve: levels, coarse inverse, vectors ----
+template <typename T> static std::vector<MGLevelT<T>> &mg_levels(cfdmx_ctx *ctx);
+template <> std::vector<MGLevelT<float>> &mg_levels<float>(cfdmx_ctx *ctx) { return ctx->mg32; }
+template <> std::vector<MGLevelT<double>> &mg_levels<double>(cfdmx_ctx *ctx) { return ctx->mg64; }
+
+// (A + 1 1^T / n)^-1 of a level's Laplacian (the operator of the multigrid kernels:
+// c_d per existing neighbour, periodic z wrapped), by Cholesky on the host in FP64.
+// A's null space is the constants, so the matrix is positive definite and its inverse
+// maps a mean-free f to the mean-free solution of A u = f.  Symmetric, so the layout
+// is both row- and column-major.
+static bool coarse_inverse(int nx, int ny, int nz, double cx, double cy, double cz, int zper,
+                           std::vector<double> &inv)
+{
+    const int n = nx * ny * nz;
+    std::vector<double> M((size_t)n * n, 1.0 / n);
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nz; ++k) {
+                const int c = (i * ny + j) * nz + k;
+                auto add = [&](int q, double w) {
+                    M[(size_t)c * n + c] += w;
+                    M[(size_t)c * n + q] -= w;
+                };
+                if (i > 0) add(c - ny * nz, cx);
+                if (i < nx - 1) add(c + ny * nz, cx);
+                if (j > 0) add(c - nz, cy);
+                if (j < ny - 1) add(c + nz, cy);
+                if (nz > 1) {
+                    if (zper) {
+                        add(k > 0 ? c - 1 : c + nz - 1, cz);
+                        add(k < nz - 1 ? c + 1 : c - (nz - 1), cz);
+                    } else {
+                        if (k > 0) add(c - 1, cz);
+                        if (k < nz - 1) add(c + 1, cz);
+                    }
+                }
+            }
+    // M = L L^T, L in the lower triangle of M
+    for (int j = 0; j < n; ++j) {
+        double d = M[(size_t)j * n + j];
+        for (int k = 0; k < j; ++k) d -= M[(size_t)j * n + k] * M[(size_t)j * n + k];
+        if (!(d > 0.0)) return false;
+        const double ljj = std::sqrt(d);
+        M[(size_t)j * n + j] = ljj;
+        for (int i = j + 1; i < n; ++i) {
+            double v = M[(size_t)i * n + j];
+            for (int k = 0; k < j; ++k) v -= M[(size_t)i * n + k] * M[(size_t)j * n + k];
+            M[(size_t)i * n + j] = v / ljj;
+        }
+    }
+    inv.assign((size_t)n * n, 0.0);
+    std::vector<double> y(n);
+    for (int col = 0; col < n; ++col) {
+        for (int i = 0; i < n; ++i) {
+            double v = i == col ? 1.0 : 0.0;
+            for (int k = 0; k < i; ++k) v -= M[(size_t)i * n + k] * y[k];
+            y[i] = v / M[(size_t)i * n + i];
+        }
+        for (int i = n - 1; i >= 0; --i) {
+            double v = y[i];
+            for (int k = i + 1; k < n; ++k) v -= M[(size_t)k * n + i] * inv[(size_t)col * n + k];
+            inv[(size_t)col * n + i] = v / M[(size_t)i * n + i];
+        }
+    }
+    return true;
+}
+
+// Levels of the multigrid preconditioner (level 0 is the SPE operator of the uniform
+// grid), the inner-solve vectors and state, in arithmetic type T.
+template <typename T> static int setup_multigrid(cfdmx_ctx *ctx)
+{
+    const Geo &g = ctx->g;
+    const size_t N = (size_t)g.N;
+    for (int q = 0; q < 8; ++q) {
+        T *z = nullptr;
+        ALLOC(z, N);
+        ctx->pb[q] = z;
+    }
+    {
+        T *z = nullptr;
+        ALLOC(z, N);
+        ctx->ecorr = z;
+    }
+    std::vector<MGLevelT<T>> &lev = mg_levels<T>(ctx);
+    lev.clear();
+    auto divs = [](MGLevelT<T> &L) {
+        L.dz = make_fastdiv((unsigned)L.nz);
+        L.dy = make_fastdiv((unsigned)L.ny);
+        L.dzh = make_fastdiv((unsigned)(L.nz >= 2 ? L.nz / 2 : 1));
+    };
+    MGLevelT<T> L = {};
+    L.nx = g.nxc;
+    L.ny = g.nyc;
+    L.nz = g.nzc;
+    L.n = (int)N;
+    double h[3] = {ctx->Dx[kGC], ctx->Dy[kGC], ctx->Dz[kGC]};
+    double cd[3] = {1.0 / (ctx->Dx[kGC] * ctx->Dxs[kGC]), 1.0 / (ctx->Dy[kGC] * ctx->Dys[kGC]),
+                    1.0 / (ctx->Dz[kGC] * ctx->Dzs[kGC])};
+    L.cx = (T)cd[0];
+    L.cy = (T)cd[1];
+    L.cz = (T)cd[2];
+    L.fx = L.fy = L.fz = 1;
+    L.zper = g.zper;
+    divs(L);
+    lev.push_back(L);
+    for (;;) {
+        const MGLevelT<T> &P = lev.back();
+        if (P.n <= 512) break;
+        const double hmin = std::min(h[0], std::min(h[1], h[2]));
+        const int fx = (P.nx % 2 == 0 && P.nx >= 4 && h[0] <= 2.0 * hmin) ? 2 : 1;
+        const int fy = (P.ny % 2 == 0 && P.ny >= 4 && h[1] <= 2.0 * hmin) ? 2 : 1;
+        const int fz = (P.nz % 2 == 0 && P.nz >= 2 && h[2] <= 2.0 * hmin) ? 2 : 1;
+        if (fx * fy * fz == 1) break;
+        MGLevelT<T> C = {};
+        C.nx = P.nx / fx;
+        C.ny = P.ny / fy;
+        C.nz = P.nz / fz;
+        C.n = C.nx * C.ny * C.nz;
+        h[0] *= fx;
+        h[1] *= fy;
+        h[2] *= fz;
+        for (int a = 0; a < 3; ++a) cd[a] = 1.0 / (h[a] * h[a]);
+        C.cx = (T)cd[0];
+        C.cy = (T)cd[1];
+        C.cz = (T)cd[2];
+        C.fx = fx;
+        C.fy = fy;
+        C.fz = fz;
+        C.zper = g.zper;
+        divs(C);
+        ALLOC(C.u, (size_t)C.n);
+        ALLOC(C.f, (size_t)C.n);
+        lev.push_back(C);
+    }
+    MGLevelT<T> &B = lev.back();
+    if (B.n > 512)
+        FAIL(CFDMX_ERR_ARG, "multigrid: the hierarchy of %dx%dx%d stops at %d cells (the "
+                            "coarse solve takes at most 512); use sizes with more factors of "
+                            "two", g.nxc, g.nyc, g.nzc, B.n);
+    if (lev.size() < 2)
+        FAIL(CFDMX_ERR_ARG, "multigrid: the grid %dx%dx%d needs at least one coarse level",
+             g.nxc, g.nyc, g.nzc);
+    std::vector<double> inv;
+    if (!coarse_inverse(B.nx, B.ny, B.nz, cd[0], cd[1], cd[2], B.zper, inv))
+        FAIL(CFDMX_ERR_ARG, "multigrid: the coarsest operator is not positive definite");
+    std::vector<T> invT(inv.begin(), inv.end());
+    T *dm = nullptr;
+    ALLOC(dm, invT.size());
+    HIPCHK(hipMemcpy(dm, invT.data(), invT.size() * sizeof(T), hipMemcpyHostToDevice));
+    B.minv = dm;
+    ALLOC(ctx->pbs, 1);
+    ALLOC(ctx->pb_part, 2 * (size_t)g.nblocks + 64);
+    ALLOC(ctx->mix, 4);
+    HIPCHK(hipHostMalloc((void **)&ctx->pbs_h, sizeof(PBState), hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void **)&ctx->mix_h, 4 * sizeof(double), hipHostMallocDefault));
+    return CFDMX_OK;
+}
+
 // metric tables of one axis (cfdmx_device.hpp:Axis) appended to buf
 static void axis_tables(const std::vector<double> &D, const std::vector<double> &Ds,
                         std::vector<double> &buf, size_t off[13])
@@ -377,65 +537,9 @@ static int setup_device(cfdmx_ctx *ctx)
     HIPCHK(hipHostMalloc((void **)&ctx->scal_h, 16 * sizeof(double), hipHostMallocDefault));
     HIPCHK(hipHostMalloc((void **)&ctx->count_h, sizeof(int), hipHostMallocDefault));
     if (c.precision == CFDMX_PRECISION_MIXED) {
-        auto falloc = [&](float **z, size_t n) -> int {
-            HIPCHK(hipMalloc((void **)z, (n > 0 ? n : 1) * sizeof(float)));
-            ctx->allocs.push_back(*z);
-            return CFDMX_OK;
-        };
-        for (int q = 0; q < 8; ++q)
-            if (int rc = falloc(&ctx->pb[q], N)) return rc;
-        if (int rc = falloc(&ctx->e32, N)) return rc;
-        // multigrid levels: level 0 is the SPE operator of the uniform grid
-        MGLevel L = {};
-        L.nx = g.nxc;
-        L.ny = g.nyc;
-        L.nz = g.nzc;
-        L.n = (int)N;
-        double h[3] = {ctx->Dx[kGC], ctx->Dy[kGC], ctx->Dz[kGC]};
-        L.cx = (float)(1.0 / (ctx->Dx[kGC] * ctx->Dxs[kGC]));
-        L.cy = (float)(1.0 / (ctx->Dy[kGC] * ctx->Dys[kGC]));
-        L.cz = (float)(1.0 / (ctx->Dz[kGC] * ctx->Dzs[kGC]));
-        L.fx = L.fy = L.fz = 1;
-        L.zper = g.zper;
-        ctx->mg.clear();
-        ctx->mg.push_back(L);
-        for (;;) {
-            const MGLevel &P = ctx->mg.back();
-            if (P.n <= 512) break;
-            const double hmin = std::min(h[0], std::min(h[1], h[2]));
-            const int fx = (P.nx % 2 == 0 && P.nx >= 4 && h[0] <= 2.0 * hmin) ? 2 : 1;
-            const int fy = (P.ny % 2 == 0 && P.ny >= 4 && h[1] <= 2.0 * hmin) ? 2 : 1;
-            const int fz = (P.nz % 2 == 0 && P.nz >= 2 && h[2] <= 2.0 * hmin) ? 2 : 1;
-            if (fx * fy * fz == 1) break;
-            MGLevel C = {};
-            C.nx = P.nx / fx;
-            C.ny = P.ny / fy;
-            C.nz = P.nz / fz;
-            C.n = C.nx * C.ny * C.nz;
-            h[0] *= fx;
-            h[1] *= fy;
-            h[2] *= fz;
-            C.cx = (float)(1.0 / (h[0] * h[0]));
-            C.cy = (float)(1.0 / (h[1] * h[1]));
-            C.cz = (float)(1.0 / (h[2] * h[2]));
-            C.fx = fx;
-            C.fy = fy;
-            C.fz = fz;
-            C.zper = g.zper;
-            if (int rc = falloc(&C.u, (size_t)C.n)) return rc;
-            if (int rc = falloc(&C.f, (size_t)C.n)) return rc;
-            ctx->mg.push_back(C);
-        }
-        if (ctx->mg.back().n > 512)
-            FAIL(CFDMX_ERR_ARG, "mixed precision: the multigrid hierarchy of %dx%dx%d stops at "
-                                "%d cells (the coarse solve takes at most 512); use sizes "
-                                "with more factors of two", g.nxc, g.nyc, g.nzc,
-                 ctx->mg.back().n);
-        ALLOC(ctx->pbs, 1);
-        ALLOC(ctx->pb_part, 2 * (size_t)g.nblocks + 64);
-        ALLOC(ctx->mix, 4);
-        HIPCHK(hipHostMalloc((void **)&ctx->pbs_h, sizeof(PBState), hipHostMallocDefault));
-        HIPCHK(hipHostMalloc((void **)&ctx->mix_h, 4 * sizeof(double), hipHostMallocDefault));
+        if (int rc = setup_multigrid<float>(ctx)) return rc;
+    } else if (c.precision == CFDMX_PRECISION_FP64_MG) {
+        if (int rc = setup_multigrid<double>(ctx)) return rc;
     }
 
     // reference initial state: u = 1, v = w = 0 in T0/T1/T3 (RUN_CPU.cpp:66-78),
@@ -484,7 +588,11 @@ static int validate(cfdmx_ctx *ctx)
     if (3.0 * G >= 2147483647.0)
         FAIL(CFDMX_ERR_ARG, "grid too large for 32-bit indexing (3*G = %.0f)", 3.0 * G);
     const double N = (double)c.nx * c.ny * c.nz;
-    if (N * c.nz >= 1099511627776.0 || (N / c.nz) * c.ny >= 1099511627776.0)
+    // FastDiv: n / d is exact for n * d < 2^40, and n * ceil(2^40 / d) must fit 64 bits,
+    // i.e. n < d * 2^24; the largest quotients taken are (i, j) row indices, so the
+    // second bound is nx * ny < 2^24
+    if (N * c.nz >= 1099511627776.0 || (N / c.nz) * c.ny >= 1099511627776.0 ||
+        (double)c.nx * c.ny >= 16777216.0)
         FAIL(CFDMX_ERR_ARG, "grid too large for the index decoder");
     if (!(c.dt > 0.0) || !(c.re > 0.0)) FAIL(CFDMX_ERR_ARG, "dt and re must be positive");
     if (c.solver != CFDMX_SOLVER_BICGSTAB && c.solver != CFDMX_SOLVER_RBSOR)
@@ -492,16 +600,18 @@ static int validate(cfdmx_ctx *ctx)
     if (c.precond != CFDMX_PRECOND_NONE && c.precond != CFDMX_PRECOND_JACOBI)
         FAIL(CFDMX_ERR_ARG, "unknown precond %d", c.precond);
     if (c.maxit < 0) FAIL(CFDMX_ERR_ARG, "maxit must be >= 0");
-    if (c.precision != CFDMX_PRECISION_FP64 && c.precision != CFDMX_PRECISION_MIXED)
+    if (c.precision != CFDMX_PRECISION_FP64 && c.precision != CFDMX_PRECISION_MIXED &&
+        c.precision != CFDMX_PRECISION_FP64_MG)
         FAIL(CFDMX_ERR_ARG, "unknown precision %d", c.precision);
-    if (c.precision == CFDMX_PRECISION_MIXED) {
+    if (c.precision != CFDMX_PRECISION_FP64) {
         if (c.solver != CFDMX_SOLVER_BICGSTAB)
-            FAIL(CFDMX_ERR_ARG, "mixed precision needs the BiCGSTAB solver");
+            FAIL(CFDMX_ERR_ARG, "the multigrid pressure solve needs the BiCGSTAB solver");
         if (!(c.inner_tol > 0.0 && c.inner_tol < 1.0))
             FAIL(CFDMX_ERR_ARG, "inner_tol must be in (0, 1)");
         if (c.grid_kind != CFDMX_GRID_UNIFORM || c.precond != CFDMX_PRECOND_NONE)
-            FAIL(CFDMX_ERR_ARG, "mixed precision needs the uniform grid and the SPE operator "
-                                "(precond none): its multigrid levels are uniform Laplacians");
+            FAIL(CFDMX_ERR_ARG, "the multigrid pressure solve needs the uniform grid and the "
+                                "SPE operator (precond none): its levels are uniform "
+                                "Laplacians");
     }
     if (c.nz > 600) FAIL(CFDMX_ERR_ARG, "nz above 600 is not supported (LDS ring of the stencil)");
     if (c.z_periodic && c.solver == CFDMX_SOLVER_RBSOR && (c.nz % 2) != 0)
@@ -928,24 +1038,30 @@ static int solve_bicgstab_impl(cfdmx_ctx *ctx, int *iters, double *res)
     return solve_bicgstab_core(ctx, T, iters, res);
 }
 
-// Mixed precision (no reference counterpart): FP64 iterative refinement.  Each outer
-// pass forms r = b - A x in FP64 and stops when ||r|| / ||b|| < tol; otherwise an FP32
-// BiCGSTAB preconditioned by an FP32 multigrid V-cycle (FP64 sums) solves A e = r to
-// the relative tolerance max(inner_tol, 0.3 tol ||b|| / ||r||) and x += e in FP64.  The
-// iteration limit bounds the inner iterations summed over the outer passes.  A pass
-// that raises the FP64 residual (a stalled FP32 Krylov process) is taken back and
-// ends the solve with the previous x.
+// Multigrid pressure solve (no reference counterpart): FP64 iterative refinement.  Each
+// outer pass forms r = b - A x in FP64 and stops when ||r|| / ||b|| < tol; otherwise a
+// BiCGSTAB preconditioned by a multigrid V-cycle (FP64 sums), in float for mixed
+// precision or in double for the FP64 multigrid mode, solves A e = r to the relative
+// tolerance max(inner_tol, 0.3 tol ||b|| / ||r||) and x += e in FP64.  The iteration
+// limit bounds the inner iterations summed over the outer passes.  A pass that raises
+// the FP64 residual (a stalled inner Krylov process) is taken back and ends the solve
+// with the previous x.
 static constexpr int kMaxOuter = 40;
 
-// FP32 BiCGSTAB with the multigrid V-cycle as right preconditioner on A e = r:
+} // extern "C" (the multigrid solve below is templated on its arithmetic type)
+
+// BiCGSTAB in T with the multigrid V-cycle as right preconditioner on A e = r:
 // vectors r, r0, p, v, s, t, p^, s^ (pb[0..7]), solution e; returns the iterations
+template <typename T>
 static int solve_pb_inner(cfdmx_ctx *ctx, double rr, double tol, int maxit, int *iters,
                           double *res)
 {
     hipStream_t s = ctx->stream;
     const Geo &g = ctx->g;
-    float *r = ctx->pb[0], *r0 = ctx->pb[1], *p = ctx->pb[2], *v = ctx->pb[3],
-          *sv = ctx->pb[4], *t = ctx->pb[5], *ph = ctx->pb[6], *sh = ctx->pb[7];
+    T *const *pv = reinterpret_cast<T *const *>(ctx->pb);
+    T *r = pv[0], *r0 = pv[1], *p = pv[2], *v = pv[3], *sv = pv[4], *t = pv[5], *ph = pv[6],
+      *sh = pv[7];
+    T *e = static_cast<T *>(ctx->ecorr);
     PBState ps = {};
     ps.rho = rr; // (r, r0) with r0 = r
     ps.norm0 = std::sqrt(rr);
@@ -954,21 +1070,21 @@ static int solve_pb_inner(cfdmx_ctx *ctx, double rr, double tol, int maxit, int 
     ps.done = (ps.norm0 > 0.0 && maxit > 0) ? 0 : 1;
     *ctx->pbs_h = ps;
     HIPCHK(hipMemcpyAsync(ctx->pbs, ctx->pbs_h, sizeof ps, hipMemcpyHostToDevice, s));
-    std::vector<MGLevel> &L = ctx->mg;
+    std::vector<MGLevelT<T>> &L = mg_levels<T>(ctx);
     const int nlev = (int)L.size();
     for (int it = 0; !ctx->pbs_h->done; ++it) {
-        PROF(CFDMX_K_PB, launch_pb_update_p(g, s, it == 0, r, v, p, ctx->pbs));
+        PROF(CFDMX_K_PB, launch_pb_update_p<T>(g, s, it == 0, r, v, p, ctx->pbs));
         L[0].f = p;
         L[0].u = ph;
-        PROF(CFDMX_K_MG, launch_mg_vcycle(s, L.data(), nlev, 0));
-        PROF(CFDMX_K_PB, launch_pb_spmv(g, s, L[0], 1, ph, v, r0, ctx->part, ctx->pbs));
-        PROF(CFDMX_K_PB, launch_pb_update_s(g, s, ctx->part, r, v, sv, ctx->pbs));
+        PROF(CFDMX_K_MG, launch_mg_vcycle<T>(s, L.data(), nlev, 0));
+        PROF(CFDMX_K_PB, launch_pb_spmv<T>(g, s, L[0], 1, ph, v, r0, ctx->part, ctx->pbs));
+        PROF(CFDMX_K_PB, launch_pb_update_s<T>(g, s, ctx->part, r, v, sv, ctx->pbs));
         L[0].f = sv;
         L[0].u = sh;
-        PROF(CFDMX_K_MG, launch_mg_vcycle(s, L.data(), nlev, 0));
-        PROF(CFDMX_K_PB, launch_pb_spmv(g, s, L[0], 2, sh, t, sv, ctx->part, ctx->pbs));
-        PROF(CFDMX_K_PB, launch_pb_update_xr(g, s, ctx->part, ph, sh, sv, t, r0, ctx->e32, r,
-                                             ctx->pb_part, ctx->pbs));
+        PROF(CFDMX_K_MG, launch_mg_vcycle<T>(s, L.data(), nlev, 0));
+        PROF(CFDMX_K_PB, launch_pb_spmv<T>(g, s, L[0], 2, sh, t, sv, ctx->part, ctx->pbs));
+        PROF(CFDMX_K_PB, launch_pb_update_xr<T>(g, s, ctx->part, ph, sh, sv, t, r0, e, r,
+                                                ctx->pb_part, ctx->pbs));
         launch_pb_end(g, s, ctx->pb_part, ctx->pbs);
         LAUNCHCHK();
         HIPCHK(hipMemcpyAsync(ctx->pbs_h, ctx->pbs, sizeof(PBState), hipMemcpyDeviceToHost, s));
@@ -980,17 +1096,20 @@ static int solve_pb_inner(cfdmx_ctx *ctx, double rr, double tol, int maxit, int 
     return CFDMX_OK;
 }
 
-static int solve_mixed_impl(cfdmx_ctx *ctx, int *iters, double *res)
+template <typename T>
+static int solve_refine(cfdmx_ctx *ctx, int *iters, double *res)
 {
     hipStream_t s = ctx->stream;
     const Geo &g = ctx->g;
     const cfdmx_config &c = ctx->cfg;
+    T *const *pv = reinterpret_cast<T *const *>(ctx->pb);
+    T *e = static_cast<T *>(ctx->ecorr);
     int total = 0, outer = 0;
     double res64 = 0.0, res_prev = 0.0;
     for (;;) {
-        if (outer > 0) PROF(CFDMX_K_BICG_INIT, launch_mixed_update(g, s, 1.0, ctx->x, ctx->e32));
-        PROF(CFDMX_K_BICG_INIT, launch_mixed_resid(g, s, 0, ctx->b, ctx->x, ctx->pb[0],
-                                                   ctx->pb[1], ctx->e32, ctx->part));
+        if (outer > 0) PROF(CFDMX_K_BICG_INIT, launch_mixed_update<T>(g, s, 1.0, ctx->x, e));
+        PROF(CFDMX_K_BICG_INIT,
+             launch_mixed_resid<T>(g, s, 0, ctx->b, ctx->x, pv[0], pv[1], e, ctx->part));
         launch_mixed_fold(g, s, ctx->part, ctx->mix);
         LAUNCHCHK();
         HIPCHK(hipMemcpyAsync(ctx->mix_h, ctx->mix, 3 * sizeof(double), hipMemcpyDeviceToHost,
@@ -1000,7 +1119,7 @@ static int solve_mixed_impl(cfdmx_ctx *ctx, int *iters, double *res)
         const double bnorm = std::sqrt(ctx->mix_h[0]);
         res64 = std::sqrt(ctx->mix_h[1]) / bnorm;
         if (outer > 0 && !(res64 <= res_prev)) {
-            PROF(CFDMX_K_BICG_INIT, launch_mixed_update(g, s, -1.0, ctx->x, ctx->e32));
+            PROF(CFDMX_K_BICG_INIT, launch_mixed_update<T>(g, s, -1.0, ctx->x, e));
             LAUNCHCHK();
             res64 = res_prev;
             break;
@@ -1010,8 +1129,9 @@ static int solve_mixed_impl(cfdmx_ctx *ctx, int *iters, double *res)
         if (total >= c.maxit || outer >= kMaxOuter) break;
         int it = 0;
         double ires = 0.0;
-        int rc = solve_pb_inner(ctx, ctx->mix_h[2], std::fmax(c.inner_tol, 0.3 * c.tol / res64),
-                                c.maxit - total, &it, &ires);
+        int rc = solve_pb_inner<T>(ctx, ctx->mix_h[2],
+                                   std::fmax(c.inner_tol, 0.3 * c.tol / res64), c.maxit - total,
+                                   &it, &ires);
         if (rc) return rc;
         total += it;
         ++outer;
@@ -1023,10 +1143,18 @@ static int solve_mixed_impl(cfdmx_ctx *ctx, int *iters, double *res)
     return CFDMX_OK;
 }
 
+static int solve_mixed_impl(cfdmx_ctx *ctx, int *iters, double *res)
+{
+    return ctx->cfg.precision == CFDMX_PRECISION_MIXED ? solve_refine<float>(ctx, iters, res)
+                                                       : solve_refine<double>(ctx, iters, res);
+}
+
+extern "C" {
+
 int cfdmx_solve_bicgstab(cfdmx_ctx *ctx, int *iters, double *residual)
 {
     if (!ctx) return CFDMX_ERR_ARG;
-    if (ctx->cfg.precision == CFDMX_PRECISION_MIXED) return solve_mixed_impl(ctx, iters, residual);
+    if (ctx->cfg.precision != CFDMX_PRECISION_FP64) return solve_mixed_impl(ctx, iters, residual);
     return solve_bicgstab_impl(ctx, iters, residual);
 }
 
@@ -1150,7 +1278,7 @@ static int step_once(cfdmx_ctx *ctx, int *iters, double *err, int *solved)
         *solved = 1;
         if (c.solver == CFDMX_SOLVER_BICGSTAB) {
             rc = cfdmx_pressure_rhs(ctx);
-            if (!rc) rc = c.precision == CFDMX_PRECISION_MIXED ? solve_mixed_impl(ctx, iters, err)
+            if (!rc) rc = c.precision != CFDMX_PRECISION_FP64 ? solve_mixed_impl(ctx, iters, err)
                                                                : solve_bicgstab_impl(ctx, iters, err);
         } else {
             rc = solve_rbsor_impl(ctx, iters, err);

@@ -76,14 +76,18 @@ template <typename T>
 void launch_bicg_C(const Geo &g, hipStream_t s, BicgBuf<T> &B, T *x, double *const part[3],
                    KState *st, int m);
 void launch_bicg_check(const Geo &g, hipStream_t s, double *const part[3], KState *st, int m);
-// mixed precision: one multigrid level of the FP32 preconditioner (interior cells,
-// n = (i*ny + j)*nz + k) and the scalars of the FP32 preconditioned BiCGSTAB
-struct MGLevel {
+// Multigrid-preconditioned solve (mixed precision: T = float; the same algorithm in
+// FP64: T = double).  One multigrid level (interior cells, n = (i*ny + j)*nz + k) and
+// the scalars of the preconditioned inner BiCGSTAB.
+template <typename T>
+struct MGLevelT {
     int nx, ny, nz, n;
-    float cx, cy, cz; // 1/h^2 per direction
-    int fx, fy, fz;   // coarsening factors from the finer level (1 or 2)
+    T cx, cy, cz;       // 1/h^2 per direction
+    int fx, fy, fz;     // coarsening factors from the finer level (1 or 2)
     int zper;
-    float *u, *f;
+    FastDiv dz, dy, dzh; // divisions by nz, ny and nz/2
+    T *u, *f;
+    const T *minv;      // coarsest level: (A + 1 1^T / n)^-1 (symmetric), n x n
 };
 
 struct PBState {
@@ -91,22 +95,30 @@ struct PBState {
     int iter, maxit, done;
 };
 
-void launch_mg_vcycle(hipStream_t s, const MGLevel *L, int nlev, int l);
-void launch_pb_spmv(const Geo &g, hipStream_t s, const MGLevel &L0, int ns, const float *in,
-                    float *out, const float *w, double *part, const PBState *st);
-void launch_pb_update_p(const Geo &g, hipStream_t s, int first, const float *r, const float *v,
-                        float *p, const PBState *st);
-void launch_pb_update_s(const Geo &g, hipStream_t s, const double *part, const float *r,
-                        const float *v, float *sv, PBState *st);
-void launch_pb_update_xr(const Geo &g, hipStream_t s, const double *part_in, const float *ph,
-                         const float *sh, const float *sv, const float *t, const float *r0,
-                         float *e, float *r, double *part_out, PBState *st);
+template <typename T>
+void launch_mg_vcycle(hipStream_t s, const MGLevelT<T> *L, int nlev, int l);
+template <typename T>
+void launch_pb_spmv(const Geo &g, hipStream_t s, const MGLevelT<T> &L0, int ns, const T *in,
+                    T *out, const T *w, double *part, const PBState *st);
+template <typename T>
+void launch_pb_update_p(const Geo &g, hipStream_t s, int first, const T *r, const T *v, T *p,
+                        const PBState *st);
+template <typename T>
+void launch_pb_update_s(const Geo &g, hipStream_t s, const double *part, const T *r, const T *v,
+                        T *sv, PBState *st);
+template <typename T>
+void launch_pb_update_xr(const Geo &g, hipStream_t s, const double *part_in, const T *ph,
+                         const T *sh, const T *sv, const T *t, const T *r0, T *e, T *r,
+                         double *part_out, PBState *st);
 void launch_pb_end(const Geo &g, hipStream_t s, const double *part, PBState *st);
 
-// mixed precision: FP64 residual into float r, r0 (e = 0), x += e, fold of the three sums
+// iterative refinement: the FP64 residual into r, r0 (e = 0), x += sign*e, fold of the
+// three sums
+template <typename T>
 void launch_mixed_resid(const Geo &g, hipStream_t s, int pre, const double *b, const double *x,
-                        float *r32, float *r0_32, float *e32, double *part);
-void launch_mixed_update(const Geo &g, hipStream_t s, double sign, double *x, const float *e);
+                        T *r, T *r0, T *e, double *part);
+template <typename T>
+void launch_mixed_update(const Geo &g, hipStream_t s, double sign, double *x, const T *e);
 void launch_mixed_fold(const Geo &g, hipStream_t s, const double *part, double *out);
 
 void launch_bicg_fin0(const Geo &g, hipStream_t s, double *part, KState *st);

@@ -1033,12 +1033,14 @@ k_march_C(Geo g, T *__restrict__ r, const T *__restrict__ ap, T *__restrict__ p,
 }
 
 // ---- mixed precision (no reference counterpart): FP64 iterative refinement around
-// an FP32 BiCGSTAB.  The outer residual r = b - A x is formed in FP64 on the same
-// operator rows as k_bicg_init; the inner solve A e = r starts from e = 0 with
-// r0 = p = float(r).  Partials: (b,b), (r,r) in FP64 and (r32,r32).
+// an inner BiCGSTAB in T (float; double for the same algorithm in FP64).  The outer
+// residual r = b - A x is formed in FP64 on the same operator rows as k_bicg_init; the
+// inner solve A e = r starts from e = 0 with r0 = p = T(r).  Partials: (b,b), (r,r) in
+// FP64 and (T(r), T(r)).
+template <typename T>
 __global__ void __launch_bounds__(kThreads)
 k_mixed_resid(Geo g, int pre, const double *__restrict__ b, const double *__restrict__ x,
-              float *__restrict__ r32, float *__restrict__ r0_32, float *__restrict__ e32,
+              T *__restrict__ r32, T *__restrict__ r0_32, T *__restrict__ e32,
               double *__restrict__ part)
 {
     const int L = logical_block(g);
@@ -1055,10 +1057,10 @@ k_mixed_resid(Geo g, int pre, const double *__restrict__ b, const double *__rest
         const double ax = row_apply(rc, n, [&](int m) { return x[m]; });
         const double bn = b[n];
         const double rr = bn - ax;
-        const float rf = (float)rr;
+        const T rf = (T)rr;
         r32[n] = rf;
         r0_32[n] = rf;
-        e32[n] = 0.0f;
+        e32[n] = (T)0;
         acc[0] += bn * bn;
         acc[1] += rr * rr;
         acc[2] += (double)rf * (double)rf;
@@ -1067,8 +1069,9 @@ k_mixed_resid(Geo g, int pre, const double *__restrict__ b, const double *__rest
 }
 
 // x += sign * e (FP64 accumulation of the FP32 correction; sign -1 takes it back)
+template <typename T>
 __global__ void k_mixed_update(int n, double sign, double *__restrict__ x,
-                               const float *__restrict__ e)
+                               const T *__restrict__ e)
 {
     for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x)
         x[q] += sign * (double)e[q];
@@ -1610,16 +1613,27 @@ void launch_bicg_check(const Geo &g, hipStream_t s, double *const part[3], KStat
                        st, m);
 }
 
+template <typename T>
 void launch_mixed_resid(const Geo &g, hipStream_t s, int pre, const double *b, const double *x,
-                        float *r32, float *r0_32, float *e32, double *part)
+                        T *r, T *r0, T *e, double *part)
 {
-    hipLaunchKernelGGL(k_mixed_resid, dim3(grid_blocks(g)), dim3(kThreads), 0, s, g, pre, b, x,
-                       r32, r0_32, e32, part);
+    hipLaunchKernelGGL(k_mixed_resid<T>, dim3(grid_blocks(g)), dim3(kThreads), 0, s, g, pre, b,
+                       x, r, r0, e, part);
 }
-void launch_mixed_update(const Geo &g, hipStream_t s, double sign, double *x, const float *e)
+template <typename T>
+void launch_mixed_update(const Geo &g, hipStream_t s, double sign, double *x, const T *e)
 {
-    hipLaunchKernelGGL(k_mixed_update, dim3(2048), dim3(kThreads), 0, s, g.N, sign, x, e);
+    hipLaunchKernelGGL(k_mixed_update<T>, dim3(2048), dim3(kThreads), 0, s, g.N, sign, x, e);
 }
+template void launch_mixed_resid<float>(const Geo &, hipStream_t, int, const double *,
+                                        const double *, float *, float *, float *, double *);
+template void launch_mixed_resid<double>(const Geo &, hipStream_t, int, const double *,
+                                         const double *, double *, double *, double *,
+                                         double *);
+template void launch_mixed_update<float>(const Geo &, hipStream_t, double, double *,
+                                         const float *);
+template void launch_mixed_update<double>(const Geo &, hipStream_t, double, double *,
+                                          const double *);
 void launch_mixed_fold(const Geo &g, hipStream_t s, const double *part, double *out)
 {
     hipLaunchKernelGGL(k_mixed_fold, dim3(1), dim3(kThreads), 0, s, part, g.nblocks, out);

@@ -1,5 +1,6 @@
-// cfdmx_mg.hip -- FP32 geometric multigrid preconditioner and the FP32 preconditioned
-// BiCGSTAB of the mixed-precision pressure solve (no reference counterpart).
+// cfdmx_mg.hip -- geometric multigrid preconditioner and the preconditioned BiCGSTAB of
+// the multigrid pressure solves (no reference counterpart), in T = float (the
+// mixed-precision inner solve) or T = double (the same algorithm in FP64).
 //
 // The SPE operator on the uniform grid (SPE_sparseMatrix.hpp:42-106) is the 7-point
 // Laplacian (A u)_c = sum over existing neighbours of c_d (u_c - u_nb), c_d = 1/h_d^2,
@@ -9,143 +10,228 @@
 // at most 512 cells or no even size left; its operator is the same Laplacian on the
 // coarse spacings.  Cycle: V(2,2) with red-black Gauss-Seidel, restriction = mean of the
 // children's residuals, prolongation = the parent's value; the coarsest level is solved
-// by one workgroup (mean removed, Gauss-Seidel sweeps in LDS).
-// Vectors are float; every inner product accumulates in double.
+// exactly by a dense (A + 1 1^T / n)^-1 computed on the host, applied to f minus its
+// mean.  Every inner product accumulates in double.
+//
+// Kernels of a level with even nz take one thread per z-pair (cells 2kk and 2kk+1 of a
+// (i, j) row), so a half sweep runs one thread per cell of its colour:
+//   first sweep from u = 0 (red u = f/d, then black from the red values) -- one kernel;
+//   half sweeps;
+//   restriction (one thread per coarse cell);
+//   prolongation fused with the next red half sweep: the red cell reads its black
+//   neighbours corrected on the fly; the black cells are left uncorrected, since the
+//   black half sweep that follows overwrites them without reading them.
+// Levels with odd nz use one thread per cell.
 #include "cfdmx_internal.hpp"
 
 namespace cfdmx {
 
 static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 
-__device__ __forceinline__ void mg_decode(const MGLevel &L, int n, int &i, int &j, int &k)
+template <typename T>
+__device__ __forceinline__ void mg_coords(const MGLevelT<T> &L, int n, int &i, int &j, int &k)
 {
-    k = n % L.nz;
-    const int t = n / L.nz;
-    j = t % L.ny;
-    i = t / L.ny;
+    const unsigned t = fdiv((unsigned)n, L.dz);
+    k = n - (int)t * L.nz;
+    i = (int)fdiv(t, L.dy);
+    j = (int)t - i * L.ny;
 }
 
-// sum of c_d * u_nb over the existing neighbours, and the diagonal
-__device__ __forceinline__ void mg_row(const MGLevel &L, const float *u, int n, int i, int j,
-                                      int k, float &s, float &d)
+// z-pair m of a level with even nz: row (i, j) and the pair's first cell 2kk
+template <typename T>
+__device__ __forceinline__ void mg_pair(const MGLevelT<T> &L, int m, int &i, int &j, int &k0,
+                                        int &row)
 {
-    s = 0.0f;
-    d = 0.0f;
-    const int sx = L.ny * L.nz, sy = L.nz;
-    if (i > 0) { s += L.cx * u[n - sx]; d += L.cx; }
-    if (i < L.nx - 1) { s += L.cx * u[n + sx]; d += L.cx; }
-    if (j > 0) { s += L.cy * u[n - sy]; d += L.cy; }
-    if (j < L.ny - 1) { s += L.cy * u[n + sy]; d += L.cy; }
+    const int half = L.nz >> 1;
+    row = (int)fdiv((unsigned)m, L.dzh);
+    k0 = 2 * (m - row * half);
+    i = (int)fdiv((unsigned)row, L.dy);
+    j = row - i * L.ny;
+}
+
+template <typename T>
+__device__ __forceinline__ T mg_diag(const MGLevelT<T> &L, int i, int j, int k)
+{
+    T d = (T)0;
+    if (i > 0) d += L.cx;
+    if (i < L.nx - 1) d += L.cx;
+    if (j > 0) d += L.cy;
+    if (j < L.ny - 1) d += L.cy;
     if (L.nz > 1) {
         if (L.zper) {
-            s += L.cz * u[k > 0 ? n - 1 : n + L.nz - 1];
-            s += L.cz * u[k < L.nz - 1 ? n + 1 : n - (L.nz - 1)];
-            d += 2.0f * L.cz;
+            d += (T)2 * L.cz;
         } else {
-            if (k > 0) { s += L.cz * u[n - 1]; d += L.cz; }
-            if (k < L.nz - 1) { s += L.cz * u[n + 1]; d += L.cz; }
+            if (k > 0) d += L.cz;
+            if (k < L.nz - 1) d += L.cz;
         }
     }
+    return d;
 }
 
-// red-black Gauss-Seidel half sweep; zero_first: u starts at zero (the other colour
-// is written 0)
-__global__ void k_mg_smooth(MGLevel L, int colour, int zero_first)
+// sum of c_d * v(nb) over the existing neighbours nb of cell n, in the order
+// -x, +x, -y, +y, -z, +z; v(m) gives the neighbour's value
+template <typename T, class V>
+__device__ __forceinline__ T mg_nbsum(const MGLevelT<T> &L, int n, int i, int j, int k, V &&v)
+{
+    const int sx = L.ny * L.nz, sy = L.nz;
+    T s = (T)0;
+    if (i > 0) s += L.cx * v(n - sx, i - 1, j, k);
+    if (i < L.nx - 1) s += L.cx * v(n + sx, i + 1, j, k);
+    if (j > 0) s += L.cy * v(n - sy, i, j - 1, k);
+    if (j < L.ny - 1) s += L.cy * v(n + sy, i, j + 1, k);
+    if (L.nz > 1) {
+        if (L.zper) {
+            s += L.cz * (k > 0 ? v(n - 1, i, j, k - 1) : v(n + L.nz - 1, i, j, L.nz - 1));
+            s += L.cz * (k < L.nz - 1 ? v(n + 1, i, j, k + 1) : v(n - (L.nz - 1), i, j, 0));
+        } else {
+            if (k > 0) s += L.cz * v(n - 1, i, j, k - 1);
+            if (k < L.nz - 1) s += L.cz * v(n + 1, i, j, k + 1);
+        }
+    }
+    return s;
+}
+
+// ---- levels with even nz: one thread per z-pair ----
+
+// red-black Gauss-Seidel half sweep over the cells with (i + j + k) & 1 == colour
+template <typename T>
+__global__ void __launch_bounds__(256) k_mg_smooth(MGLevelT<T> L, int colour)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= (L.n >> 1)) return;
+    int i, j, k0, row;
+    mg_pair(L, m, i, j, k0, row);
+    const int k = k0 + ((i + j + colour) & 1);
+    const int n = row * L.nz + k;
+    const T *u = L.u;
+    const T s = mg_nbsum(L, n, i, j, k, [&](int q, int, int, int) { return u[q]; });
+    L.u[n] = (L.f[n] + s) / mg_diag(L, i, j, k);
+}
+
+// first sweep from u = 0: red u = f/d, then black u = (f + sum c (f/d)(red nb)) / d
+template <typename T>
+__global__ void __launch_bounds__(256) k_mg_smooth_first(MGLevelT<T> L)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= (L.n >> 1)) return;
+    int i, j, k0, row;
+    mg_pair(L, m, i, j, k0, row);
+    const int kr = k0 + ((i + j) & 1), kb = k0 + ((i + j + 1) & 1);
+    const int nr = row * L.nz + kr, nb = row * L.nz + kb;
+    const T *f = L.f;
+    L.u[nr] = f[nr] / mg_diag(L, i, j, kr);
+    const T s = mg_nbsum(L, nb, i, j, kb,
+                         [&](int q, int a, int b, int c) { return f[q] / mg_diag(L, a, b, c); });
+    L.u[nb] = (f[nb] + s) / mg_diag(L, i, j, kb);
+}
+
+// prolongation fused with the red half sweep (see the file comment)
+template <typename T>
+__global__ void __launch_bounds__(256) k_mg_prolong_smooth(MGLevelT<T> F, MGLevelT<T> C)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= (F.n >> 1)) return;
+    int i, j, k0, row;
+    mg_pair(F, m, i, j, k0, row);
+    const int k = k0 + ((i + j) & 1);
+    const int n = row * F.nz + k;
+    const int shx = C.fx >> 1, shy = C.fy >> 1, shz = C.fz >> 1;
+    const T *u = F.u, *e = C.u;
+    const T s = mg_nbsum(F, n, i, j, k, [&](int q, int a, int b, int c) {
+        return u[q] + e[((a >> shx) * C.ny + (b >> shy)) * C.nz + (c >> shz)];
+    });
+    F.u[n] = (F.f[n] + s) / mg_diag(F, i, j, k);
+}
+
+// ---- levels with odd nz: one thread per cell ----
+
+// half sweep; zero_first: u starts at zero (the other colour is written 0)
+template <typename T>
+__global__ void __launch_bounds__(256) k_mg_smooth_cell(MGLevelT<T> L, int colour, int zero_first)
 {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= L.n) return;
     int i, j, k;
-    mg_decode(L, n, i, j, k);
+    mg_coords(L, n, i, j, k);
     if (((i + j + k) & 1) != colour) {
-        if (zero_first) L.u[n] = 0.0f;
+        if (zero_first) L.u[n] = (T)0;
         return;
     }
-    float s, d;
-    if (zero_first) {
-        float dd = 0.0f;
-        if (i > 0) dd += L.cx;
-        if (i < L.nx - 1) dd += L.cx;
-        if (j > 0) dd += L.cy;
-        if (j < L.ny - 1) dd += L.cy;
-        if (L.nz > 1) dd += L.zper ? 2.0f * L.cz : (k > 0 ? L.cz : 0.0f) + (k < L.nz - 1 ? L.cz : 0.0f);
-        L.u[n] = L.f[n] / dd;
-        return;
-    }
-    mg_row(L, L.u, n, i, j, k, s, d);
-    L.u[n] = (L.f[n] + s) / d;
+    const T *u = L.u;
+    const T s = zero_first ? (T)0 : mg_nbsum(L, n, i, j, k, [&](int q, int, int, int) { return u[q]; });
+    L.u[n] = (L.f[n] + s) / mg_diag(L, i, j, k);
 }
 
-// coarse right-hand side: mean over the children of the fine residual f - A u
-__global__ void k_mg_restrict(MGLevel F, MGLevel C)
-{
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= C.n) return;
-    int I, J, K;
-    mg_decode(C, m, I, J, K);
-    float acc = 0.0f;
-    for (int a = 0; a < C.fx; ++a)
-        for (int b = 0; b < C.fy; ++b)
-            for (int c = 0; c < C.fz; ++c) {
-                const int i = I * C.fx + a, j = J * C.fy + b, k = K * C.fz + c;
-                const int n = (i * F.ny + j) * F.nz + k;
-                float s, d;
-                mg_row(F, F.u, n, i, j, k, s, d);
-                acc += F.f[n] - (d * F.u[n] - s);
-            }
-    C.f[m] = acc / (float)(C.fx * C.fy * C.fz);
-}
-
-__global__ void k_mg_prolong(MGLevel F, MGLevel C)
+template <typename T>
+__global__ void __launch_bounds__(256) k_mg_prolong(MGLevelT<T> F, MGLevelT<T> C)
 {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= F.n) return;
     int i, j, k;
-    mg_decode(F, n, i, j, k);
-    F.u[n] += C.u[((i / C.fx) * C.ny + j / C.fy) * C.nz + k / C.fz];
+    mg_coords(F, n, i, j, k);
+    F.u[n] += C.u[((i >> (C.fx >> 1)) * C.ny + (j >> (C.fy >> 1))) * C.nz + (k >> (C.fz >> 1))];
 }
 
-// coarsest level in one workgroup: f minus its mean, then Gauss-Seidel sweeps in LDS
-constexpr int kMGCoarseMax = 512;
-__global__ void __launch_bounds__(kThreads) k_mg_coarse(MGLevel L, int sweeps)
+// coarse right-hand side: mean over the children of the fine residual f - A u
+template <typename T>
+__global__ void __launch_bounds__(256) k_mg_restrict(MGLevelT<T> F, MGLevelT<T> C)
 {
-    __shared__ float u[kMGCoarseMax], f[kMGCoarseMax];
-    __shared__ double red[4];
-    double part = 0.0;
-    for (int n = threadIdx.x; n < L.n; n += kThreads) {
-        f[n] = L.f[n];
-        u[n] = 0.0f;
-        part += (double)L.f[n];
-    }
-    const double w = wave_fold(part);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = w;
-    __syncthreads();
-    const float mean = (float)(((red[0] + red[1]) + (red[2] + red[3])) / (double)L.n);
-    for (int n = threadIdx.x; n < L.n; n += kThreads) f[n] -= mean;
-    __syncthreads();
-    MGLevel S = L;
-    S.u = u;
-    for (int it = 0; it < sweeps; ++it)
-        for (int colour = 0; colour < 2; ++colour) {
-            for (int n = threadIdx.x; n < L.n; n += kThreads) {
-                int i, j, k;
-                mg_decode(L, n, i, j, k);
-                if (((i + j + k) & 1) != colour) continue;
-                float s, d;
-                mg_row(S, u, n, i, j, k, s, d);
-                u[n] = (f[n] + s) / d;
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= C.n) return;
+    int I, J, K;
+    mg_coords(C, m, I, J, K);
+    const T *u = F.u;
+    T acc = (T)0;
+    // at most 2 x 2 x 2 children, unrolled so that their loads are in flight together
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                if (a >= C.fx || b >= C.fy || c >= C.fz) continue;
+                const int i = I * C.fx + a, j = J * C.fy + b, k = K * C.fz + c;
+                const int n = (i * F.ny + j) * F.nz + k;
+                const T s = mg_nbsum(F, n, i, j, k, [&](int q, int, int, int) { return u[q]; });
+                acc += F.f[n] - (mg_diag(F, i, j, k) * u[n] - s);
             }
-            __syncthreads();
-        }
-    for (int n = threadIdx.x; n < L.n; n += kThreads) L.u[n] = u[n];
+    C.f[m] = acc / (T)(C.fx * C.fy * C.fz);
 }
 
-// ---- FP32 preconditioned BiCGSTAB pieces (linear cell order, canonical block sums) ----
-// out = A in on level 0 (float, double accumulation of the 7 terms); partials of
-// (out, w) and, with NS = 2, (out, out)
-template <int NS>
+// coarsest level: u = M^-1 (f - mean f).  One wave per row: lane l takes the columns
+// c = l + 64 t (M is symmetric, so row r is contiguous and the loads coalesce), the
+// wave folds its lanes; every wave forms the mean of f itself, in double.
+constexpr int kMGCoarseMax = 512;
+template <typename T>
+__global__ void __launch_bounds__(256) k_mg_coarse(MGLevelT<T> L)
+{
+    __shared__ T g[kMGCoarseMax];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double part = 0.0;
+    for (int n = lane; n < L.n; n += 64) part += (double)L.f[n];
+    const double mean = __shfl(wave_fold(part), 0, 64) / (double)L.n;
+    for (int n = threadIdx.x; n < L.n; n += 256) g[n] = (T)((double)L.f[n] - mean);
+    __syncthreads();
+    const int r = blockIdx.x * 4 + wave;
+    if (r >= L.n) return;
+    const T *row = L.minv + (size_t)r * L.n;
+    T acc = (T)0;
+#pragma unroll
+    for (int t = 0; t < kMGCoarseMax / 64; ++t) {
+        const int c = lane + 64 * t;
+        if (c < L.n) acc += row[c] * g[c];
+    }
+    for (int sft = 32; sft >= 1; sft >>= 1) acc += __shfl_down(acc, sft, 64);
+    if (lane == 0) L.u[r] = acc;
+}
+
+// ---- preconditioned BiCGSTAB pieces (linear cell order, canonical block sums) ----
+// out = A in on level 0 (double accumulation of the 7 terms); partials of (out, w) and,
+// with NS = 2, (out, out)
+template <int NS, typename T>
 __global__ void __launch_bounds__(kThreads)
-k_pb_spmv(Geo g, MGLevel L, const float *__restrict__ in, float *__restrict__ out,
-          const float *__restrict__ w, double *part, const PBState *st)
+k_pb_spmv(Geo g, MGLevelT<T> L, const T *__restrict__ in, T *__restrict__ out,
+          const T *__restrict__ w, double *part, const PBState *st)
 {
     if (st->done) return;
     const int B = logical_block(g);
@@ -156,7 +242,7 @@ k_pb_spmv(Geo g, MGLevel L, const float *__restrict__ in, float *__restrict__ ou
         const int n = B * kPerBlock + c * kThreads + threadIdx.x;
         if (n < g.N) {
             int i, j, k;
-            mg_decode(L, n, i, j, k);
+            mg_coords(L, n, i, j, k);
             const int sx = L.ny * L.nz, sy = L.nz;
             double s = 0.0, d = 0.0;
             const double cx = L.cx, cy = L.cy, cz = L.cz;
@@ -172,7 +258,7 @@ k_pb_spmv(Geo g, MGLevel L, const float *__restrict__ in, float *__restrict__ ou
                 if (k > 0) { s += cz * in[n - 1]; d += cz; }
                 if (k < L.nz - 1) { s += cz * in[n + 1]; d += cz; }
             }
-            const float o = (float)(d * (double)in[n] - s);
+            const T o = (T)(d * (double)in[n] - s);
             out[n] = o;
             acc[0] += (double)o * (double)w[n];
             if (NS == 2) acc[NS - 1] += (double)o * (double)o;
@@ -182,11 +268,11 @@ k_pb_spmv(Geo g, MGLevel L, const float *__restrict__ in, float *__restrict__ ou
 }
 
 // p = r (first iteration) or r + beta (p - omega v)
-__global__ void k_pb_update_p(Geo g, int first, const float *r, const float *v, float *p,
-                              const PBState *st)
+template <typename T>
+__global__ void k_pb_update_p(Geo g, int first, const T *r, const T *v, T *p, const PBState *st)
 {
     if (st->done) return;
-    const float beta = (float)st->beta, om = (float)st->omega;
+    const T beta = (T)st->beta, om = (T)st->omega;
     for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < g.N; n += gridDim.x * blockDim.x)
         p[n] = first ? r[n] : r[n] + beta * (p[n] - om * v[n]);
 }
@@ -210,29 +296,31 @@ __global__ void __launch_bounds__(kThreads) k_pb_omega(Geo g, const double *part
 }
 
 // s = r - alpha v
-__global__ void k_pb_update_s(Geo g, const float *r, const float *v, float *s, const PBState *st)
+template <typename T>
+__global__ void k_pb_update_s(Geo g, const T *r, const T *v, T *s, const PBState *st)
 {
     if (st->done) return;
-    const float a = (float)st->alpha;
+    const T a = (T)st->alpha;
     for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < g.N; n += gridDim.x * blockDim.x)
         s[n] = r[n] - a * v[n];
 }
 
 // e += alpha p^ + omega s^; r = s - omega t; partials (r, r), (r, r0)
+template <typename T>
 __global__ void __launch_bounds__(kThreads)
-k_pb_update_xr(Geo g, const float *ph, const float *sh, const float *s, const float *t,
-               const float *r0, float *e, float *r, double *part_out, const PBState *st)
+k_pb_update_xr(Geo g, const T *ph, const T *sh, const T *s, const T *t, const T *r0, T *e, T *r,
+               double *part_out, const PBState *st)
 {
     if (st->done) return;
     const int B = logical_block(g);
     if (B >= g.nblocks) return;
-    const float a = (float)st->alpha, om = (float)st->omega;
+    const T a = (T)st->alpha, om = (T)st->omega;
     double acc[2] = {0.0, 0.0};
     for (int c = 0; c < kCells; ++c) {
         const int n = B * kPerBlock + c * kThreads + threadIdx.x;
         if (n < g.N) {
             e[n] += a * ph[n] + om * sh[n];
-            const float rn = s[n] - om * t[n];
+            const T rn = s[n] - om * t[n];
             r[n] = rn;
             acc[0] += (double)rn * (double)rn;
             acc[1] += (double)rn * (double)r0[n];
@@ -260,25 +348,39 @@ __global__ void __launch_bounds__(kThreads) k_pb_end(Geo g, const double *part, 
 }
 
 // ---- launchers ----
-void launch_mg_vcycle(hipStream_t s, const MGLevel *L, int nlev, int l)
+template <typename T>
+void launch_mg_vcycle(hipStream_t s, const MGLevelT<T> *L, int nlev, int l)
 {
-    const MGLevel &F = L[l];
+    const MGLevelT<T> &F = L[l];
     if (l == nlev - 1) {
-        hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(kThreads), 0, s, F, 60);
+        hipLaunchKernelGGL(k_mg_coarse<T>, dim3(cdiv(F.n, 4)), dim3(256), 0, s, F);
+        return;
+    }
+    const MGLevelT<T> &C = L[l + 1];
+    if ((F.nz & 1) == 0) {
+        const int nb = cdiv(F.n >> 1, 256);
+        hipLaunchKernelGGL(k_mg_smooth_first<T>, dim3(nb), dim3(256), 0, s, F);
+        hipLaunchKernelGGL(k_mg_smooth<T>, dim3(nb), dim3(256), 0, s, F, 0);
+        hipLaunchKernelGGL(k_mg_smooth<T>, dim3(nb), dim3(256), 0, s, F, 1);
+        hipLaunchKernelGGL(k_mg_restrict<T>, dim3(cdiv(C.n, 256)), dim3(256), 0, s, F, C);
+        launch_mg_vcycle(s, L, nlev, l + 1);
+        hipLaunchKernelGGL(k_mg_prolong_smooth<T>, dim3(nb), dim3(256), 0, s, F, C);
+        hipLaunchKernelGGL(k_mg_smooth<T>, dim3(nb), dim3(256), 0, s, F, 1);
+        hipLaunchKernelGGL(k_mg_smooth<T>, dim3(nb), dim3(256), 0, s, F, 0);
+        hipLaunchKernelGGL(k_mg_smooth<T>, dim3(nb), dim3(256), 0, s, F, 1);
         return;
     }
     const int nb = cdiv(F.n, 256);
-    hipLaunchKernelGGL(k_mg_smooth, dim3(nb), dim3(256), 0, s, F, 0, 1);
-    hipLaunchKernelGGL(k_mg_smooth, dim3(nb), dim3(256), 0, s, F, 1, 0);
-    hipLaunchKernelGGL(k_mg_smooth, dim3(nb), dim3(256), 0, s, F, 0, 0);
-    hipLaunchKernelGGL(k_mg_smooth, dim3(nb), dim3(256), 0, s, F, 1, 0);
-    const MGLevel &C = L[l + 1];
-    hipLaunchKernelGGL(k_mg_restrict, dim3(cdiv(C.n, 256)), dim3(256), 0, s, F, C);
+    hipLaunchKernelGGL(k_mg_smooth_cell<T>, dim3(nb), dim3(256), 0, s, F, 0, 1);
+    hipLaunchKernelGGL(k_mg_smooth_cell<T>, dim3(nb), dim3(256), 0, s, F, 1, 0);
+    hipLaunchKernelGGL(k_mg_smooth_cell<T>, dim3(nb), dim3(256), 0, s, F, 0, 0);
+    hipLaunchKernelGGL(k_mg_smooth_cell<T>, dim3(nb), dim3(256), 0, s, F, 1, 0);
+    hipLaunchKernelGGL(k_mg_restrict<T>, dim3(cdiv(C.n, 256)), dim3(256), 0, s, F, C);
     launch_mg_vcycle(s, L, nlev, l + 1);
-    hipLaunchKernelGGL(k_mg_prolong, dim3(nb), dim3(256), 0, s, F, C);
+    hipLaunchKernelGGL(k_mg_prolong<T>, dim3(nb), dim3(256), 0, s, F, C);
     for (int q = 0; q < 2; ++q) {
-        hipLaunchKernelGGL(k_mg_smooth, dim3(nb), dim3(256), 0, s, F, 0, 0);
-        hipLaunchKernelGGL(k_mg_smooth, dim3(nb), dim3(256), 0, s, F, 1, 0);
+        hipLaunchKernelGGL(k_mg_smooth_cell<T>, dim3(nb), dim3(256), 0, s, F, 0, 0);
+        hipLaunchKernelGGL(k_mg_smooth_cell<T>, dim3(nb), dim3(256), 0, s, F, 1, 0);
     }
 }
 
@@ -287,43 +389,62 @@ static inline int lin_blocks(const Geo &g)
     return g.xcd_remap ? ((g.nblocks + 7) / 8) * 8 : g.nblocks;
 }
 
-void launch_pb_spmv(const Geo &g, hipStream_t s, const MGLevel &L0, int ns, const float *in,
-                    float *out, const float *w, double *part, const PBState *st)
+template <typename T>
+void launch_pb_spmv(const Geo &g, hipStream_t s, const MGLevelT<T> &L0, int ns, const T *in,
+                    T *out, const T *w, double *part, const PBState *st)
 {
     if (ns == 1)
-        hipLaunchKernelGGL(k_pb_spmv<1>, dim3(lin_blocks(g)), dim3(kThreads), 0, s, g, L0, in,
-                           out, w, part, st);
+        hipLaunchKernelGGL((k_pb_spmv<1, T>), dim3(lin_blocks(g)), dim3(kThreads), 0, s, g, L0,
+                           in, out, w, part, st);
     else
-        hipLaunchKernelGGL(k_pb_spmv<2>, dim3(lin_blocks(g)), dim3(kThreads), 0, s, g, L0, in,
-                           out, w, part, st);
+        hipLaunchKernelGGL((k_pb_spmv<2, T>), dim3(lin_blocks(g)), dim3(kThreads), 0, s, g, L0,
+                           in, out, w, part, st);
 }
 
-void launch_pb_update_p(const Geo &g, hipStream_t s, int first, const float *r, const float *v,
-                        float *p, const PBState *st)
+template <typename T>
+void launch_pb_update_p(const Geo &g, hipStream_t s, int first, const T *r, const T *v, T *p,
+                        const PBState *st)
 {
-    hipLaunchKernelGGL(k_pb_update_p, dim3(2048), dim3(256), 0, s, g, first, r, v, p, st);
+    hipLaunchKernelGGL(k_pb_update_p<T>, dim3(2048), dim3(256), 0, s, g, first, r, v, p, st);
 }
 
-void launch_pb_update_s(const Geo &g, hipStream_t s, const double *part, const float *r,
-                        const float *v, float *sv, PBState *st)
+template <typename T>
+void launch_pb_update_s(const Geo &g, hipStream_t s, const double *part, const T *r, const T *v,
+                        T *sv, PBState *st)
 {
     hipLaunchKernelGGL(k_pb_alpha, dim3(1), dim3(kThreads), 0, s, g, part, st);
-    hipLaunchKernelGGL(k_pb_update_s, dim3(2048), dim3(256), 0, s, g, r, v, sv,
+    hipLaunchKernelGGL(k_pb_update_s<T>, dim3(2048), dim3(256), 0, s, g, r, v, sv,
                        (const PBState *)st);
 }
 
-void launch_pb_update_xr(const Geo &g, hipStream_t s, const double *part_in, const float *ph,
-                         const float *sh, const float *sv, const float *t, const float *r0,
-                         float *e, float *r, double *part_out, PBState *st)
+template <typename T>
+void launch_pb_update_xr(const Geo &g, hipStream_t s, const double *part_in, const T *ph,
+                         const T *sh, const T *sv, const T *t, const T *r0, T *e, T *r,
+                         double *part_out, PBState *st)
 {
     hipLaunchKernelGGL(k_pb_omega, dim3(1), dim3(kThreads), 0, s, g, part_in, st);
-    hipLaunchKernelGGL(k_pb_update_xr, dim3(lin_blocks(g)), dim3(kThreads), 0, s, g, ph, sh, sv,
-                       t, r0, e, r, part_out, (const PBState *)st);
+    hipLaunchKernelGGL(k_pb_update_xr<T>, dim3(lin_blocks(g)), dim3(kThreads), 0, s, g, ph, sh,
+                       sv, t, r0, e, r, part_out, (const PBState *)st);
 }
 
 void launch_pb_end(const Geo &g, hipStream_t s, const double *part, PBState *st)
 {
     hipLaunchKernelGGL(k_pb_end, dim3(1), dim3(kThreads), 0, s, g, part, st);
 }
+
+#define CFDMX_MG_INSTANCES(T)                                                                 \
+    template void launch_mg_vcycle<T>(hipStream_t, const MGLevelT<T> *, int, int);            \
+    template void launch_pb_spmv<T>(const Geo &, hipStream_t, const MGLevelT<T> &, int,       \
+                                    const T *, T *, const T *, double *, const PBState *);    \
+    template void launch_pb_update_p<T>(const Geo &, hipStream_t, int, const T *, const T *,  \
+                                        T *, const PBState *);                                \
+    template void launch_pb_update_s<T>(const Geo &, hipStream_t, const double *, const T *,  \
+                                        const T *, T *, PBState *);                           \
+    template void launch_pb_update_xr<T>(const Geo &, hipStream_t, const double *, const T *, \
+                                         const T *, const T *, const T *, const T *, T *,     \
+                                         T *, double *, PBState *);
+CFDMX_MG_INSTANCES(float)
+CFDMX_MG_INSTANCES(double)
+#undef CFDMX_MG_INSTANCES
 
 } // namespace cfdmx

@@ -44,6 +44,9 @@ extern "C" {
 
 #define CFDMX_PRECISION_FP64 0
 #define CFDMX_PRECISION_MIXED 1 /* FP32 inner Krylov, FP64 outer refinement */
+/* the mixed mode's algorithm (multigrid-preconditioned inner BiCGSTAB, FP64 refinement)
+   with the inner solve in FP64 as well: the same-algorithm FP64 comparison */
+#define CFDMX_PRECISION_FP64_MG 2
 
 typedef struct cfdmx_config {
     /* grid (class grid, structureGrid.hpp:20-40, made runtime) */
@@ -67,7 +70,7 @@ typedef struct cfdmx_config {
     double sor_omega;
     int skip_first_pressure; /* RUN_CPU.cpp:303: no pressure solve on step 1 */
     int precision;
-    double inner_tol; /* mixed precision: relative tolerance of each FP32 inner solve
+    double inner_tol; /* multigrid modes: relative tolerance of each inner solve
                          (default 1e-5) */
     /* launch shaping */
     int xcd_remap; /* map consecutive logical blocks to one XCD (0 = hardware order) */

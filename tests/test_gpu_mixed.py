@@ -1,32 +1,38 @@
-"""Mixed precision: FP32 BiCGSTAB preconditioned by an FP32 multigrid V-cycle (float
-vectors, FP64 sums) inside FP64 iterative refinement of the pressure residual.  It has
-no counterpart in the reference, so it is checked against the FP64 oracle to the
-tolerances the mode promises: the pressure system's FP64 residual, recomputed on the CPU
+"""Multigrid pressure solves: a BiCGSTAB preconditioned by a multigrid V-cycle inside
+FP64 iterative refinement of the pressure residual, with the inner solve in FP32 (mixed
+precision: float vectors, FP64 sums) or in FP64 (the same algorithm in FP64).  They have
+no counterpart in the reference, so they are checked against the FP64 oracle to the
+tolerances the modes promise: the pressure system's FP64 residual, recomputed on the CPU
 with the oracle's operator, is below the requested tolerance, and velocities, pressure
-and Cd/Cl after several time steps agree with the FP64 oracle run to 1e-6 relative L2.
-The multigrid preconditioner is checked by what it must deliver: a few inner iterations
-per solve, independent of the grid size."""
+and Cd/Cl after several time steps agree with the FP64 oracle run to 1e-6 relative L2
+(mixed) or 1e-8 (FP64 multigrid; both solve to tol 1e-10, so what is left is the
+difference between two solutions within that tolerance).  The multigrid preconditioner
+is checked by what it must deliver: a few inner iterations per solve, independent of
+the grid size."""
 import numpy as np
 import pytest
 
 import oracle_py as op
-from cfd_nssolver_forhpc_flowpassingcylinder_amd.api import PRECISION_MIXED
+from cfd_nssolver_forhpc_flowpassingcylinder_amd.api import PRECISION_FP64_MG, PRECISION_MIXED
 from gpu_helpers import make_pair
 
 pytestmark = pytest.mark.gpu
 
-REL_TOL = 1e-6  # velocity / pressure / Cd, Cl error bound of the mixed mode
+# velocity / pressure / Cd, Cl error bound of each multigrid mode
+REL_TOL = {PRECISION_MIXED: 1e-6, PRECISION_FP64_MG: 1e-8}
+PRECISIONS = [PRECISION_MIXED, PRECISION_FP64_MG]
 
 
 def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
+@pytest.mark.parametrize("precision", PRECISIONS)
 @pytest.mark.parametrize("n", [(48, 24, 8), (64, 32, 16), (128, 64, 32)])
-def test_mixed_solve_reaches_fp64_residual(n):
+def test_mixed_solve_reaches_fp64_residual(n, precision):
     tol, precond = 1e-10, 0
     gpu, sim, o, g = make_pair(*n, bc="cylinder", precond=precond, tol=tol,
-                               precision=PRECISION_MIXED)
+                               precision=precision)
     gpu.convdiff()
     o.convdiff(g, 0.002, 1.0 / 100.0, sim.field("u0"), sim.field("v0"), sim.field("w0"),
                sim.field("u1"), sim.field("v1"), sim.field("w1"), z_periodic=1)
@@ -53,12 +59,13 @@ def test_mixed_solve_reaches_fp64_residual(n):
 
 # consistent pressure systems only (flux-corrected outlet, periodic z): at tol 1e-10 an
 # inconsistent system ends at the iteration limit in either precision
+@pytest.mark.parametrize("precision", PRECISIONS)
 @pytest.mark.parametrize("kind,n,precond,steps", [(0, (48, 24, 8), 0, 6),
                                                   (0, (64, 32, 16), 0, 4)])
-def test_mixed_time_steps_against_fp64_oracle(kind, n, precond, steps):
+def test_mixed_time_steps_against_fp64_oracle(kind, n, precond, steps, precision):
     tol = 1e-10
     gpu, sim, o, g = make_pair(*n, bc="cylinder", kind=kind, precond=precond, tol=tol,
-                               precision=PRECISION_MIXED)
+                               precision=precision)
     for q in range(steps):
         info = gpu.step(1)
         out = sim.step()
@@ -77,25 +84,47 @@ def test_mixed_time_steps_against_fp64_oracle(kind, n, precond, steps):
     cd, cl, ocd, ocl = info["cd"], info["cl"], out["cd"], out["cl"]
     errs["cd"] = abs(cd - ocd) / abs(ocd)
     print("relative errors vs FP64 oracle:", {k: f"{v:.2e}" for k, v in errs.items()})
+    bound = REL_TOL[precision]
     for k in ("u", "p", "cd"):
-        assert errs[k] <= REL_TOL, (k, errs)
+        assert errs[k] <= bound, (k, errs)
     # v, w and Cl are O(1e-3) perturbations here; bound them against |u|
     for name, got in zip(("v", "w"), gpu.download_velocity(0)[1:]):
         ref = sim.field(name + "0")
-        assert np.linalg.norm(got - ref) / np.linalg.norm(sim.field("u0")) <= REL_TOL
-    assert abs(cl - ocl) <= REL_TOL * abs(ocd)
+        assert np.linalg.norm(got - ref) / np.linalg.norm(sim.field("u0")) <= bound
+    assert abs(cl - ocl) <= bound * abs(ocd)
     gpu.close()
 
 
-def test_mixed_configuration_limits():
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_mixed_configuration_limits(precision):
     import cfd_nssolver_forhpc_flowpassingcylinder_amd as pkg
     from cfd_nssolver_forhpc_flowpassingcylinder_amd.api import CfdmxError
     # multigrid levels are uniform SPE Laplacians: no Jacobi rows, no SML grid
     for kw in (dict(precond=1), dict(grid_kind=1, nx=112, ny=104)):
-        cfg = dict(nx=48, ny=24, nz=8, precision=PRECISION_MIXED)
+        cfg = dict(nx=48, ny=24, nz=8, precision=precision)
         cfg.update(kw)
         with pytest.raises(CfdmxError):
             pkg.FlowSolver(pkg.default_config(**cfg))
     # a hierarchy that cannot reach 512 cells is refused at creation
     with pytest.raises(CfdmxError, match="multigrid"):
-        pkg.FlowSolver(pkg.default_config(nx=45, ny=27, nz=7, precision=PRECISION_MIXED))
+        pkg.FlowSolver(pkg.default_config(nx=45, ny=27, nz=7, precision=precision))
+
+
+def test_odd_nz_levels_take_the_per_cell_kernels():
+    """nz = 5 (lz chosen so that all spacings are equal): the levels 48x24x5 and 24x12x5
+    have odd nz, so the V-cycle runs its one-thread-per-cell smoother and prolongation
+    (coarse level 12x6x5); the solve must still reach the FP64 tolerance."""
+    tol = 1e-10
+    gpu, sim, o, g = make_pair(48, 24, 5, lengths=(20.0, 10.0, 20.0 * 5 / 48), bc="cylinder",
+                               tol=tol, precision=PRECISION_MIXED)
+    gpu.convdiff()
+    gpu.pressure_rhs()
+    o.convdiff(g, 0.002, 1.0 / 100.0, sim.field("u0"), sim.field("v0"), sim.field("w0"),
+               sim.field("u1"), sim.field("v1"), sim.field("w1"), z_periodic=1)
+    b = o.rhs(g, 0.002, sim.field("u1"), sim.field("v1"), sim.field("w1"), 0, 1)
+    it, res = gpu.solve_bicgstab()
+    r = b - o.apply_A(g, gpu.xresult(), 0, 1)
+    rel = np.linalg.norm(r) / np.linalg.norm(b)
+    print(f"48x24x5: {it} inner iterations, residual {rel:.3e}")
+    assert rel < tol and it <= 60, (it, rel)
+    gpu.close()
