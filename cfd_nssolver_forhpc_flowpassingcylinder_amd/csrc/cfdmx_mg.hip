@@ -69,24 +69,25 @@ __device__ __forceinline__ T mg_diag(const MGLevelT<T> &L, int i, int j, int k)
 }
 
 // sum of c_d * v(nb) over the existing neighbours nb of cell n, in the order
-// -x, +x, -y, +y, -z, +z; v(m) gives the neighbour's value
+// -x, +x, -y, +y, -z, +z; v(m, a, b, c) gives the value of cell m = (a, b, c).  A
+// missing neighbour is taken as the cell itself with weight 0 (adding +0 changes no
+// sum), so that every value is fetched without a branch and the loads overlap.
 template <typename T, class V>
 __device__ __forceinline__ T mg_nbsum(const MGLevelT<T> &L, int n, int i, int j, int k, V &&v)
 {
     const int sx = L.ny * L.nz, sy = L.nz;
+    const bool xm = i > 0, xp = i < L.nx - 1, ym = j > 0, yp = j < L.ny - 1;
     T s = (T)0;
-    if (i > 0) s += L.cx * v(n - sx, i - 1, j, k);
-    if (i < L.nx - 1) s += L.cx * v(n + sx, i + 1, j, k);
-    if (j > 0) s += L.cy * v(n - sy, i, j - 1, k);
-    if (j < L.ny - 1) s += L.cy * v(n + sy, i, j + 1, k);
+    s += (xm ? L.cx : (T)0) * v(xm ? n - sx : n, xm ? i - 1 : i, j, k);
+    s += (xp ? L.cx : (T)0) * v(xp ? n + sx : n, xp ? i + 1 : i, j, k);
+    s += (ym ? L.cy : (T)0) * v(ym ? n - sy : n, i, ym ? j - 1 : j, k);
+    s += (yp ? L.cy : (T)0) * v(yp ? n + sy : n, i, yp ? j + 1 : j, k);
     if (L.nz > 1) {
-        if (L.zper) {
-            s += L.cz * (k > 0 ? v(n - 1, i, j, k - 1) : v(n + L.nz - 1, i, j, L.nz - 1));
-            s += L.cz * (k < L.nz - 1 ? v(n + 1, i, j, k + 1) : v(n - (L.nz - 1), i, j, 0));
-        } else {
-            if (k > 0) s += L.cz * v(n - 1, i, j, k - 1);
-            if (k < L.nz - 1) s += L.cz * v(n + 1, i, j, k + 1);
-        }
+        const bool zm = L.zper || k > 0, zp = L.zper || k < L.nz - 1;
+        const int km = k > 0 ? k - 1 : (zm ? L.nz - 1 : k);
+        const int kp = k < L.nz - 1 ? k + 1 : (zp ? 0 : k);
+        s += (zm ? L.cz : (T)0) * v(n + (km - k), i, j, km);
+        s += (zp ? L.cz : (T)0) * v(n + (kp - k), i, j, kp);
     }
     return s;
 }
