@@ -518,13 +518,19 @@ __device__ __forceinline__ void row_coef(const Geo &g, int i, int j, int k, RowC
     }
 }
 
+// The seven values are fetched unconditionally (a missing neighbour reads the cell
+// itself) and a missing term adds +0.0: the sum starts at +0.0 and so is never -0.0,
+// which makes +0.0 an exact identity -- the result equals skipping the term.  Without a
+// branch around each load, all seven are in flight together.
 template <class F>
 __device__ __forceinline__ double row_apply(const RowCoef &rc, int n, F &&val)
 {
+    double x[7];
+#pragma unroll
+    for (int q = 0; q < 7; ++q) x[q] = val(rc.has[q] ? n + rc.off[q] : n);
     double s = 0.0;
 #pragma unroll
-    for (int q = 0; q < 7; ++q)
-        if (rc.has[q]) s += rc.v[q] * val(n + rc.off[q]);
+    for (int q = 0; q < 7; ++q) s += rc.has[q] ? rc.v[q] * x[q] : 0.0;
     return s;
 }
 

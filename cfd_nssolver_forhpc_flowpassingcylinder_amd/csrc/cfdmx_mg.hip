@@ -244,21 +244,23 @@ k_pb_spmv(Geo g, MGLevelT<T> L, const T *__restrict__ in, T *__restrict__ out,
         if (n < g.N) {
             int i, j, k;
             mg_coords(L, n, i, j, k);
+            // branch-free: a missing neighbour reads the cell itself with weight 0
             const int sx = L.ny * L.nz, sy = L.nz;
-            double s = 0.0, d = 0.0;
+            const bool xm = i > 0, xp = i < L.nx - 1, ym = j > 0, yp = j < L.ny - 1;
+            const bool zm = L.zper || k > 0, zp = L.zper || k < L.nz - 1;
             const double cx = L.cx, cy = L.cy, cz = L.cz;
-            if (i > 0) { s += cx * in[n - sx]; d += cx; }
-            if (i < L.nx - 1) { s += cx * in[n + sx]; d += cx; }
-            if (j > 0) { s += cy * in[n - sy]; d += cy; }
-            if (j < L.ny - 1) { s += cy * in[n + sy]; d += cy; }
-            if (L.zper) {
-                s += cz * in[k > 0 ? n - 1 : n + L.nz - 1];
-                s += cz * in[k < L.nz - 1 ? n + 1 : n - (L.nz - 1)];
-                d += 2.0 * cz;
-            } else {
-                if (k > 0) { s += cz * in[n - 1]; d += cz; }
-                if (k < L.nz - 1) { s += cz * in[n + 1]; d += cz; }
-            }
+            const double wxm = xm ? cx : 0.0, wxp = xp ? cx : 0.0, wym = ym ? cy : 0.0,
+                         wyp = yp ? cy : 0.0, wzm = zm ? cz : 0.0, wzp = zp ? cz : 0.0;
+            const int km = k > 0 ? n - 1 : (L.zper ? n + L.nz - 1 : n);
+            const int kp = k < L.nz - 1 ? n + 1 : (L.zper ? n - (L.nz - 1) : n);
+            double s = 0.0;
+            s += wxm * (double)in[xm ? n - sx : n];
+            s += wxp * (double)in[xp ? n + sx : n];
+            s += wym * (double)in[ym ? n - sy : n];
+            s += wyp * (double)in[yp ? n + sy : n];
+            s += wzm * (double)in[km];
+            s += wzp * (double)in[kp];
+            const double d = wxm + wxp + wym + wyp + wzm + wzp;
             const T o = (T)(d * (double)in[n] - s);
             out[n] = o;
             acc[0] += (double)o * (double)w[n];
