@@ -203,24 +203,99 @@ def run_mode(wl, args, precision):
 ADJOINT_CASE = dict(nx=256, ny=128, nz=32, re=100.0, dt=0.002, tol=1e-10, nsteps=50, K=10)
 
 
-def adjoint_run(seed):
+SLIP_NUM = [[0, 0, 0], [1, 1, 1], [1, 0, 1], [1, 0, 1], [1, 1, 1], [1, 1, 1]]
+SLIP_DIR = [[1, 0, 0], [0, 0, 0], [0, 0, 0], [0, 0, 0], [0, 0, 0], [0, 0, 0]]
+
+# pressure tolerance of the gradient checks: at tol 1e-10 the solve's own error in the
+# smooth pressure modes (up to the residual times the condition number) moves J by more
+# than the h * dJ/dR that central differences with h = 1e-6 resolve
+GRAD_CHECK_TOL = 1e-13
+
+
+def case_solver(a, seed, precision=0, tol=None, inflow_amp=0.0):
+    """A solver of case a (cylinder BCs, smoothed mask 1.5 cells wide at (5, 5), R = 0.5)
+    and its inflow profile (1 + inflow_amp * seeded noise)."""
     import numpy as np
     import cfd_nssolver_forhpc_flowpassingcylinder_amd as pkg
-    a = ADJOINT_CASE
-    slip_num = [[0, 0, 0], [1, 1, 1], [1, 0, 1], [1, 0, 1], [1, 1, 1], [1, 1, 1]]
-    slip_dir = [[1, 0, 0], [0, 0, 0], [0, 0, 0], [0, 0, 0], [0, 0, 0], [0, 0, 0]]
     cfg = pkg.default_config(
         nx=a["nx"], ny=a["ny"], nz=a["nz"], lx=20.0, ly=10.0, lz=0.5, re=a["re"], dt=a["dt"],
-        tol=a["tol"], maxit=5000, bc_num=slip_num, bc_dir=slip_dir, z_periodic=1,
-        outflow_convective=1, convective_speed=1.0, flux_correction=1, skip_first_pressure=1)
+        tol=tol if tol else a["tol"], maxit=5000, bc_num=SLIP_NUM, bc_dir=SLIP_DIR,
+        z_periodic=1, outflow_convective=1, convective_speed=1.0, flux_correction=1,
+        skip_first_pressure=1, precision=precision, inner_tol=MIXED_INNER_TOL)
     s = pkg.FlowSolver(cfg)
-    rng = np.random.default_rng(seed)
-    prof = np.ones(a["ny"] + 4)
-    prof[2:-2] += 0.01 * (2.0 * rng.random(a["ny"]) - 1.0)
-    s.set_inflow(prof)
+    prof = None
+    if inflow_amp:
+        rng = np.random.default_rng(seed)
+        prof = np.ones(a["ny"] + 4)
+        prof[2:-2] += inflow_amp * (2.0 * rng.random(a["ny"]) - 1.0)
+        s.set_inflow(prof)
     s.init_flow(1.0, 1e-3, seed)
+    s.set_eta_smoothed(5.0, 5.0, 0.5, 1.5 * 20.0 / a["nx"])
+    return s, prof
+
+
+def gradient_check(a, seed, nsteps, K, params, inflow_amp=0.0, h=1e-6, extra_h=()):
+    """Adjoint gradients against central differences (step h) of the same objective, both
+    with the FP64 multigrid pressure solve at tol GRAD_CHECK_TOL.  params: list of
+    ("R" | "cx" | "cy" | inflow index j).  extra_h: further steps for the mask parameters,
+    reported with the one-sided differences (J(p+h) - J)/h and (J - J(p-h))/h, which
+    differ by more than O(h) when a QUICK upwind switch (a transport velocity crossing
+    zero, where the update is not differentiable) lies between p - h and p + h."""
+    import numpy as np
+    s, prof = case_solver(a, seed, precision=2, tol=GRAD_CHECK_TOL, inflow_amp=inflow_amp)
     eps = 1.5 * 20.0 / a["nx"]
-    s.set_eta_smoothed(5.0, 5.0, 0.5, eps)
+    J, gi, gm, info = s.adjoint_drag(nsteps, K)
+    mask = [5.0, 5.0, 0.5]
+    out = {"tol": GRAD_CHECK_TOL, "h": h, "nsteps": nsteps, "checkpoint_every": K, "J": J}
+    for q in params:
+        if isinstance(q, str):
+            m = {"cx": 0, "cy": 1, "R": 2}[q]
+
+            def pm(hh):
+                vals = []
+                for sgn in (1.0, -1.0):
+                    mm = list(mask)
+                    mm[m] += sgn * hh
+                    s.set_eta_smoothed(mm[0], mm[1], mm[2], eps)
+                    vals.append(s.drag_objective(nsteps))
+                s.set_eta_smoothed(mask[0], mask[1], mask[2], eps)
+                return vals
+
+            vals = pm(h)
+            fd, adj, key = (vals[0] - vals[1]) / (2 * h), gm[m], q
+            scale = abs(fd)
+            steps = {}
+            for hh in (h,) + tuple(extra_h):
+                v2 = vals if hh == h else pm(hh)
+                steps[f"{hh:g}"] = {"central": (v2[0] - v2[1]) / (2 * hh),
+                                    "forward": (v2[0] - J) / hh, "backward": (J - v2[1]) / hh}
+                print(f"gradient check {q} h {hh:g}: {steps[f'{hh:g}']}", file=sys.stderr,
+                      flush=True)
+            if extra_h:
+                out[f"{q}_by_step"] = steps
+        else:
+            vals = []
+            for sgn in (1.0, -1.0):
+                pp = prof.copy()
+                pp[q] += sgn * h
+                s.set_inflow(pp)
+                vals.append(s.drag_objective(nsteps))
+            s.set_inflow(prof)
+            fd, adj, key = (vals[0] - vals[1]) / (2 * h), gi[q], f"u_in[{q}]"
+            scale = float(np.max(np.abs(gi)))  # relative to the largest profile entry
+        out[key] = {"adjoint": adj, "central_fd": fd, "rel_diff": abs(adj - fd) / scale}
+        print(f"gradient check {key}: adjoint {adj:.10e} FD {fd:.10e}", file=sys.stderr,
+              flush=True)
+    s.close()
+    out["max_rel_diff"] = max(v["rel_diff"] for v in out.values()
+                              if isinstance(v, dict) and "rel_diff" in v)
+    return out
+
+
+def adjoint_run(seed, fd=True):
+    import numpy as np
+    a = ADJOINT_CASE
+    s, prof = case_solver(a, seed, inflow_amp=0.01)
     t = time.perf_counter()
     J0 = s.drag_objective(a["nsteps"])
     plain = time.perf_counter() - t
@@ -230,7 +305,7 @@ def adjoint_run(seed):
           f"{info['reverse_ms'] / 1e3:.2f} s", file=sys.stderr, flush=True)
     s.close()
     n = a["nsteps"]
-    return {
+    out = {
         "config": (f"{a['nx']}x{a['ny']}x{a['nz']}, Re={a['re']:g}, dt={a['dt']:g}, "
                    f"{n} steps, checkpoint every {a['K']}, tol {a['tol']:g}, FP64; objective "
                    "mean Cd; parameters: 128-point inflow profile, cylinder (cx, cy, R) of a "
@@ -246,6 +321,113 @@ def adjoint_run(seed):
                                    "the reverse pass re-runs each checkpointed segment"),
         "forward_iters": info["forward_iters"], "adjoint_iters": info["adjoint_iters"],
     }
+    if fd:
+        out["gradient_check"] = gradient_check(a, seed, n, a["K"],
+                                               ["R", "cx", "cy", 2 + 20, 2 + 64, 2 + 100],
+                                               inflow_amp=0.01)
+    return out
+
+
+# 1024 x 512 x 128 (67M cells), Re 3900, dt 0.0005, 100 steps: mixed-precision forward
+# and the drag adjoint w.r.t. the cylinder radius with checkpoints every 20 steps; the
+# smoothed mask (1.5 cells) makes the radius differentiable, and the forward uses the
+# same mask so that both solve one problem
+LARGE_CASE = dict(nx=1024, ny=512, nz=128, re=3900.0, dt=0.0005, tol=1e-10, nsteps=100, K=20)
+
+
+def hbm_used_bytes():
+    """Device memory in use (total - free) from hipMemGetInfo."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    free, total = ctypes.c_size_t(), ctypes.c_size_t()
+    if hip.hipMemGetInfo(ctypes.byref(free), ctypes.byref(total)) != 0:
+        return None, None
+    return total.value - free.value, total.value
+
+
+def large_run(seed, nsteps=None, fd=True):
+    import numpy as np
+    import cfd_nssolver_forhpc_flowpassingcylinder_amd as pkg
+    a = dict(LARGE_CASE)
+    if nsteps:
+        a["nsteps"] = nsteps
+    n = a["nsteps"]
+    n_cells = a["nx"] * a["ny"] * a["nz"]
+    shp = (a["nx"] + 4, a["ny"] + 4, a["nz"] + 4)
+    used0, total = hbm_used_bytes()
+
+    def make(precision):
+        return case_solver(a, seed, precision=precision)[0]
+
+    def forward(sv, tag):
+        sv.synchronize()
+        t = time.perf_counter()
+        infos = []
+        for q in range(n):
+            infos.append(sv.step(1))
+            if (q + 1) % 10 == 0:
+                print(f"large {tag}: step {q + 1}, {infos[-1]['iters']} inner iterations, "
+                      f"residual {infos[-1]['err']:.2e}, cd {infos[-1]['cd']:.6f}",
+                      file=sys.stderr, flush=True)
+        sv.synchronize()
+        wall = time.perf_counter() - t
+        u = sv.download_velocity(0)
+        pr = sv.pressure().reshape(shp)[2:-2, 2:-2, 2:-2]
+        return wall, infos, u, pr
+
+    out = {"config": (f"{a['nx']}x{a['ny']}x{a['nz']} uniform over 20x10x0.5, Re={a['re']:g}, "
+                      f"dt={a['dt']:g}, {n} steps, smoothed mask (1.5 cells) at (5, 5) R=0.5, "
+                      f"convective outflow, slip y walls, periodic z, tol {a['tol']:g}; "
+                      f"adjoint of mean Cd w.r.t. R, checkpoint every {a['K']}"),
+           "hbm_total_bytes": total}
+    # FP64 multigrid: the converged FP64 solution of the same steps
+    sg = make(2)
+    gwall, ginfos, gu, gp = forward(sg, "FP64-MG")
+    sg.close()
+    sm = make(1)
+    mwall, minfos, mu, mp = forward(sm, "mixed")
+
+    def rel(x, y):
+        return float(np.linalg.norm(x - y) / np.linalg.norm(y))
+
+    unorm = float(np.linalg.norm(gu[0]))
+    out["mixed_forward"] = {
+        "value": round(n_cells * n / mwall / 1e6, 3), "unit": "Mcell-steps/s",
+        "ms_per_step": round(mwall / n * 1e3, 3),
+        "mean_inner_iters": sum(i["iters"] for i in minfos) / n,
+        "errors_vs_fp64_multigrid": {
+            "u": rel(mu[0], gu[0]), "v": float(np.linalg.norm(mu[1] - gu[1])) / unorm,
+            "w": float(np.linalg.norm(mu[2] - gu[2])) / unorm,
+            "p": rel(mp - mp.mean(), gp - gp.mean()),
+            "cd_last": abs(minfos[-1]["cd"] - ginfos[-1]["cd"]) / abs(ginfos[-1]["cd"])},
+        "cd_last": minfos[-1]["cd"], "cl_last": minfos[-1]["cl"],
+    }
+    out["fp64_multigrid_forward"] = {
+        "value": round(n_cells * n / gwall / 1e6, 3), "unit": "Mcell-steps/s",
+        "ms_per_step": round(gwall / n * 1e3, 3),
+        "mean_inner_iters": sum(i["iters"] for i in ginfos) / n}
+    del gu, gp, mu, mp
+    # adjoint from the initial field (a fresh context of the same configuration)
+    sm.close()
+    sa = make(1)
+    used_pre, _ = hbm_used_bytes()
+    J, gi, gm, info = sa.adjoint_drag(n, a["K"], mask=True, inflow=False)
+    used_adj, _ = hbm_used_bytes()
+    print(f"large adjoint: J {J:.10f}, dJ/dR {gm[2]:.8e}, forward {info['forward_ms']:.0f} "
+          f"ms, reverse {info['reverse_ms']:.0f} ms", file=sys.stderr, flush=True)
+    out["adjoint"] = {
+        "J": J, "dJ_dR": gm[2], "dJ_dcx": gm[0], "dJ_dcy": gm[1],
+        "checkpointed_forward_ms_per_step": round(info["forward_ms"] / n, 3),
+        "reverse_ms_per_step": round(info["reverse_ms"] / n, 3),
+        "adjoint_step_cost": round(info["reverse_ms"] / (mwall * 1e3), 3),
+        "forward_iters": info["forward_iters"], "adjoint_iters": info["adjoint_iters"],
+        "hbm_used_bytes": used_adj, "hbm_used_before_adjoint_bytes": used_pre,
+        "hbm_used_by_process_at_start_bytes": used0,
+    }
+    sa.close()
+    if fd:
+        out["gradient_check"] = gradient_check(a, seed, n, a["K"], ["R"], extra_h=(1e-7, 1e-8))
+    return out
 
 
 def kernel_table(prof, n_cells, bytes_per_cell):
@@ -275,9 +457,14 @@ def main():
     ap.add_argument("--maxit", type=int, default=5000,
                     help="BiCGSTAB iteration cap (5000, the reference default; lower only for "
                          "counter-collection passes, which need few launches)")
-    ap.add_argument("--part", default="all", choices=["all", "mixed", "fp64mg", "adjoint"],
+    ap.add_argument("--part", default="all",
+                    choices=["all", "mixed", "fp64mg", "adjoint", "large"],
                     help="run only the mixed-precision or only the adjoint measurement and "
                          "print its JSON object (for profiling one part)")
+    ap.add_argument("--large-steps", type=int, default=0,
+                    help="steps of --part large (default: the configuration's 100)")
+    ap.add_argument("--no-fd", action="store_true",
+                    help="--part large: skip the finite-difference check of dJ/dR")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB three times (the "
                          "calibration kernel for HBM byte counters) and exit")
@@ -296,6 +483,10 @@ def main():
     n_cells = wl["nx"] * wl["ny"] * wl["nz"]
     if args.part == "adjoint":
         print(json.dumps({"adjoint": adjoint_run(args.seed)}), flush=True)
+        return
+    if args.part == "large":
+        print(json.dumps({"large": large_run(args.seed, args.large_steps or None,
+                                             not args.no_fd)}), flush=True)
         return
     if args.part in ("mixed", "fp64mg"):
         sm, mwarm, minfos, mwall, mprof = run_mode(wl, args, 1 if args.part == "mixed" else 2)

@@ -323,12 +323,18 @@ k_adj_sum_partial(Geo g, const double *a, double *part)
     block_sums<1>(v, part, g.nblocks, L);
 }
 
-__global__ void __launch_bounds__(kThreads)
-k_adj_sub_mean(Geo g, const double *part, double *a)
+// one block: the mean into part[nblocks]
+__global__ void __launch_bounds__(kThreads) k_adj_mean_fold(Geo g, double *part)
 {
     double t[1];
     fold_partials<1>(part, g.nblocks, g.nblocks, t);
-    const double mean = t[0] / (double)g.N;
+    if (threadIdx.x == 0) part[g.nblocks] = t[0] / (double)g.N;
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_adj_sub_mean(Geo g, const double *part, double *a)
+{
+    const double mean = part[g.nblocks];
     for (int n = blockIdx.x * kThreads + threadIdx.x; n < g.N; n += gridDim.x * kThreads)
         a[n] -= mean;
 }
@@ -650,6 +656,7 @@ void launch_adj_remove_mean(const Geo &g, hipStream_t s, double *a, double *part
     const int nb = g.xcd_remap ? ((g.nblocks + 7) / 8) * 8 : g.nblocks;
     hipLaunchKernelGGL(k_adj_sum_partial, dim3(nb), dim3(kThreads), 0, s, g, (const double *)a,
                        part);
+    hipLaunchKernelGGL(k_adj_mean_fold, dim3(1), dim3(kThreads), 0, s, g, part);
     hipLaunchKernelGGL(k_adj_sub_mean, dim3(cdiv(g.N, kThreads)), dim3(kThreads), 0, s, g,
                        (const double *)part, a);
 }

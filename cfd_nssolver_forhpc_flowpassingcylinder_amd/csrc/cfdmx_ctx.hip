@@ -1097,7 +1097,7 @@ static int solve_pb_inner(cfdmx_ctx *ctx, double rr, double tol, int maxit, int 
 }
 
 template <typename T>
-static int solve_refine(cfdmx_ctx *ctx, int *iters, double *res)
+static int solve_refine(cfdmx_ctx *ctx, const double *b, double *x, int *iters, double *res)
 {
     hipStream_t s = ctx->stream;
     const Geo &g = ctx->g;
@@ -1107,9 +1107,8 @@ static int solve_refine(cfdmx_ctx *ctx, int *iters, double *res)
     int total = 0, outer = 0;
     double res64 = 0.0, res_prev = 0.0;
     for (;;) {
-        if (outer > 0) PROF(CFDMX_K_BICG_INIT, launch_mixed_update<T>(g, s, 1.0, ctx->x, e));
-        PROF(CFDMX_K_BICG_INIT,
-             launch_mixed_resid<T>(g, s, 0, ctx->b, ctx->x, pv[0], pv[1], e, ctx->part));
+        if (outer > 0) PROF(CFDMX_K_BICG_INIT, launch_mixed_update<T>(g, s, 1.0, x, e));
+        PROF(CFDMX_K_BICG_INIT, launch_mixed_resid<T>(g, s, 0, b, x, pv[0], pv[1], e, ctx->part));
         launch_mixed_fold(g, s, ctx->part, ctx->mix);
         LAUNCHCHK();
         HIPCHK(hipMemcpyAsync(ctx->mix_h, ctx->mix, 3 * sizeof(double), hipMemcpyDeviceToHost,
@@ -1119,7 +1118,7 @@ static int solve_refine(cfdmx_ctx *ctx, int *iters, double *res)
         const double bnorm = std::sqrt(ctx->mix_h[0]);
         res64 = std::sqrt(ctx->mix_h[1]) / bnorm;
         if (outer > 0 && !(res64 <= res_prev)) {
-            PROF(CFDMX_K_BICG_INIT, launch_mixed_update<T>(g, s, -1.0, ctx->x, e));
+            PROF(CFDMX_K_BICG_INIT, launch_mixed_update<T>(g, s, -1.0, x, e));
             LAUNCHCHK();
             res64 = res_prev;
             break;
@@ -1143,10 +1142,18 @@ static int solve_refine(cfdmx_ctx *ctx, int *iters, double *res)
     return CFDMX_OK;
 }
 
+// the multigrid solve of A x = b (pressure, or the pressure adjoint) in the
+// configured precision
+static int solve_mg_target(cfdmx_ctx *ctx, const double *b, double *x, int *iters, double *res)
+{
+    return ctx->cfg.precision == CFDMX_PRECISION_MIXED
+               ? solve_refine<float>(ctx, b, x, iters, res)
+               : solve_refine<double>(ctx, b, x, iters, res);
+}
+
 static int solve_mixed_impl(cfdmx_ctx *ctx, int *iters, double *res)
 {
-    return ctx->cfg.precision == CFDMX_PRECISION_MIXED ? solve_refine<float>(ctx, iters, res)
-                                                       : solve_refine<double>(ctx, iters, res);
+    return solve_mg_target(ctx, ctx->b, ctx->x, iters, res);
 }
 
 extern "C" {
@@ -1370,8 +1377,9 @@ int cfdmx_profile_read(cfdmx_ctx *ctx, int kernel, double *total_ms, long long *
 // The forward pass keeps a checkpoint (velocity, pressure, X_result, history vectors,
 // solver state) every K steps; the reverse pass restores each checkpoint, recomputes
 // its K steps onto a tape, and runs the transposed step kernels back through them.
-// The pressure adjoint reuses the FP64 BiCGSTAB (A is symmetric on the uniform grid)
-// with its own solution, history and state.  The context is returned to the state
+// The pressure adjoint reuses the forward pressure solver (A is symmetric on the
+// uniform grid): the FP64 BiCGSTAB with its own solution, history and state, or, in
+// the multigrid precisions, the multigrid-preconditioned solve with FP64 refinement.  The context is returned to the state
 // at entry (start boundary conditions applied).
 static int adj_alloc(cfdmx_ctx *ctx, int K, int ncp)
 {
@@ -1484,9 +1492,9 @@ static int adj_check(cfdmx_ctx *ctx, int nsteps, int K)
     const cfdmx_config &c = ctx->cfg;
     if (nsteps < 1 || K < 1) FAIL(CFDMX_ERR_ARG, "nsteps and checkpoint_every must be >= 1");
     if (c.grid_kind != CFDMX_GRID_UNIFORM || c.precond != CFDMX_PRECOND_NONE ||
-        c.solver != CFDMX_SOLVER_BICGSTAB || c.precision != CFDMX_PRECISION_FP64)
+        c.solver != CFDMX_SOLVER_BICGSTAB)
         FAIL(CFDMX_ERR_ARG, "the drag adjoint needs the uniform grid, the SPE operator "
-                            "(precond none), BiCGSTAB and FP64");
+                            "(precond none) and BiCGSTAB");
     if (!ctx->started) FAIL(CFDMX_ERR_STATE, "call cfdmx_init_flow or cfdmx_start first");
     return CFDMX_OK;
 }
@@ -1614,10 +1622,20 @@ int cfdmx_adjoint_drag(cfdmx_ctx *ctx, int nsteps, int checkpoint_every, double 
             LAUNCHCHK();
             if (A.solved[slot]) {
                 // pressure: A mu = lx, mean-free, then the right-hand side's transpose
-                SolveTarget T = {A.lx, A.mu, &A.mpx1, &A.mpx2, A.kadj, A.kadj_h};
                 int it = 0;
                 double res = 0.0;
-                if ((rc = solve_bicgstab_core(ctx, T, &it, &res))) return rc;
+                if (c.precision == CFDMX_PRECISION_FP64) {
+                    SolveTarget T = {A.lx, A.mu, &A.mpx1, &A.mpx2, A.kadj, A.kadj_h};
+                    if ((rc = solve_bicgstab_core(ctx, T, &it, &res))) return rc;
+                } else {
+                    // the multigrid solve in the forward pass's precision; lx is made
+                    // mean-free first (J does not depend on the mean of X_result, so
+                    // its adjoint has zero mean up to rounding), so that the singular
+                    // system is consistent and the FP64 refinement can reach tol
+                    launch_adj_remove_mean(g, s, A.lx, A.apart);
+                    LAUNCHCHK();
+                    if ((rc = solve_mg_target(ctx, A.lx, A.mu, &it, &res))) return rc;
+                }
                 adj_iters += it;
                 launch_adj_remove_mean(g, s, A.mu, A.apart);
                 launch_adj_rhs(g, s, c.dt, A.mu, A.ls);

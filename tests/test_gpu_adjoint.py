@@ -8,6 +8,7 @@ import torch
 
 import cfd_nssolver_forhpc_flowpassingcylinder_amd as pkg
 import oracle_py as op
+from cfd_nssolver_forhpc_flowpassingcylinder_amd.api import PRECISION_FP64_MG, PRECISION_MIXED
 from cfd_nssolver_forhpc_flowpassingcylinder_amd.autograd import drag_objective
 
 pytestmark = pytest.mark.gpu
@@ -15,7 +16,7 @@ pytestmark = pytest.mark.gpu
 SEED = 20261202
 
 
-def _pair(nxc, nyc, nzc, nsteps, skip_first=1, tol=1e-13, re=100.0, dt=0.002):
+def _pair(nxc, nyc, nzc, nsteps, skip_first=1, tol=1e-13, re=100.0, dt=0.002, precision=0):
     rng = np.random.default_rng(7)
     inflow = np.ones(nyc + 4)
     inflow[2:-2] += 0.01 * (2 * rng.random(nyc) - 1)
@@ -24,7 +25,8 @@ def _pair(nxc, nyc, nzc, nsteps, skip_first=1, tol=1e-13, re=100.0, dt=0.002):
     cfg = pkg.default_config(nx=nxc, ny=nyc, nz=nzc, lx=20.0, ly=10.0, lz=0.5, re=re, dt=dt,
                              tol=tol, maxit=5000, bc_num=op.SLIP_NUM, bc_dir=op.SLIP_DIR,
                              z_periodic=1, outflow_convective=1, convective_speed=1.0,
-                             flux_correction=1, skip_first_pressure=skip_first)
+                             flux_correction=1, skip_first_pressure=skip_first,
+                             precision=precision)
     gpu = pkg.FlowSolver(cfg)
     gpu.set_inflow(inflow)
     gpu.init_flow(1.0, 1e-3, SEED)
@@ -109,4 +111,30 @@ def test_torch_autograd_wrapper():
     f = lambda m: drag_objective(inf_t.detach(), m, gpu, nsteps, K, eps)
     assert torch.autograd.gradcheck(f, (m_t.detach().clone().requires_grad_(True),),
                                     eps=1e-6, atol=1e-5, rtol=1e-5)
+    gpu.close()
+
+
+@pytest.mark.parametrize("precision,bound", [(PRECISION_MIXED, 1e-6), (PRECISION_FP64_MG, 1e-8)])
+def test_multigrid_adjoint_against_fp64_adjoint(precision, bound):
+    """The adjoint with the multigrid pressure solves (forward, recompute and the
+    pressure adjoint) against the FP64 path's adjoint of the same objective: every solve
+    reaches tol 1e-12 in FP64, so J and the gradients agree to what the mode promises
+    (1e-6 relative for mixed precision, 1e-8 for the FP64 multigrid)."""
+    nsteps, K, tol = 6, 3, 1e-12
+    ref, *_ = _pair(48, 24, 8, nsteps, tol=tol)
+    J0, gi0, gm0, info0 = ref.adjoint_drag(nsteps, K)
+    ref.close()
+    gpu, *_ = _pair(48, 24, 8, nsteps, tol=tol, precision=precision)
+    J, gi, gm, info = gpu.adjoint_drag(nsteps, K)
+    print(f"J fp64 {J0!r} multigrid {J!r}; iterations fp64 {info0['forward_iters']} + "
+          f"{info0['adjoint_iters']}, multigrid {info['forward_iters']} + "
+          f"{info['adjoint_iters']}")
+    assert abs(J - J0) <= bound * abs(J0)
+    for q in range(3):
+        print(f"mask[{q}]: fp64 {gm0[q]:.12e} multigrid {gm[q]:.12e}")
+        assert abs(gm[q] - gm0[q]) <= bound * max(abs(gm0[q]), 1e-3 * abs(gm0[2]))
+    scale = np.max(np.abs(gi0))
+    assert np.max(np.abs(gi - gi0)) <= bound * scale, np.max(np.abs(gi - gi0)) / scale
+    # recompute from checkpoints is bit-identical to the first forward pass
+    assert gpu.drag_objective(nsteps) == J
     gpu.close()
