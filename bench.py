@@ -464,7 +464,7 @@ def main():
     ap.add_argument("--large-steps", type=int, default=0,
                     help="steps of --part large (default: the configuration's 100)")
     ap.add_argument("--no-fd", action="store_true",
-                    help="--part large: skip the finite-difference check of dJ/dR")
+                    help="--part large / adjoint: skip the finite-difference gradient checks")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB three times (the "
                          "calibration kernel for HBM byte counters) and exit")
@@ -482,7 +482,7 @@ def main():
     wl = WORKLOADS[args.workload]
     n_cells = wl["nx"] * wl["ny"] * wl["nz"]
     if args.part == "adjoint":
-        print(json.dumps({"adjoint": adjoint_run(args.seed)}), flush=True)
+        print(json.dumps({"adjoint": adjoint_run(args.seed, not args.no_fd)}), flush=True)
         return
     if args.part == "large":
         print(json.dumps({"large": large_run(args.seed, args.large_steps or None,
