@@ -33,7 +33,8 @@ EXPORTS = [
     "cfdmx_transform", "cfdmx_project_ib", "cfdmx_swap_t0_t3", "cfdmx_cd_cl", "cfdmx_cfl",
     "cfdmx_apply_operator", "cfdmx_step", "cfdmx_synchronize", "cfdmx_profile",
     "cfdmx_profile_read", "cfdmx_set_tolerance", "cfdmx_set_physics", "cfdmx_upload_force",
-    "cfdmx_drag_objective", "cfdmx_adjoint_drag",
+    "cfdmx_drag_objective", "cfdmx_adjoint_drag", "cfdmx_plot3d_q", "cfdmx_plot3d_flush",
+    "cfdmx_plot3d_x",
 ]
 
 # kernel classes of cfdmx_profile_read
@@ -103,6 +104,9 @@ def load_library(path=LIB_PATH):
               "cfdmx_download_eta", "cfdmx_download_force", "cfdmx_set_inflow"):
         getattr(L, n).argtypes = [vp, DP]
     L.cfdmx_init_flow.argtypes = [vp, C.c_double, C.c_double, C.c_ulonglong]
+    L.cfdmx_plot3d_q.argtypes = [vp, C.c_char_p, C.c_double]
+    L.cfdmx_plot3d_flush.argtypes = [vp]
+    L.cfdmx_plot3d_x.argtypes = [vp, C.c_char_p]
     L.cfdmx_set_eta_cylinder_z.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_int]
     L.cfdmx_set_eta_smoothed.argtypes = [vp] + [C.c_double] * 4
     for n in ("cfdmx_start", "cfdmx_convdiff", "cfdmx_bc_main", "cfdmx_bc_copy",
@@ -240,6 +244,17 @@ class FlowSolver:
 
     def force(self):
         return self._get("cfdmx_download_force", 3 * self.G)
+
+    def plot3d_q(self, path, time):
+        """Stage the Plot3D q file (p, cell-centred u, v, w, eta); written by the next
+        plot3d_q, plot3d_flush or close."""
+        self._chk(self.L.cfdmx_plot3d_q(self.h, os.fsencode(path), time), "plot3d_q")
+
+    def plot3d_flush(self):
+        self._chk(self.L.cfdmx_plot3d_flush(self.h), "plot3d_flush")
+
+    def plot3d_x(self, path):
+        self._chk(self.L.cfdmx_plot3d_x(self.h, os.fsencode(path)), "plot3d_x")
 
     def set_inflow(self, profile):
         if profile is None:

@@ -40,6 +40,16 @@ struct cfdmx_ctx {
     double *pb_part = nullptr;
     double *mix = nullptr, *mix_h = nullptr;
     int mix_outer = 0;
+    // Plot3D q output: staged float arrays on the device, pinned host copy filled on a
+    // second stream, the file written when the next output or a flush comes
+    struct P3D {
+        float *dev = nullptr, *host = nullptr;
+        hipStream_t side = nullptr;
+        hipEvent_t ready = nullptr;
+        bool pending = false;
+        std::string path;
+        float reyn = 0.0f, time = 0.0f;
+    } p3d;
     // smoothed immersed-boundary mask (cfdmx_set_eta_smoothed), the adjoint's parameters
     int mask_smoothed = 0;
     double mask_cx = 0.0, mask_cy = 0.0, mask_r = 0.0, mask_eps = 0.0;
@@ -669,9 +679,15 @@ const char *cfdmx_last_error(const cfdmx_ctx *ctx)
     return ctx ? ctx->err.c_str() : g_create_error.c_str();
 }
 
+static int plot3d_flush_impl(cfdmx_ctx *ctx);
+
 int cfdmx_destroy(cfdmx_ctx *ctx)
 {
     if (!ctx) return CFDMX_OK;
+    (void)plot3d_flush_impl(ctx);
+    if (ctx->p3d.side) (void)hipStreamDestroy(ctx->p3d.side);
+    if (ctx->p3d.ready) (void)hipEventDestroy(ctx->p3d.ready);
+    if (ctx->p3d.host) (void)hipHostFree(ctx->p3d.host);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (void *p : ctx->allocs) (void)hipFree(p);
     prof_harvest(ctx);
@@ -1365,6 +1381,84 @@ int cfdmx_profile_read(cfdmx_ctx *ctx, int kernel, double *total_ms, long long *
     prof_harvest(ctx);
     if (total_ms) *total_ms = ctx->prof_ms[kernel];
     if (launches) *launches = ctx->prof_n[kernel];
+    return CFDMX_OK;
+}
+
+// ---- Plot3D output (OutputPLOT3D_Qfile, qfile.hpp:6-108; OutputPlot3D_Xfile,
+// xfileWrite.hpp:6-64) ----
+static int plot3d_flush_impl(cfdmx_ctx *ctx)
+{
+    cfdmx_ctx::P3D &P = ctx->p3d;
+    if (!P.pending) return CFDMX_OK;
+    P.pending = false;
+    HIPCHK(hipStreamSynchronize(P.side));
+    FILE *fp = std::fopen(P.path.c_str(), "wb");
+    if (!fp) FAIL(CFDMX_ERR_ARG, "plot3d: cannot open %s for writing", P.path.c_str());
+    const int32_t hdr[4] = {1, ctx->g.nxc, ctx->g.nyc, ctx->g.nzc};
+    const float fh[4] = {0.0f, 0.0f, P.reyn, P.time}; // mach, alpha, Re, time
+    const size_t n = 5 * (size_t)ctx->g.N;
+    const bool ok = std::fwrite(hdr, sizeof hdr, 1, fp) == 1 &&
+                    std::fwrite(fh, sizeof fh, 1, fp) == 1 &&
+                    std::fwrite(P.host, sizeof(float), n, fp) == n;
+    if (std::fclose(fp) != 0 || !ok)
+        FAIL(CFDMX_ERR_ARG, "plot3d: writing %s failed", P.path.c_str());
+    return CFDMX_OK;
+}
+
+int cfdmx_plot3d_q(cfdmx_ctx *ctx, const char *path, double time)
+{
+    if (!ctx || !path) return CFDMX_ERR_ARG;
+    int rc = plot3d_flush_impl(ctx);
+    if (rc) return rc;
+    cfdmx_ctx::P3D &P = ctx->p3d;
+    const size_t n = 5 * (size_t)ctx->g.N;
+    if (!P.dev) {
+        ALLOC(P.dev, n);
+        HIPCHK(hipHostMalloc((void **)&P.host, n * sizeof(float), hipHostMallocDefault));
+        HIPCHK(hipStreamCreateWithFlags(&P.side, hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&P.ready, hipEventDisableTiming));
+    }
+    launch_plot3d_q(ctx->g, ctx->stream, ctx->t0[0], ctx->t0[1], ctx->t0[2], ctx->p, ctx->eta,
+                    P.dev);
+    LAUNCHCHK();
+    HIPCHK(hipEventRecord(P.ready, ctx->stream));
+    HIPCHK(hipStreamWaitEvent(P.side, P.ready, 0));
+    HIPCHK(hipMemcpyAsync(P.host, P.dev, n * sizeof(float), hipMemcpyDeviceToHost, P.side));
+    P.path = path;
+    P.reyn = (float)ctx->cfg.re;
+    P.time = (float)time;
+    P.pending = true;
+    return CFDMX_OK;
+}
+
+int cfdmx_plot3d_flush(cfdmx_ctx *ctx)
+{
+    if (!ctx) return CFDMX_ERR_ARG;
+    return plot3d_flush_impl(ctx);
+}
+
+int cfdmx_plot3d_x(cfdmx_ctx *ctx, const char *path)
+{
+    if (!ctx || !path) return CFDMX_ERR_ARG;
+    const int nxc = ctx->g.nxc, nyc = ctx->g.nyc, nzc = ctx->g.nzc;
+    // cell centres (structureGrid.hpp:216-229): (X[i+gC] + X[i+gC+1]) / 2
+    std::vector<float> xc(nxc), yc(nyc), zc(nzc);
+    for (int i = 0; i < nxc; ++i) xc[i] = (float)((ctx->X[i + kGC] + ctx->X[i + kGC + 1]) / 2.0);
+    for (int j = 0; j < nyc; ++j) yc[j] = (float)((ctx->Y[j + kGC] + ctx->Y[j + kGC + 1]) / 2.0);
+    for (int k = 0; k < nzc; ++k) zc[k] = (float)((ctx->Z[k + kGC] + ctx->Z[k + kGC + 1]) / 2.0);
+    const size_t n = (size_t)nxc * nyc * nzc;
+    std::vector<float> buf(3 * n);
+    size_t o = 0;
+    for (int a = 0; a < 3; ++a)
+        for (int k = 0; k < nzc; ++k)
+            for (int j = 0; j < nyc; ++j)
+                for (int i = 0; i < nxc; ++i) buf[o++] = a == 0 ? xc[i] : (a == 1 ? yc[j] : zc[k]);
+    FILE *fp = std::fopen(path, "wb");
+    if (!fp) FAIL(CFDMX_ERR_ARG, "plot3d: cannot open %s for writing", path);
+    const int32_t hdr[4] = {1, nxc, nyc, nzc};
+    const bool ok = std::fwrite(hdr, sizeof hdr, 1, fp) == 1 &&
+                    std::fwrite(buf.data(), sizeof(float), buf.size(), fp) == buf.size();
+    if (std::fclose(fp) != 0 || !ok) FAIL(CFDMX_ERR_ARG, "plot3d: writing %s failed", path);
     return CFDMX_OK;
 }
 

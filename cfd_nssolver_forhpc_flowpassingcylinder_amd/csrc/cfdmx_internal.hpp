@@ -143,6 +143,8 @@ void launch_init_flow(const Geo &g, hipStream_t s, double u0, double amp,
                       unsigned long long seed, double *u, double *v, double *w, double *p);
 
 void launch_transform(const Geo &g, hipStream_t s, const double *x, double *p);
+void launch_plot3d_q(const Geo &g, hipStream_t s, const double *u, const double *v,
+                     const double *w, const double *p, const double *eta, float *out);
 void launch_set_int(hipStream_t s, int *a, int val);
 
 // adjoint of the drag (cfdmx_adjoint.hip)

@@ -202,6 +202,51 @@ k_convdiff(Geo g, const double *__restrict__ u, const double *__restrict__ v,
 }
 
 // ---------------------------------------------------------------------------
+// Plot3D q arrays (OutputPLOT3D_Qfile, qfile.hpp:63-103): p, cell-centred u, v, w
+// (0.5 * (face below + face above) in FP64, rounded to float) and eta, each an interior
+// array with i fastest.  The fields have k fastest, so a 32 x 32 (i, k) tile of one j
+// row goes through LDS: reads coalesce along k, writes along i.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k_plot3d_q(Geo g, const double *__restrict__ u, const double *__restrict__ v,
+           const double *__restrict__ w, const double *__restrict__ p,
+           const double *__restrict__ eta, float *__restrict__ out)
+{
+    __shared__ float tile[32][33];
+    const int i0 = blockIdx.x * 32, k0 = blockIdx.y * 32, j = blockIdx.z;
+    const size_t N = (size_t)g.N;
+    for (int q = 0; q < 5; ++q) {
+        for (int ii = threadIdx.y; ii < 32; ii += 8) {
+            const int i = i0 + ii, k = k0 + threadIdx.x;
+            if (i < g.nxc && k < g.nzc) {
+                const int ic = gidx(g, i + kGC, j + kGC, k + kGC);
+                double val;
+                if (q == 0) val = p[ic];
+                else if (q == 1) val = 0.5 * (u[ic - g.sx] + u[ic]);
+                else if (q == 2) val = 0.5 * (v[ic - g.sy] + v[ic]);
+                else if (q == 3) val = 0.5 * (w[ic - 1] + w[ic]);
+                else val = eta[ic];
+                tile[ii][threadIdx.x] = (float)val;
+            }
+        }
+        __syncthreads();
+        for (int kk = threadIdx.y; kk < 32; kk += 8) {
+            const int i = i0 + threadIdx.x, k = k0 + kk;
+            if (i < g.nxc && k < g.nzc)
+                out[q * N + ((size_t)k * g.nyc + j) * g.nxc + i] = tile[threadIdx.x][kk];
+        }
+        __syncthreads();
+    }
+}
+
+void launch_plot3d_q(const Geo &g, hipStream_t s, const double *u, const double *v,
+                     const double *w, const double *p, const double *eta, float *out)
+{
+    hipLaunchKernelGGL(k_plot3d_q, dim3((g.nxc + 31) / 32, (g.nzc + 31) / 32, g.nyc),
+                       dim3(32, 8), 0, s, g, u, v, w, p, eta, out);
+}
+
+// ---------------------------------------------------------------------------
 // Boundary conditions, BC_staggered_main (Boundary_Condition.hpp:37-176) with the
 // Num/Dir table, applied per face group in the reference's order (x, y, z faces),
 // plus pressure ghosts.  With t1 != nullptr the same ghost values are written into

@@ -122,6 +122,24 @@ int cfdmx_download_force(cfdmx_ctx *ctx, double *f); /* DfibArray::f, 3*G */
 int cfdmx_upload_force(cfdmx_ctx *ctx, const double *f);
 int cfdmx_set_inflow(cfdmx_ctx *ctx, const double *profile); /* u inflow per ghosted j, ny */
 
+/* Plot3D output: OutputPLOT3D_Qfile (qfile.hpp:6-108) and OutputPlot3D_Xfile
+ * (xfileWrite.hpp:6-64), single-block files in the reference's layout.
+ * q file: int32 1, nx, ny, nz (interior); float32 0, 0, Re, time; then five float32
+ * interior arrays with i fastest: p; u, v, w averaged to cell centres from the faces
+ * on either side (the average formed in FP64, then rounded); eta.  The velocity is T0
+ * (u^n with its boundary faces set) where the reference reads T3; the two differ only
+ * on faces its T3 never refreshes (the outlet face and the ghost faces keep their
+ * initial values there).
+ * cfdmx_plot3d_q returns once the fields are staged on the device: the copy to host
+ * memory runs on a second stream, and the file is written by the next cfdmx_plot3d_q,
+ * by cfdmx_plot3d_flush or by cfdmx_destroy, so time steps proceed meanwhile.
+ * x file: int32 1, nx, ny, nz; the cell-centre coordinates Xc, Yc, Zc
+ * (structureGrid.hpp:216-229) as float32 arrays with i fastest.  A file that cannot be
+ * written returns CFDMX_ERR_ARG (for q, from the call that writes it). */
+int cfdmx_plot3d_q(cfdmx_ctx *ctx, const char *path, double time);
+int cfdmx_plot3d_flush(cfdmx_ctx *ctx);
+int cfdmx_plot3d_x(cfdmx_ctx *ctx, const char *path);
+
 /* Seeded initial field on the device: u = u0 + amp*(2U-1) on interior cells with U
  * from splitmix64(seed ^ interior index), v = w = 0, p = 0; then the initial
  * BC_staggered_main + BC_staggered_copy (RUN_CPU.cpp:66-109). */

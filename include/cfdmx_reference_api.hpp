@@ -19,6 +19,7 @@
 
 #include <cstdio>
 #include <fstream>
+#include <sys/stat.h>
 #include <iostream>
 #include <stdexcept>
 #include <string>
@@ -303,19 +304,67 @@ bool CD_CL(Simu &simu, Lo &, Dfib &dfib, Grid &gA)
     return true;
 }
 
+// Plot3D files in mx_out/ (created when missing): the reference writes them relative to
+// the working directory under these names (qfile.hpp:40-47, xfileWrite.hpp:37-39)
+inline std::string plot3d_q_name(size_t io)
+{
+    std::string name = "mx_out/P3D";
+    if (io < 10) name += "0";
+    return name + std::to_string(io) + ".q";
+}
+inline void plot3d_dir() { (void)::mkdir("mx_out", 0755); }
+
+// OutputPLOT3D_Qfile, qfile.hpp:6-12: p, cell-centred u, v, w and eta of the given fields
+// to mx_out/P3D<nn>.q (the copy to host memory and the file write are asynchronous; see
+// cfdmx_plot3d_q)
+template <class Dfib, class Simu, class Pre, class Vel, class Lo, class Grid>
+int OutputPLOT3D_Qfile(Dfib &dfib, Simu &simu, Pre &t1, Vel &vel, Lo &, Grid &gA)
+{
+    cfdmx_ctx *c = context(gA, simu);
+    const long long G = session().G;
+    std::vector<double> p(t1.p.begin(), t1.p.end());
+    fit(p, G);
+    check(cfdmx_upload_pressure(c, p.data()), "upload pressure");
+    fit(dfib.eta, G);
+    check(cfdmx_upload_eta(c, dfib.eta.data()), "upload eta");
+    Vel v = vel;
+    put_vel(0, v);
+    plot3d_dir();
+    std::cout << "(Output Plaot3D):" << simu.currentFile << std::endl;
+    check(cfdmx_plot3d_q(c, plot3d_q_name(simu.currentFile).c_str(), simu.get_time()),
+          "OutputPLOT3D_Qfile");
+    return 0;
+}
+
+// OutputPlot3D_Xfile, xfileWrite.hpp:6-9: the cell-centre coordinates to mx_out/P3D.x
+template <class Simu, class Grid> int OutputPlot3D_Xfile(Simu &simu, Grid &gA)
+{
+    cfdmx_ctx *c = context(gA, simu);
+    plot3d_dir();
+    check(cfdmx_plot3d_x(c, "mx_out/P3D.x"), "OutputPlot3D_Xfile");
+    return 0;
+}
+
 // runSeqOmp, RUN_CPU.cpp:9-440 / run/1_1_Sequential.hpp:5-10, with every field resident
 // on the device: the reference initial state (u = 1, v = w = p = 0), its boundary
 // conditions and mask, then the loop body per step; the per-step report goes to simu
-// (iters, error) and stdout as in the reference.
+// (iters, error) and stdout as in the reference, and the Plot3D files are written at its
+// points: the grid and the initial field before the loop (RUN_CPU.cpp:84,241), then a q
+// file whenever simu.get_writefile() says so (RUN_CPU.cpp:411-413), staged on the device
+// while the next steps run.
 template <class Grid, class Timer, class Simu>
 bool runSeqOmp(Grid &gA, Timer &, Simu &simu, int, char **)
 {
     cfdmx_ctx *c = context(gA, simu);
+    plot3d_dir();
+    check(cfdmx_plot3d_x(c, "mx_out/P3D.x"), "OutputPlot3D_Xfile");
     check(cfdmx_init_flow(c, 1.0, 0.0, 0ull), "initial field");
     if (simu.DfibMethod == "DFIB_Cylinder-Z")
         check(cfdmx_set_eta_cylinder_z(c, 6.5, gA.ly / 2.0, 0.5, 100), "DFIB_CylinderZ");
     else if (simu.DfibMethod != "OFF")
         throw std::invalid_argument("DFIB method??"); // RUN_CPU.cpp:236
+    check(cfdmx_plot3d_q(c, plot3d_q_name(simu.currentFile).c_str(), simu.get_time()),
+          "OutputPLOT3D_Qfile");
     for (simu.loop = 1; simu.get_finishloop(); simu.finishloop()) {
         cfdmx_step_info info;
         check(cfdmx_step(c, 1, &info), "time step");
@@ -323,7 +372,11 @@ bool runSeqOmp(Grid &gA, Timer &, Simu &simu, int, char **)
         simu.error = info.err;
         simu.printInfo();
         std::cout << "[cD, cL] : " << info.cd << ", " << info.cl << std::endl;
+        if (simu.get_writefile())
+            check(cfdmx_plot3d_q(c, plot3d_q_name(simu.currentFile).c_str(), simu.get_time()),
+                  "OutputPLOT3D_Qfile");
     }
+    check(cfdmx_plot3d_flush(c), "OutputPLOT3D_Qfile");
     return true;
 }
 

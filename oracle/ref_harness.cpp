@@ -280,6 +280,23 @@ void ref_step(void *h, int loop, double *out)
     c->simu.time += c->simu.dt;  // finishloop (0_1_SimulationVariable.hpp:139-156)
 }
 
+// OutputPLOT3D_Qfile (qfile.hpp:6-108) writes mx_out/P3D<io>.q and OutputPlot3D_Xfile
+// (xfileWrite.hpp:6-64) mx_out/P3D.x, relative to the working directory; both create
+// ../mx_out when it is missing, so the caller provides mx_out/ and ../mx_out first.
+int ref_plot3d_q(void *h, int io, double time)
+{
+    auto *c = static_cast<RefCtx *>(h);
+    c->simu.currentFile = (size_t)io;
+    c->simu.time = time;
+    return OutputPLOT3D_Qfile(c->Dfib, c->simu, c->t1, c->T3, c->Lo, *c->g);
+}
+
+int ref_plot3d_x(void *h)
+{
+    auto *c = static_cast<RefCtx *>(h);
+    return OutputPlot3D_Xfile(c->simu, *c->g);
+}
+
 int ref_threads(void) { return omp_get_max_threads(); }
 
 void ref_set_threads(int threads) { omp_set_num_threads(threads); }

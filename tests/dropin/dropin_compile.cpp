@@ -23,6 +23,8 @@ bool dropin_loop(grid &gA, clockstruct &timer, simpulationVariable &simu, int ar
     cfdmx_ref::bicgstabRe2<cfdmx_ref::SPE_matrix<double>> pSolver(A);
     pSolver.setTolerance(simu.p_criteria);
     cfdmx_ref::DFIB_CylinderZ(Dfib, Lo, gA);
+    cfdmx_ref::OutputPlot3D_Xfile(simu, gA);
+    cfdmx_ref::OutputPLOT3D_Qfile(Dfib, simu, t1, T0, Lo, gA);
     for (simu.loop = 1; simu.get_finishloop(); simu.finishloop()) {
         cfdmx_ref::ConvectionDifussion(simu, T0, T1, Lo, gA);
         if (simu.loop > 1) {
@@ -35,6 +37,7 @@ bool dropin_loop(grid &gA, clockstruct &timer, simpulationVariable &simu, int ar
         cfdmx_ref::BC_staggered_main(Lo, T0, t1, gA);
         cfdmx_ref::BC_staggered_copy(Lo, T0, T1, gA);
         cfdmx_ref::CD_CL(simu, Lo, Dfib, gA);
+        if (simu.get_writefile()) cfdmx_ref::OutputPLOT3D_Qfile(Dfib, simu, t1, T3, Lo, gA);
     }
     cfdmx_ref::SorPipeLine_omp(ShareM, Sor, simu, T1, t1, Lo, gA);
     std::vector<double> x(8), r(8);

@@ -341,6 +341,8 @@ class RefHarness:
         L.ref_set_maxit.argtypes = [C.c_void_p, C.c_int]
         L.ref_set_tol.argtypes = [C.c_void_p, C.c_double]
         L.ref_set_threads.argtypes = [C.c_int]
+        L.ref_plot3d_q.argtypes = [C.c_void_p, C.c_int, C.c_double]
+        L.ref_plot3d_x.argtypes = [C.c_void_p]
         self.nx, self.ny, self.nz = nxc + 4, nyc + 4, nzc + 4
         self.G = self.nx * self.ny * self.nz
         self.N = nxc * nyc * nzc
