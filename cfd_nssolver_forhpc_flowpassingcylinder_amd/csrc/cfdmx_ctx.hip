@@ -467,6 +467,12 @@ static int setup_device(cfdmx_ctx *ctx)
     g.ntile = (g.cyz + kTile - 1) / kTile;
     g.nchunk = (c.nx + kChunk - 1) / kChunk;
     g.nmarch = g.ntile * g.nchunk;
+    g.cd_tk = 4;
+    while (g.cd_tk < g.nzc && g.cd_tk < 32) g.cd_tk *= 2;
+    g.cd_tj = kThreads / g.cd_tk;
+    g.cd_nkt = (g.nzc + g.cd_tk - 1) / g.cd_tk;
+    g.cd_njt = (g.nyc + g.cd_tj - 1) / g.cd_tj;
+    g.cd_nb = g.cd_nkt * g.cd_njt * g.nchunk;
 
     // metric tables in one buffer
     std::vector<double> buf;
@@ -534,7 +540,7 @@ static int setup_device(cfdmx_ctx *ctx)
     ALLOC(ctx->B.ap[0], N);
     ALLOC(ctx->B.ap[1], N);
     ALLOC(ctx->part, 4 * (size_t)(g.nblocks > g.nmarch ? g.nblocks : g.nmarch) + 64);
-    ALLOC(ctx->cfl_part, (size_t)g.nblocks + 8);
+    ALLOC(ctx->cfl_part, (size_t)g.cd_nb + 8);
     for (int q = 0; q < 3; ++q) ALLOC(ctx->bpart[q], 2 * (size_t)g.nmarch + 64);
     ALLOC(ctx->flux_part, 3 * ((size_t)g.cyz / kPerBlock + 2));
     ALLOC(ctx->scal, 16);

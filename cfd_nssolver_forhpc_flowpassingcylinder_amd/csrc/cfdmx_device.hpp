@@ -51,17 +51,20 @@ __device__ __forceinline__ unsigned fdiv(unsigned n, const FastDiv &f)
     return (unsigned)(((unsigned long long)n * f.m) >> 40);
 }
 
-// Correctly rounded a / b from rb = RN(1/b): two residual corrections
+// Correctly rounded a / b for b > 0 from rb = RN(1/b): two residual corrections
 // (q += (a - b q) rb, with the residual exact under fma), which reproduce IEEE
-// division for finite operands; a zero numerator keeps its sign like a / b.
+// division for finite operands.  The residual is formed as -(b q - a): the same value
+// (round-to-nearest is odd-symmetric), and with a = -0 it is -0, so that the
+// corrections keep q = -0 and a zero numerator keeps its sign like a / b (every
+// divisor here is a grid spacing or the time step, so b > 0).
 __device__ __forceinline__ double divr(double a, double b, double rb)
 {
     double q = a * rb;
-    double r = __builtin_fma(-q, b, a);
-    q = __builtin_fma(r, rb, q);
-    r = __builtin_fma(-q, b, a);
-    q = __builtin_fma(r, rb, q);
-    return (a == 0.0) ? a * rb : q;
+    double t = __builtin_fma(q, b, -a);
+    q = __builtin_fma(-t, rb, q);
+    t = __builtin_fma(q, b, -a);
+    q = __builtin_fma(-t, rb, q);
+    return q;
 }
 
 // Per-axis metric tables (ghosted length), all computed on the host with the
@@ -95,6 +98,9 @@ struct Geo {
     int xcd_remap;
     FastDiv dz, dy;    // by nzc, by nyc
     int ntile, nchunk, nmarch; // plane tiles, plane chunks, march blocks
+    // conv-diff tiles: cd_tk cells of k by cd_tj rows of j (cd_tk * cd_tj = 256), tiles
+    // along k and j, blocks (tiles x plane chunks); the CFL partials are per block
+    int cd_tk, cd_tj, cd_nkt, cd_njt, cd_nb;
     Axis ax[3];
     PAxis pa[3];
 };

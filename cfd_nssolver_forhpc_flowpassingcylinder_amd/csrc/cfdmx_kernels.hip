@@ -28,17 +28,38 @@ __device__ __forceinline__ void quick_grads_A(const Axis &a, int I, const double
     g2 = divr(f[4] - f[3], a.D[I + 2], a.rD[I + 2]);
 }
 
+// face values from the four A quotients g(-1..2) = (f[q+1] - f[q]) / D[I+q-1]
+__device__ __forceinline__ void quickA_g(const Axis &a, int I, double Up, double Un,
+                                         const double f[5], double gm1, double g0, double g1,
+                                         double g2, double &P, double &N)
+{
+    const double cp = (Up > 0.0 ? a.qa[0] : a.qa[1])[I];
+    const double dp = Up > 0.0 ? (g1 - g0) : (g2 - g1);
+    P = 0.5 * (f[2] + f[3]) - cp * dp;
+    const double cn = (Un > 0.0 ? a.qa[2] : a.qa[3])[I];
+    const double dn = Un > 0.0 ? (g0 - gm1) : (g1 - g0);
+    N = 0.5 * (f[1] + f[2]) - cn * dn;
+}
+
 __device__ __forceinline__ void quickA(const Axis &a, int I, double Up, double Un,
                                        const double f[5], double &P, double &N, double &g0,
                                        double &g1)
 {
     double gm1, g2;
     quick_grads_A(a, I, f, gm1, g0, g1, g2);
-    const double cp = Up > 0.0 ? a.qa[0][I] : a.qa[1][I];
-    const double dp = Up > 0.0 ? (g1 - g0) : (g2 - g1);
+    quickA_g(a, I, Up, Un, f, gm1, g0, g1, g2, P, N);
+}
+
+// face values from the four B quotients h(-2..1) = (f[q+1] - f[q]) / Ds[I+q-2]
+__device__ __forceinline__ void quickB_h(const Axis &a, int I, double Up, double Un,
+                                         const double f[5], double hm2, double hm1, double h0,
+                                         double h1, double &P, double &N)
+{
+    const double cp = (Up > 0.0 ? a.qb[0] : a.qb[1])[I];
+    const double dp = Up > 0.0 ? (h0 - hm1) : (h1 - h0);
     P = 0.5 * (f[2] + f[3]) - cp * dp;
-    const double cn = Un > 0.0 ? a.qa[2][I] : a.qa[3][I];
-    const double dn = Un > 0.0 ? (g0 - gm1) : (g1 - g0);
+    const double cn = (Un > 0.0 ? a.qb[2] : a.qb[3])[I];
+    const double dn = Un > 0.0 ? (hm1 - hm2) : (h0 - hm1);
     N = 0.5 * (f[1] + f[2]) - cn * dn;
 }
 
@@ -50,12 +71,7 @@ __device__ __forceinline__ void quickB(const Axis &a, int I, double Up, double U
     hm1 = divr(f[2] - f[1], a.Ds[I - 1], a.rDs[I - 1]);
     h0 = divr(f[3] - f[2], a.Ds[I], a.rDs[I]);
     const double h1 = divr(f[4] - f[3], a.Ds[I + 1], a.rDs[I + 1]);
-    const double cp = Up > 0.0 ? a.qb[0][I] : a.qb[1][I];
-    const double dp = Up > 0.0 ? (h0 - hm1) : (h1 - h0);
-    P = 0.5 * (f[2] + f[3]) - cp * dp;
-    const double cn = Un > 0.0 ? a.qb[2][I] : a.qb[3][I];
-    const double dn = Un > 0.0 ? (hm1 - hm2) : (h0 - hm1);
-    N = 0.5 * (f[1] + f[2]) - cn * dn;
+    quickB_h(a, I, Up, Un, f, hm2, hm1, h0, h1, P, N);
 }
 
 __device__ __forceinline__ void load5(const double *phi, int c, int s, double f[5])
@@ -72,132 +88,283 @@ __device__ __forceinline__ void load5(const double *phi, int c, int s, double f[
 // interior cell's u, v and w faces, skipping the last face in each component's own
 // direction (:198-202,278-284,361-367) unless z is periodic.  Also the per-cell
 // CFL sum of get_Cfl (cfl.hpp:24-63) as a block maximum.
+//
+// Plane-marching and LDS-tiled: a block owns a tile of TJ rows x TK cells of the (j, k)
+// plane (TK the power of two >= nzc, 4 to 32; TJ = 256 / TK; one cell per thread)
+// and marches over kChunk planes in x.  A 3-slot LDS ring holds u, v, w of the planes
+// behind, at and ahead of the current one over the tile plus two rows and two cells of
+// halo on each side (the y and z stencils and the cross terms); the thread's own column
+// at two planes behind and ahead (the rest of the x stencil) stays in registers.  The
+// next ring plane is loaded into registers a plane ahead, and the 1-D metric tables of
+// the chunk, the tile's rows and its k range sit in LDS.  Every value is formed with the
+// reference's expressions from the same operands, so the result is bit-identical to a
+// cell-by-cell evaluation.
 // ---------------------------------------------------------------------------
+constexpr int kCdTables = 13; // D, Ds, rD, rDs, IT, qa[4], qb[4]
+
+// copy of the 13 tables of one axis for ghosted indices [lo, lo + n) into LDS at dst
+// (table t at dst + t*n), and the Axis that reads them with the global indices
+__device__ __forceinline__ Axis cd_axis_lds(const Axis &a, int lo, int n, int lim, double *dst)
+{
+    const double *src[kCdTables] = {a.D,     a.Ds,    a.rD,    a.rDs,   a.IT,
+                                    a.qa[0], a.qa[1], a.qa[2], a.qa[3], a.qb[0],
+                                    a.qb[1], a.qb[2], a.qb[3]};
+    for (int e = threadIdx.x; e < kCdTables * n; e += blockDim.x) {
+        const int t = e / n, x = lo + e % n;
+        dst[e] = (x >= 0 && x < lim) ? src[t][x] : 0.0;
+    }
+    Axis r;
+    double *base[kCdTables];
+    for (int t = 0; t < kCdTables; ++t) base[t] = dst + t * n - lo;
+    r.D = base[0];
+    r.Ds = base[1];
+    r.rD = base[2];
+    r.rDs = base[3];
+    r.IT = base[4];
+    for (int q = 0; q < 4; ++q) {
+        r.qa[q] = base[5 + q];
+        r.qb[q] = base[9 + q];
+    }
+    return r;
+}
+
+__host__ __device__ inline size_t cd_lds_bytes(const Geo &g)
+{
+    const size_t P = (size_t)(g.cd_tj + 4) * (g.cd_tk + 4);
+    return (9 * P + kCdTables * ((size_t)(kChunk + 4) + (g.cd_tj + 4) + (g.cd_tk + 4))) *
+           sizeof(double);
+}
+
+template <int NE>
 __global__ void __launch_bounds__(kThreads)
 k_convdiff(Geo g, const double *__restrict__ u, const double *__restrict__ v,
            const double *__restrict__ w, double *__restrict__ us, double *__restrict__ vs,
            double *__restrict__ ws, double mdt, double nudt, double dt,
            double *__restrict__ cfl_part)
 {
-    const int L = logical_block(g);
-    if (L >= g.nblocks) return;
+    const int b = logical_block(g);
+    if (b >= g.cd_nb) return;
+    extern __shared__ double lds[];
+    const int TK = g.cd_tk, TJ = g.cd_tj, RW = TK + 4, P = (TJ + 4) * RW;
+    const int kt = b % g.cd_nkt, jt = (b / g.cd_nkt) % g.cd_njt, ch = b / (g.cd_nkt * g.cd_njt);
+    const int k0 = kt * TK, j0 = jt * TJ, i0 = ch * kChunk;
+    const int I0 = i0 + kGC, len = min(kChunk, g.nxc - i0);
     const int sx = g.sx, sy = g.sy;
-    const Axis &X = g.ax[0], &Y = g.ax[1], &Z = g.ax[2];
-    double cmax = -1.0e300;
-    for (int c = 0; c < kCells; ++c) {
-        const int n = L * kPerBlock + c * kThreads + threadIdx.x;
-        if (n >= g.N) break;
-        int i, j, k;
-        decode(g, n, i, j, k);
-        const int I = i + kGC, J = j + kGC, K = k + kGC;
-        const int ic = gidx(g, I, J, K);
-        double cfl = 0.0;
-        // ---------------- u (:205-273) ----------------
-        if (I < g.nx - 3) {
-            const double IT = X.IT[I];
-            const double Up = 0.5 * (u[ic + sx] + u[ic]);
-            const double Un = 0.5 * (u[ic - sx] + u[ic]);
-            const double Vp = v[ic] + (v[ic + sx] - v[ic]) * IT;
-            const double Vn = v[ic - sy] + (v[ic - sy + sx] - v[ic - sy]) * IT;
-            const double Wp = w[ic] + (w[ic + sx] - w[ic]) * IT;
-            const double Wn = w[ic - 1] + (w[ic - 1 + sx] - w[ic - 1]) * IT;
-            double f[5], uPx, uNx, uPy, uNy, uPz, uNz, ax0, ax1, ay0, ay1, az0, az1;
-            load5(u, ic, sx, f);
-            quickA(X, I, Up, Un, f, uPx, uNx, ax0, ax1);
-            load5(u, ic, sy, f);
-            quickB(Y, J, Vp, Vn, f, uPy, uNy, ay0, ay1);
-            load5(u, ic, 1, f);
-            quickB(Z, K, Wp, Wn, f, uPz, uNz, az0, az1);
-            const double convection =
-                mdt * (divr(uPx * uPx - uNx * uNx, X.Ds[I], X.rDs[I]) +
-                       divr(uPy * Vp - uNy * Vn, Y.D[J], Y.rD[J]) +
-                       divr(uPz * Wp - uNz * Wn, Z.D[K], Z.rD[K]));
-            // ((u[xp]-u)/Dx[i+1] - (u-u[xm])/Dx[i]) = ax1 - ax0, etc.
-            const double difussion = nudt * (divr(ax1 - ax0, X.Ds[I], X.rDs[I]) +
-                                             divr(ay1 - ay0, Y.D[J], Y.rD[J]) +
-                                             divr(az1 - az0, Z.D[K], Z.rD[K]));
-            const double r = u[ic] + convection + difussion;
-            us[ic] = r;
-            cfl += divr(r, X.Ds[I], X.rDs[I]);
+    double *ring = lds;                 // [slot][field][P]
+    double *mt = lds + 9 * P;           // metric tables
+    const Axis X = cd_axis_lds(g.ax[0], I0 - 2, kChunk + 4, g.nx, mt);
+    const Axis Y = cd_axis_lds(g.ax[1], j0, TJ + 4, g.ny, mt + kCdTables * (kChunk + 4));
+    const Axis Z = cd_axis_lds(g.ax[2], k0, TK + 4,
+                               g.nz, mt + kCdTables * (kChunk + 4 + TJ + 4));
+    // own cell
+    const int tj = threadIdx.x / TK, tk = threadIdx.x - tj * TK;
+    const int J = j0 + tj + kGC, K = k0 + tk + kGC;
+    const bool own = (j0 + tj < g.nyc) && (k0 + tk < g.nzc);
+    const int c = (tj + 2) * RW + tk + 2;
+    // ring tiles: element e of a plane is (row e / RW, column e % RW) at ghosted
+    // (J, K) = (j0 + row, k0 + column); NE = ceil(P / 256) elements per thread
+    auto tile_index = [&](int I, int e) -> int {
+        const int row = e / RW, col = e - row * RW;
+        const int Jg = j0 + row, Kg = k0 + col;
+        const int ok = (int)(e < P) & (int)(Jg < g.ny) & (int)(Kg < g.nz) & (int)(I < g.nx);
+        return ((I * g.ny + Jg) * g.nz + Kg) & -ok;
+    };
+    double pu[NE], pv[NE], pw[NE];
+    auto load_tile = [&](int I) {
+#pragma unroll
+        for (int q = 0; q < NE; ++q) {
+            const int n = tile_index(I, threadIdx.x + q * kThreads);
+            pu[q] = u[n];
+            pv[q] = v[n];
+            pw[q] = w[n];
         }
-        // ---------------- v (:286-354) ----------------
-        if (J < g.ny - 3) {
-            const double IT = Y.IT[J];
-            const double Vp = 0.5 * (v[ic + sy] + v[ic]);
-            const double Vn = 0.5 * (v[ic - sy] + v[ic]);
-            const double Up = u[ic] + (u[ic + sy] - u[ic]) * IT;
-            const double Un = u[ic - sx] + (u[ic - sx + sy] - u[ic - sx]) * IT;
-            const double Wp = w[ic] + (w[ic + sy] - w[ic]) * IT;
-            const double Wn = w[ic - 1] + (w[ic - 1 + sy] - w[ic - 1]) * IT;
-            double f[5], vPx, vNx, vPy, vNy, vPz, vNz, ax0, ax1, ay0, ay1, az0, az1;
-            load5(v, ic, sx, f);
-            quickB(X, I, Up, Un, f, vPx, vNx, ax0, ax1);
-            load5(v, ic, sy, f);
-            quickA(Y, J, Vp, Vn, f, vPy, vNy, ay0, ay1);
-            load5(v, ic, 1, f);
-            quickB(Z, K, Wp, Wn, f, vPz, vNz, az0, az1);
-            const double convection =
-                mdt * (divr(Up * vPx - Un * vNx, X.D[I], X.rD[I]) +
-                       divr(vPy * vPy - vNy * vNy, Y.Ds[J], Y.rDs[J]) +
-                       divr(Wp * vPz - Wn * vNz, Z.D[K], Z.rD[K]));
-            const double difussion = nudt * (divr(ax1 - ax0, X.D[I], X.rD[I]) +
-                                             divr(ay1 - ay0, Y.Ds[J], Y.rDs[J]) +
-                                             divr(az1 - az0, Z.D[K], Z.rD[K]));
-            const double r = v[ic] + convection + difussion;
-            vs[ic] = r;
-            cfl += divr(r, Y.Ds[J], Y.rDs[J]);
-        }
-        // ---------------- w (:368-437) ----------------
-        if (K < g.nz - 3 || g.zper) {
-            const double IT = Z.IT[K];
-            const double Wp = 0.5 * (w[ic + 1] + w[ic]);
-            const double Wn = 0.5 * (w[ic - 1] + w[ic]);
-            const double Up = u[ic] + (u[ic + 1] - u[ic]) * IT;
-            const double Un = u[ic - sx] + (u[ic - sx + 1] - u[ic - sx]) * IT;
-            const double Vp = v[ic] + (v[ic + 1] - v[ic]) * IT;
-            const double Vn = v[ic - sy] + (v[ic - sy + 1] - v[ic - sy]) * IT;
-            double f[5], wPx, wNx, wPy, wNy, wPz, wNz, ax0, ax1, ay0, ay1, az0, az1;
-            load5(w, ic, sx, f);
-            quickB(X, I, Up, Un, f, wPx, wNx, ax0, ax1);
-            load5(w, ic, sy, f);
-            quickB(Y, J, Vp, Vn, f, wPy, wNy, ay0, ay1);
-            load5(w, ic, 1, f);
-            quickA(Z, K, Wp, Wn, f, wPz, wNz, az0, az1);
-            const double convection =
-                mdt * (divr(Up * wPx - Un * wNx, X.D[I], X.rD[I]) +
-                       divr(Vp * wPy - Vn * wNy, Y.D[J], Y.rD[J]) +
-                       divr(wPz * wPz - wNz * wNz, Z.Ds[K], Z.rDs[K]));
-            const double difussion = nudt * (divr(ax1 - ax0, X.D[I], X.rD[I]) +
-                                             divr(ay1 - ay0, Y.D[J], Y.rD[J]) +
-                                             divr(az1 - az0, Z.Ds[K], Z.rDs[K]));
-            const double r = w[ic] + convection + difussion;
-            ws[ic] = r;
-            cfl += divr(r, Z.Ds[K], Z.rDs[K]);
-        }
-        cmax = cfl > cmax ? cfl : cmax;
-        if (g.zper) {
-            // u* z-ghosts are images of the interior (face k = 1 == nz-3)
-            const int nzc = g.nzc;
-            int img = -1;
-            if (K == 2) img = ic + nzc;
-            else if (K == 3) img = ic + nzc;
-            else if (K == g.nz - 3) img = ic - nzc;
-            else if (K == g.nz - 4) img = ic - nzc;
-            if (img >= 0) {
-                us[img] = us[ic];
-                vs[img] = vs[ic];
-                ws[img] = ws[ic];
+    };
+    auto put_tile = [&](int slot) {
+        double *S = ring + slot * 3 * P;
+#pragma unroll
+        for (int q = 0; q < NE; ++q) {
+            const int e = threadIdx.x + q * kThreads;
+            if (e < P) {
+                S[e] = pu[q];
+                S[P + e] = pv[q];
+                S[2 * P + e] = pw[q];
             }
         }
+    };
+    // own column at ghosted plane I (zero-masked index when out of the grid)
+    auto own_index = [&](int I) -> int {
+        const int ok = (int)own & (int)(I < g.nx);
+        return ((I * g.ny + J) * g.nz + K) & -ok;
+    };
+    // prologue: ring planes I0-1, I0, I0+1 (slots 0, 1, 2), own column at I0-2 and I0+2
+    for (int p = 0; p < 3; ++p) {
+        load_tile(I0 - 1 + p);
+        put_tile(p);
+    }
+    int nm2 = own_index(I0 - 2), np2 = own_index(I0 + 2);
+    double um2 = u[nm2], vm2 = v[nm2], wm2 = w[nm2];
+    double up2 = u[np2], vp2 = v[np2], wp2 = w[np2];
+    load_tile(I0 + 2);
+    __syncthreads();
+    // x quotients of the QUICK stencils carried from plane to plane: A for u,
+    // qa(I') = (u[I'] - u[I'-1]) / D[I'] at I' = I-1, I, I+1; B for v and w,
+    // hb(I') = (f[I'+1] - f[I']) / Ds[I'] at I' = I-2, I-1, I
+    double gu[3], gv[3], gw[3];
+    {
+        const double *S0 = ring, *S1 = ring + 3 * P, *S2 = ring + 6 * P;
+        gu[0] = divr(S0[c] - um2, X.D[I0 - 1], X.rD[I0 - 1]);
+        gu[1] = divr(S1[c] - S0[c], X.D[I0], X.rD[I0]);
+        gu[2] = divr(S2[c] - S1[c], X.D[I0 + 1], X.rD[I0 + 1]);
+        gv[0] = divr(S0[P + c] - vm2, X.Ds[I0 - 2], X.rDs[I0 - 2]);
+        gv[1] = divr(S1[P + c] - S0[P + c], X.Ds[I0 - 1], X.rDs[I0 - 1]);
+        gv[2] = divr(S2[P + c] - S1[P + c], X.Ds[I0], X.rDs[I0]);
+        gw[0] = divr(S0[2 * P + c] - wm2, X.Ds[I0 - 2], X.rDs[I0 - 2]);
+        gw[1] = divr(S1[2 * P + c] - S0[2 * P + c], X.Ds[I0 - 1], X.rDs[I0 - 1]);
+        gw[2] = divr(S2[2 * P + c] - S1[2 * P + c], X.Ds[I0], X.rDs[I0]);
+    }
+    double cmax = -1.0e300;
+    for (int s = 0; s < len; ++s) {
+        const int I = I0 + s;
+        const double *Sm = ring + (s % 3) * 3 * P, *Sc = ring + ((s + 1) % 3) * 3 * P,
+                     *Sp = ring + ((s + 2) % 3) * 3 * P;
+        const double *um = Sm, *vm = Sm + P, *wm = Sm + 2 * P;
+        const double *uc = Sc, *vc = Sc + P, *wc = Sc + 2 * P;
+        const double *upl = Sp, *vpl = Sp + P, *wpl = Sp + 2 * P;
+        const double gu3 = divr(up2 - upl[c], X.D[I + 2], X.rD[I + 2]);
+        const double gv3 = divr(vp2 - vpl[c], X.Ds[I + 1], X.rDs[I + 1]);
+        const double gw3 = divr(wp2 - wpl[c], X.Ds[I + 1], X.rDs[I + 1]);
+        if (own) {
+            const int ic = gidx(g, I, J, K);
+            double cfl = 0.0;
+            // ---------------- u (:205-273) ----------------
+            if (I < g.nx - 3) {
+                const double IT = X.IT[I];
+                const double Up = 0.5 * (upl[c] + uc[c]);
+                const double Un = 0.5 * (um[c] + uc[c]);
+                const double Vp = vc[c] + (vpl[c] - vc[c]) * IT;
+                const double Vn = vc[c - RW] + (vpl[c - RW] - vc[c - RW]) * IT;
+                const double Wp = wc[c] + (wpl[c] - wc[c]) * IT;
+                const double Wn = wc[c - 1] + (wpl[c - 1] - wc[c - 1]) * IT;
+                double uPx, uNx, uPy, uNy, uPz, uNz, ax0, ax1, ay0, ay1, az0, az1;
+                const double fx[5] = {um2, um[c], uc[c], upl[c], up2};
+                quickA_g(X, I, Up, Un, fx, gu[0], gu[1], gu[2], gu3, uPx, uNx);
+                ax0 = gu[1];
+                ax1 = gu[2];
+                const double fy[5] = {uc[c - 2 * RW], uc[c - RW], uc[c], uc[c + RW], uc[c + 2 * RW]};
+                quickB(Y, J, Vp, Vn, fy, uPy, uNy, ay0, ay1);
+                const double fz[5] = {uc[c - 2], uc[c - 1], uc[c], uc[c + 1], uc[c + 2]};
+                quickB(Z, K, Wp, Wn, fz, uPz, uNz, az0, az1);
+                const double convection =
+                    mdt * (divr(uPx * uPx - uNx * uNx, X.Ds[I], X.rDs[I]) +
+                           divr(uPy * Vp - uNy * Vn, Y.D[J], Y.rD[J]) +
+                           divr(uPz * Wp - uNz * Wn, Z.D[K], Z.rD[K]));
+                // ((u[xp]-u)/Dx[i+1] - (u-u[xm])/Dx[i]) = ax1 - ax0, etc.
+                const double difussion = nudt * (divr(ax1 - ax0, X.Ds[I], X.rDs[I]) +
+                                                 divr(ay1 - ay0, Y.D[J], Y.rD[J]) +
+                                                 divr(az1 - az0, Z.D[K], Z.rD[K]));
+                const double r = uc[c] + convection + difussion;
+                us[ic] = r;
+                cfl += divr(r, X.Ds[I], X.rDs[I]);
+            }
+            // ---------------- v (:286-354) ----------------
+            if (J < g.ny - 3) {
+                const double IT = Y.IT[J];
+                const double Vp = 0.5 * (vc[c + RW] + vc[c]);
+                const double Vn = 0.5 * (vc[c - RW] + vc[c]);
+                const double Up = uc[c] + (uc[c + RW] - uc[c]) * IT;
+                const double Un = um[c] + (um[c + RW] - um[c]) * IT;
+                const double Wp = wc[c] + (wc[c + RW] - wc[c]) * IT;
+                const double Wn = wc[c - 1] + (wc[c - 1 + RW] - wc[c - 1]) * IT;
+                double vPx, vNx, vPy, vNy, vPz, vNz, ax0, ax1, ay0, ay1, az0, az1;
+                const double fx[5] = {vm2, vm[c], vc[c], vpl[c], vp2};
+                quickB_h(X, I, Up, Un, fx, gv[0], gv[1], gv[2], gv3, vPx, vNx);
+                ax0 = gv[1];
+                ax1 = gv[2];
+                const double fy[5] = {vc[c - 2 * RW], vc[c - RW], vc[c], vc[c + RW], vc[c + 2 * RW]};
+                quickA(Y, J, Vp, Vn, fy, vPy, vNy, ay0, ay1);
+                const double fz[5] = {vc[c - 2], vc[c - 1], vc[c], vc[c + 1], vc[c + 2]};
+                quickB(Z, K, Wp, Wn, fz, vPz, vNz, az0, az1);
+                const double convection =
+                    mdt * (divr(Up * vPx - Un * vNx, X.D[I], X.rD[I]) +
+                           divr(vPy * vPy - vNy * vNy, Y.Ds[J], Y.rDs[J]) +
+                           divr(Wp * vPz - Wn * vNz, Z.D[K], Z.rD[K]));
+                const double difussion = nudt * (divr(ax1 - ax0, X.D[I], X.rD[I]) +
+                                                 divr(ay1 - ay0, Y.Ds[J], Y.rDs[J]) +
+                                                 divr(az1 - az0, Z.D[K], Z.rD[K]));
+                const double r = vc[c] + convection + difussion;
+                vs[ic] = r;
+                cfl += divr(r, Y.Ds[J], Y.rDs[J]);
+            }
+            // ---------------- w (:368-437) ----------------
+            if (K < g.nz - 3 || g.zper) {
+                const double IT = Z.IT[K];
+                const double Wp = 0.5 * (wc[c + 1] + wc[c]);
+                const double Wn = 0.5 * (wc[c - 1] + wc[c]);
+                const double Up = uc[c] + (uc[c + 1] - uc[c]) * IT;
+                const double Un = um[c] + (um[c + 1] - um[c]) * IT;
+                const double Vp = vc[c] + (vc[c + 1] - vc[c]) * IT;
+                const double Vn = vc[c - RW] + (vc[c - RW + 1] - vc[c - RW]) * IT;
+                double wPx, wNx, wPy, wNy, wPz, wNz, ax0, ax1, ay0, ay1, az0, az1;
+                const double fx[5] = {wm2, wm[c], wc[c], wpl[c], wp2};
+                quickB_h(X, I, Up, Un, fx, gw[0], gw[1], gw[2], gw3, wPx, wNx);
+                ax0 = gw[1];
+                ax1 = gw[2];
+                const double fy[5] = {wc[c - 2 * RW], wc[c - RW], wc[c], wc[c + RW], wc[c + 2 * RW]};
+                quickB(Y, J, Vp, Vn, fy, wPy, wNy, ay0, ay1);
+                const double fz[5] = {wc[c - 2], wc[c - 1], wc[c], wc[c + 1], wc[c + 2]};
+                quickA(Z, K, Wp, Wn, fz, wPz, wNz, az0, az1);
+                const double convection =
+                    mdt * (divr(Up * wPx - Un * wNx, X.D[I], X.rD[I]) +
+                           divr(Vp * wPy - Vn * wNy, Y.D[J], Y.rD[J]) +
+                           divr(wPz * wPz - wNz * wNz, Z.Ds[K], Z.rDs[K]));
+                const double difussion = nudt * (divr(ax1 - ax0, X.D[I], X.rD[I]) +
+                                                 divr(ay1 - ay0, Y.D[J], Y.rD[J]) +
+                                                 divr(az1 - az0, Z.Ds[K], Z.rDs[K]));
+                const double r = wc[c] + convection + difussion;
+                ws[ic] = r;
+                cfl += divr(r, Z.Ds[K], Z.rDs[K]);
+            }
+            cmax = cfl > cmax ? cfl : cmax;
+            if (g.zper) {
+                // u* z-ghosts are images of the interior (face k = 1 == nz-3)
+                const int nzc = g.nzc;
+                int img = -1;
+                if (K == 2) img = ic + nzc;
+                else if (K == 3) img = ic + nzc;
+                else if (K == g.nz - 3) img = ic - nzc;
+                else if (K == g.nz - 4) img = ic - nzc;
+                if (img >= 0) {
+                    us[img] = us[ic];
+                    vs[img] = vs[ic];
+                    ws[img] = ws[ic];
+                }
+            }
+        }
+        // shift the own column (plane I-1 becomes the one two behind) and the quotients
+        gu[0] = gu[1], gu[1] = gu[2], gu[2] = gu3;
+        gv[0] = gv[1], gv[1] = gv[2], gv[2] = gv3;
+        gw[0] = gw[1], gw[1] = gw[2], gw[2] = gw3;
+        um2 = um[c];
+        vm2 = vm[c];
+        wm2 = wm[c];
+        __syncthreads();
+        // plane I+2 into the slot of plane I-1; loads for the next step
+        put_tile(s % 3);
+        np2 = own_index(I + 3);
+        up2 = u[np2];
+        vp2 = v[np2];
+        wp2 = w[np2];
+        load_tile(I + 3);
+        __syncthreads();
     }
     __shared__ double red[4];
-    const double wm = wave_max(cmax);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = wm;
+    const double wmx = wave_max(cmax);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = wmx;
     __syncthreads();
     if (threadIdx.x == 0) {
         double m = red[0];
         for (int q = 1; q < 4; ++q) m = red[q] > m ? red[q] : m;
-        cfl_part[L] = m * dt;
+        cfl_part[b] = m * dt;
     }
 }
 
@@ -1514,8 +1681,14 @@ void launch_convdiff(const Geo &g, hipStream_t s, const double *u, const double 
                      const double *w, double *us, double *vs, double *ws, double dt,
                      double nu, double *cfl_part)
 {
-    hipLaunchKernelGGL(k_convdiff, dim3(grid_blocks(g)), dim3(kThreads), 0, s, g, u, v, w, us,
-                       vs, ws, -dt, nu * dt, dt, cfl_part);
+    const int nb = g.xcd_remap ? ((g.cd_nb + 7) / 8) * 8 : g.cd_nb;
+    const int P = (g.cd_tj + 4) * (g.cd_tk + 4);
+    if (P <= 2 * kThreads)
+        hipLaunchKernelGGL(k_convdiff<2>, dim3(nb), dim3(kThreads), cd_lds_bytes(g), s, g, u, v,
+                           w, us, vs, ws, -dt, nu * dt, dt, cfl_part);
+    else
+        hipLaunchKernelGGL(k_convdiff<3>, dim3(nb), dim3(kThreads), cd_lds_bytes(g), s, g, u, v,
+                           w, us, vs, ws, -dt, nu * dt, dt, cfl_part);
 }
 
 void launch_bc(const Geo &g, hipStream_t s, const BCParams &bc, int advance, double *u,
@@ -1754,7 +1927,7 @@ void launch_force_partial(const Geo &g, hipStream_t s, const double *f, double *
 
 void launch_max_fin(const Geo &g, hipStream_t s, const double *part, double *out)
 {
-    hipLaunchKernelGGL(k_max_fin, dim3(1), dim3(kThreads), 0, s, part, g.nblocks, out);
+    hipLaunchKernelGGL(k_max_fin, dim3(1), dim3(kThreads), 0, s, part, g.cd_nb, out);
 }
 
 void launch_eta_cylz(const Geo &g, hipStream_t s, const double *Xn, const double *Yn, double cx,

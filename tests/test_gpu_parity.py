@@ -38,8 +38,14 @@ def _random_velocity(g, seed):
     return (np.ascontiguousarray(a.ravel()) for a in (u, v, w))
 
 
+# the conv-diff tiles are 4..32 cells of k by 256/that rows of j, marched over 8-plane
+# chunks of x: the cases cover each tile shape, partial tiles in j and k and a partial
+# last chunk
 @pytest.mark.parametrize("kind,n,L", [(0, (40, 24, 8), (20.0, 10.0, 0.5)),
-                                      (1, (112, 104, 6), (17.0, 12.0, 3.0))])
+                                      (1, (112, 104, 6), (17.0, 12.0, 3.0)),
+                                      (0, (20, 9, 40), (10.0, 5.0, 4.0)),
+                                      (0, (13, 70, 4), (6.5, 20.0, 0.4)),
+                                      (0, (27, 21, 17), (9.0, 7.0, 3.0))])
 @pytest.mark.parametrize("zper", [0, 1])
 def test_convdiff_bitwise(kind, n, L, zper):
     cfg = pkg.default_config(nx=n[0], ny=n[1], nz=n[2], lx=L[0], ly=L[1], lz=L[2],
