@@ -34,7 +34,7 @@ EXPORTS = [
     "cfdmx_apply_operator", "cfdmx_step", "cfdmx_synchronize", "cfdmx_profile",
     "cfdmx_profile_read", "cfdmx_set_tolerance", "cfdmx_set_physics", "cfdmx_upload_force",
     "cfdmx_drag_objective", "cfdmx_adjoint_drag", "cfdmx_plot3d_q", "cfdmx_plot3d_flush",
-    "cfdmx_plot3d_x",
+    "cfdmx_plot3d_x", "cfdmx_checkpoint_save", "cfdmx_checkpoint_load",
 ]
 
 # kernel classes of cfdmx_profile_read
@@ -107,6 +107,8 @@ def load_library(path=LIB_PATH):
     L.cfdmx_plot3d_q.argtypes = [vp, C.c_char_p, C.c_double]
     L.cfdmx_plot3d_flush.argtypes = [vp]
     L.cfdmx_plot3d_x.argtypes = [vp, C.c_char_p]
+    L.cfdmx_checkpoint_save.argtypes = [vp, C.c_char_p]
+    L.cfdmx_checkpoint_load.argtypes = [vp, C.c_char_p]
     L.cfdmx_set_eta_cylinder_z.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_int]
     L.cfdmx_set_eta_smoothed.argtypes = [vp] + [C.c_double] * 4
     for n in ("cfdmx_start", "cfdmx_convdiff", "cfdmx_bc_main", "cfdmx_bc_copy",
@@ -255,6 +257,15 @@ class FlowSolver:
 
     def plot3d_x(self, path):
         self._chk(self.L.cfdmx_plot3d_x(self.h, os.fsencode(path)), "plot3d_x")
+
+    def checkpoint_save(self, path):
+        """FP64 state of the run (fields, X_result history, step, time, solver state)."""
+        self._chk(self.L.cfdmx_checkpoint_save(self.h, os.fsencode(path)), "checkpoint_save")
+
+    def checkpoint_load(self, path):
+        """Restore a checkpoint of a run with this configuration; later steps continue it
+        bit for bit."""
+        self._chk(self.L.cfdmx_checkpoint_load(self.h, os.fsencode(path)), "checkpoint_load")
 
     def set_inflow(self, profile):
         if profile is None:

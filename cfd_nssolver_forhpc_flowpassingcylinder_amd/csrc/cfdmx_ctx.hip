@@ -1443,6 +1443,156 @@ int cfdmx_plot3d_flush(cfdmx_ctx *ctx)
     return plot3d_flush_impl(ctx);
 }
 
+// ---- FP64 checkpoint / resume (no reference counterpart: the reference's resume reads
+// a float32 q file back, QfileRead in qfileRead.hpp:6-100, which loses precision and the
+// solver state).  The file holds every array a later step reads, so that loading it into
+// a context of the same configuration continues the run bit for bit.
+namespace {
+constexpr char kCkMagic[8] = {'C', 'F', 'D', 'M', 'X', 'C', 'K', '1'};
+struct CkHeader {
+    char magic[8];
+    int32_t version, nx, ny, nz, grid_kind, precision, solver, z_periodic;
+    int64_t G, N;
+    double lx, ly, lz;
+    int32_t loop, resets, started, mask_smoothed, has_inflow, pad;
+    double time, dt, re, mask_cx, mask_cy, mask_r, mask_eps;
+    KState kst;
+    SorState sst;
+};
+
+// the checkpointed arrays and their lengths, in file order
+static std::vector<std::pair<double *, size_t>> ck_arrays(cfdmx_ctx *ctx)
+{
+    const size_t G = (size_t)ctx->g.G, N = (size_t)ctx->g.N;
+    std::vector<std::pair<double *, size_t>> a;
+    for (int q = 0; q < 3; ++q) a.push_back({ctx->t0[q], G});
+    for (int q = 0; q < 3; ++q) a.push_back({ctx->t1[q], G});
+    for (int q = 0; q < 3; ++q) a.push_back({ctx->t3[q], G});
+    a.push_back({ctx->p, G});
+    a.push_back({ctx->eta, G});
+    a.push_back({ctx->f, 3 * G});
+    a.push_back({ctx->x, N});
+    a.push_back({ctx->px1, N});
+    a.push_back({ctx->px2, N});
+    a.push_back({ctx->inflow, (size_t)ctx->g.ny});
+    return a;
+}
+} // namespace
+
+int cfdmx_checkpoint_save(cfdmx_ctx *ctx, const char *path)
+{
+    if (!ctx || !path) return CFDMX_ERR_ARG;
+    if (!ctx->started) FAIL(CFDMX_ERR_STATE, "checkpoint: nothing to save before cfdmx_start");
+    CkHeader h;
+    std::memset(&h, 0, sizeof h);
+    std::memcpy(h.magic, kCkMagic, sizeof h.magic);
+    const cfdmx_config &c = ctx->cfg;
+    h.version = 1;
+    h.nx = c.nx, h.ny = c.ny, h.nz = c.nz;
+    h.grid_kind = c.grid_kind, h.precision = c.precision, h.solver = c.solver;
+    h.z_periodic = c.z_periodic;
+    h.G = ctx->g.G, h.N = ctx->g.N;
+    h.lx = c.lx, h.ly = c.ly, h.lz = c.lz;
+    h.loop = ctx->loop, h.resets = ctx->resets, h.started = ctx->started;
+    h.mask_smoothed = ctx->mask_smoothed;
+    h.has_inflow = ctx->bc.inflow != nullptr;
+    h.time = ctx->time, h.dt = c.dt, h.re = c.re;
+    h.mask_cx = ctx->mask_cx, h.mask_cy = ctx->mask_cy, h.mask_r = ctx->mask_r;
+    h.mask_eps = ctx->mask_eps;
+    HIPCHK(hipMemcpyAsync(ctx->kst_h, ctx->kst, sizeof(KState), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->sst_h, ctx->sst, sizeof(SorState), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    h.kst = *ctx->kst_h;
+    h.sst = *ctx->sst_h;
+    const auto arrays = ck_arrays(ctx);
+    size_t most = 0;
+    for (const auto &a : arrays) most = std::max(most, a.second);
+    std::vector<double> buf(most);
+    FILE *fp = std::fopen(path, "wb");
+    if (!fp) FAIL(CFDMX_ERR_ARG, "checkpoint: cannot open %s for writing", path);
+    bool ok = std::fwrite(&h, sizeof h, 1, fp) == 1;
+    for (const auto &a : arrays) {
+        if (!ok) break;
+        const int rc = copy_d2h(ctx, buf.data(), a.first, a.second);
+        if (rc) {
+            std::fclose(fp);
+            return rc;
+        }
+        ok = std::fwrite(buf.data(), sizeof(double), a.second, fp) == a.second;
+    }
+    if (std::fclose(fp) != 0 || !ok) FAIL(CFDMX_ERR_ARG, "checkpoint: writing %s failed", path);
+    return CFDMX_OK;
+}
+
+int cfdmx_checkpoint_load(cfdmx_ctx *ctx, const char *path)
+{
+    if (!ctx || !path) return CFDMX_ERR_ARG;
+    FILE *fp = std::fopen(path, "rb");
+    if (!fp) FAIL(CFDMX_ERR_ARG, "checkpoint: cannot open %s", path);
+    CkHeader h;
+    if (std::fread(&h, sizeof h, 1, fp) != 1 ||
+        std::memcmp(h.magic, kCkMagic, sizeof h.magic) != 0 || h.version != 1) {
+        std::fclose(fp);
+        FAIL(CFDMX_ERR_ARG, "checkpoint: %s is not a cfdmx checkpoint", path);
+    }
+    const cfdmx_config &c = ctx->cfg;
+    if (h.nx != c.nx || h.ny != c.ny || h.nz != c.nz || h.grid_kind != c.grid_kind ||
+        h.lx != c.lx || h.ly != c.ly || h.lz != c.lz || h.z_periodic != c.z_periodic ||
+        h.G != ctx->g.G || h.N != ctx->g.N) {
+        std::fclose(fp);
+        FAIL(CFDMX_ERR_ARG,
+             "checkpoint: %s holds a %dx%dx%d grid (kind %d) of another size or domain than "
+             "this context's %dx%dx%d (kind %d)",
+             path, h.nx, h.ny, h.nz, h.grid_kind, c.nx, c.ny, c.nz, c.grid_kind);
+    }
+    const auto arrays = ck_arrays(ctx);
+    size_t most = 0, total = sizeof h;
+    for (const auto &a : arrays) {
+        most = std::max(most, a.second);
+        total += a.second * sizeof(double);
+    }
+    // a short file is refused before anything is overwritten
+    if (std::fseek(fp, 0, SEEK_END) != 0 || std::ftell(fp) < (long)total ||
+        std::fseek(fp, (long)sizeof h, SEEK_SET) != 0) {
+        std::fclose(fp);
+        FAIL(CFDMX_ERR_ARG, "checkpoint: %s is truncated", path);
+    }
+    std::vector<double> buf(most);
+    for (const auto &a : arrays) {
+        if (std::fread(buf.data(), sizeof(double), a.second, fp) != a.second) {
+            std::fclose(fp);
+            FAIL(CFDMX_ERR_ARG, "checkpoint: reading %s failed", path);
+        }
+        const int rc = copy_h2d(ctx, a.first, buf.data(), a.second);
+        if (rc) {
+            std::fclose(fp);
+            return rc;
+        }
+    }
+    std::fclose(fp);
+    *ctx->kst_h = h.kst;
+    *ctx->sst_h = h.sst;
+    HIPCHK(hipMemcpyAsync(ctx->kst, ctx->kst_h, sizeof(KState), hipMemcpyHostToDevice,
+                          ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->sst, ctx->sst_h, sizeof(SorState), hipMemcpyHostToDevice,
+                          ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->loop = h.loop;
+    ctx->resets = h.resets;
+    ctx->started = h.started;
+    ctx->time = h.time;
+    ctx->cfg.dt = h.dt;
+    ctx->cfg.re = h.re;
+    ctx->bc.conv_coef = ctx->cfg.convective_speed * h.dt / ctx->Dx[ctx->g.nx - 3];
+    ctx->bc.inflow = h.has_inflow ? ctx->inflow : nullptr;
+    ctx->mask_smoothed = h.mask_smoothed;
+    ctx->mask_cx = h.mask_cx, ctx->mask_cy = h.mask_cy, ctx->mask_r = h.mask_r;
+    ctx->mask_eps = h.mask_eps;
+    return CFDMX_OK;
+}
+
 int cfdmx_plot3d_x(cfdmx_ctx *ctx, const char *path)
 {
     if (!ctx || !path) return CFDMX_ERR_ARG;

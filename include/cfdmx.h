@@ -140,6 +140,18 @@ int cfdmx_plot3d_q(cfdmx_ctx *ctx, const char *path, double time);
 int cfdmx_plot3d_flush(cfdmx_ctx *ctx);
 int cfdmx_plot3d_x(cfdmx_ctx *ctx, const char *path);
 
+/* FP64 checkpoint / resume.  No reference counterpart: it stands where the reference
+ * would resume from a float32 q file (QfileRead, qfileRead.hpp:6-100, called only from
+ * commented-out code in RUN_CPU.cpp:89).  cfdmx_checkpoint_save writes every array a
+ * later step reads (T0, T1, T3, p, eta, the forcing, X_result and its two history
+ * copies, the inflow profile) with the step counter, time, dt, Re, the mask parameters
+ * and the solver states; cfdmx_checkpoint_load restores them into a context created
+ * with the same configuration (size, grid kind, domain, z periodicity: CFDMX_ERR_ARG
+ * otherwise), after which steps continue bit for bit as in the saved run.  Both
+ * synchronise the context's stream. */
+int cfdmx_checkpoint_save(cfdmx_ctx *ctx, const char *path);
+int cfdmx_checkpoint_load(cfdmx_ctx *ctx, const char *path);
+
 /* Seeded initial field on the device: u = u0 + amp*(2U-1) on interior cells with U
  * from splitmix64(seed ^ interior index), v = w = 0, p = 0; then the initial
  * BC_staggered_main + BC_staggered_copy (RUN_CPU.cpp:66-109). */
