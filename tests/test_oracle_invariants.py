@@ -241,6 +241,67 @@ def test_operator_eigenpairs(o, zper):
     assert np.allclose(Av, lam * v, rtol=0, atol=1e-9 * lam)
 
 
+def _axis_chains(o, g):
+    """The x and y parts of the SPE operator as dense 1-D chains, read from the rows at
+    an interior (i, k) / (j, k): the operator is a Kronecker sum of per-axis chains."""
+    V = o.spe_vals(g).reshape(g.nxc, g.nyc, g.nzc, 7)
+    i, j, k = g.nxc // 2, g.nyc // 2, 1
+    Ay = np.zeros((g.nyc, g.nyc))
+    for jj in range(g.nyc):
+        r = V[i, jj, k]
+        Ay[jj, jj] = r[3] + r[0] + r[6] + r[2] + r[4]
+        if jj > 0:
+            Ay[jj, jj - 1] = r[1]
+        if jj < g.nyc - 1:
+            Ay[jj, jj + 1] = r[5]
+    Ax = np.zeros((g.nxc, g.nxc))
+    for ii in range(g.nxc):
+        r = V[ii, j, k]
+        Ax[ii, ii] = r[3] + r[1] + r[5] + r[2] + r[4]
+        if ii > 0:
+            Ax[ii, ii - 1] = r[0]
+        if ii < g.nxc - 1:
+            Ax[ii, ii + 1] = r[6]
+    return Ax, Ay
+
+
+def _left_null(A):
+    w, vl = np.linalg.eig(A.T)
+    m = np.argmin(np.abs(w))
+    v = np.real(vl[:, m])
+    return v / v.sum()
+
+
+def test_sml_y_slot_swap_moves_the_left_null_vector(o):
+    """Why the SML grid has no adjoint or multigrid mode.  The SPE rows sum to zero on
+    either grid, so A 1 = 0.  The discrete divergence b of a flux-balanced field is
+    orthogonal to the cell volumes, so A p = b is consistent only if the volumes span
+    A's left null space.  That holds per axis where the chain is the flux form divided
+    by the widths (Dx on either grid, Dy on the uniform grid), but the y-slot swap of
+    SPE_sparseMatrix.hpp:79,83 breaks it on the stretched y of the SML grid: the left
+    null vector leaves the widths by O(1), and the chain gains a second near-null
+    direction.  Tight-tolerance pressure solves there stall (the reference's own
+    BiCGSTAB stops at its iteration cap near 1e-6 relative residual), so a time step is
+    not a smooth function of its inputs and an adjoint has no finite-difference pin."""
+    for kind, n, L in ((1, (112, 104, 4), (17.0, 12.0, 3.0)), (0, (40, 24, 4), (20.0, 10.0, 0.5))):
+        g = o.grid(*n, *L, kind=kind)
+        Ax, Ay = _axis_chains(o, g)
+        for A, D in ((Ax, g.Dx[2:2 + g.nxc]), (Ay, g.Dy[2:2 + g.nyc])):
+            scale = np.abs(np.diag(A)).max()
+            assert np.abs(A.sum(axis=1)).max() <= 1e-12 * scale
+        lx, ly = _left_null(Ax), _left_null(Ay)
+        wx = g.Dx[2:2 + g.nxc] / g.Dx[2:2 + g.nxc].sum()
+        wy = g.Dy[2:2 + g.nyc] / g.Dy[2:2 + g.nyc].sum()
+        assert np.abs(lx - wx).max() <= 1e-9 * wx.max()
+        sv = np.linalg.svd(Ay, compute_uv=False)
+        if kind == 0:
+            assert np.abs(ly - wy).max() <= 1e-9 * wy.max()
+            assert sv[-2] > 1e-2
+        else:
+            assert np.abs(ly - wy).max() >= 0.5 * wy.max()
+            assert sv[-2] < 1e-4 * sv[0]
+
+
 def test_jacobi_operator_is_row_scaled_spe(o):
     """CSR + Jacobi rows are the SPE rows (without the y-slot swap, equal on a uniform
     grid) divided by the diagonal."""
