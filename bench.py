@@ -15,7 +15,10 @@ The JSON line also carries:
                    launch duration, from HIP events on the library's stream during the
                    timed steps, against the 8 TB/s HBM peak;
   cpu_baseline  -- the reference's own CPU code (oracle/_ref, built from the reference
-                   headers) timed on this host on a bounded sample of the same grid.
+                   headers) timed on this host on a bounded sample of the same grid;
+  stream_ceiling -- what plain streaming kernels (copy, and a two-read-one-write triad
+                   of phase B's shape) reach at the same array size with the caches
+                   flushed: the attainable rate the phases are read against.
 """
 import argparse
 import json
@@ -87,6 +90,39 @@ def copy_probe(mib):
     torch.cuda.synchronize()
     print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8,
                       "launches": 3}), flush=True)
+
+
+def stream_ceiling(n_cells, launches=60):
+    """Streaming rate that HBM gives this array size when nothing is reused from the
+    caches: torch's elementwise copy (16 B/cell) and z = x - 0.5 y (24 B/cell, the
+    shape of BiCGSTAB phase B: two arrays read, one written), rotating over nine arrays
+    of n_cells doubles (more than the 256 MB memory-side cache) so that each launch reads
+    arrays the previous two launches did not touch."""
+    import torch
+    bufs = [torch.zeros(n_cells, dtype=torch.float64, device="cuda") for _ in range(9)]
+    out = {}
+    for name, nbytes in (("copy", 16), ("triad", 24)):
+        def launch(q):
+            x, y, z = bufs[(3 * q) % 9], bufs[(3 * q + 1) % 9], bufs[(3 * q + 2) % 9]
+            if name == "copy":
+                z.copy_(x)
+            else:
+                torch.add(x, y, alpha=-0.5, out=z)
+        for q in range(6):
+            launch(q)
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        a.record()
+        for q in range(launches):
+            launch(q)
+        b.record()
+        torch.cuda.synchronize()
+        us = a.elapsed_time(b) * 1e3 / launches
+        out[name] = {"us": round(us, 2), "GB/s": round(nbytes * n_cells / us / 1e3, 1),
+                     "bytes_per_cell": nbytes}
+    del bufs
+    torch.cuda.empty_cache()
+    return out
 
 
 def measured_traffic(workload, kernel):
@@ -558,6 +594,7 @@ def main():
                 "avg_launch_us": round(avg_s * 1e6, 2),
                 "bytes_per_launch": alg_bytes}
     kernels = kernel_table(prof, n_cells, BYTES_PER_CELL)
+    ceiling = stream_ceiling(n_cells)
 
     mixed = None
     if not args.no_mixed and args.solver == "bicgstab":
@@ -661,6 +698,7 @@ def main():
         "profile_steps": args.profile_steps,
         "cpu_baseline": cpu,
         "kernels": kernels,
+        "stream_ceiling": ceiling,
         "mixed": mixed,
         "adjoint": adjoint,
     }
