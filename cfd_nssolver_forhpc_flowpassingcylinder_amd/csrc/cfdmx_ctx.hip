@@ -329,6 +329,18 @@ static bool coarse_inverse(int nx, int ny, int nz, double cx, double cy, double 
 
 // Levels of the multigrid preconditioner (level 0 is the SPE operator of the uniform
 // grid), the inner-solve vectors and state, in arithmetic type T.
+// right-hand side and result of one V-cycle on level 0 (natural-order vectors)
+template <typename T> static void mg_bind_level0(MGLevelT<T> &L0, const T *rhs, T *out)
+{
+    if (L0.split) {
+        L0.fin = rhs;
+        L0.uout = out;
+    } else {
+        L0.f = const_cast<T *>(rhs);
+        L0.u = out;
+    }
+}
+
 template <typename T> static int setup_multigrid(cfdmx_ctx *ctx)
 {
     const Geo &g = ctx->g;
@@ -364,6 +376,13 @@ template <typename T> static int setup_multigrid(cfdmx_ctx *ctx)
     L.fx = L.fy = L.fz = 1;
     L.zper = g.zper;
     divs(L);
+    // a level with even nz is stored by colour; level 0 then has u and f of its own,
+    // filled from and returned to the natural-order Krylov vectors
+    L.split = L.nz % 2 == 0;
+    if (L.split) {
+        ALLOC(L.u, N);
+        ALLOC(L.f, N);
+    }
     lev.push_back(L);
     for (;;) {
         const MGLevelT<T> &P = lev.back();
@@ -390,6 +409,7 @@ template <typename T> static int setup_multigrid(cfdmx_ctx *ctx)
         C.fz = fz;
         C.zper = g.zper;
         divs(C);
+        C.split = C.nz % 2 == 0;
         ALLOC(C.u, (size_t)C.n);
         ALLOC(C.f, (size_t)C.n);
         lev.push_back(C);
@@ -1096,13 +1116,11 @@ static int solve_pb_inner(cfdmx_ctx *ctx, double rr, double tol, int maxit, int 
     const int nlev = (int)L.size();
     for (int it = 0; !ctx->pbs_h->done; ++it) {
         PROF(CFDMX_K_PB, launch_pb_update_p<T>(g, s, it == 0, r, v, p, ctx->pbs));
-        L[0].f = p;
-        L[0].u = ph;
+        mg_bind_level0(L[0], p, ph);
         PROF(CFDMX_K_MG, launch_mg_vcycle<T>(s, L.data(), nlev, 0));
         PROF(CFDMX_K_PB, launch_pb_spmv<T>(g, s, L[0], 1, ph, v, r0, ctx->part, ctx->pbs));
         PROF(CFDMX_K_PB, launch_pb_update_s<T>(g, s, ctx->part, r, v, sv, ctx->pbs));
-        L[0].f = sv;
-        L[0].u = sh;
+        mg_bind_level0(L[0], sv, sh);
         PROF(CFDMX_K_MG, launch_mg_vcycle<T>(s, L.data(), nlev, 0));
         PROF(CFDMX_K_PB, launch_pb_spmv<T>(g, s, L[0], 2, sh, t, sv, ctx->part, ctx->pbs));
         PROF(CFDMX_K_PB, launch_pb_update_xr<T>(g, s, ctx->part, ph, sh, sv, t, r0, e, r,

@@ -86,7 +86,15 @@ struct MGLevelT {
     int fx, fy, fz;     // coarsening factors from the finer level (1 or 2)
     int zper;
     FastDiv dz, dy, dzh; // divisions by nz, ny and nz/2
+    // u and f of the level.  split = 1 (levels with even nz): stored by colour, the red
+    // cells ((i + j + k) even) in [0, n/2) and the black ones in [n/2, n), cell k of row
+    // (i, j) at (i*ny + j)*(nz/2) + k/2 of its colour's half.  split = 0: natural order.
     T *u, *f;
+    int split;
+    // level 0 when split: the natural-order right-hand side and result vectors of the
+    // Krylov solve (read by the first sweep, written by the last)
+    const T *fin;
+    T *uout;
     const T *minv;      // coarsest level: (A + 1 1^T / n)^-1 (symmetric), n x n
 };
 

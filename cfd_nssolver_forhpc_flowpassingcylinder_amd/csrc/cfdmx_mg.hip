@@ -13,15 +13,18 @@
 // exactly by a dense (A + 1 1^T / n)^-1 computed on the host, applied to f minus its
 // mean.  Every inner product accumulates in double.
 //
-// Kernels of a level with even nz take one thread per z-pair (cells 2kk and 2kk+1 of a
-// (i, j) row), so a half sweep runs one thread per cell of its colour:
+// A level with even nz stores u and f by colour (red cells, then black cells), and its
+// kernels take one thread per z-pair (cells 2kk and 2kk+1 of an (i, j) row), so a half
+// sweep runs one thread per cell of its colour and touches only the values it uses:
 //   first sweep from u = 0 (red u = f/d, then black from the red values) -- one kernel;
-//   half sweeps;
+//     on level 0 it also stores the natural-order right-hand side by colour;
+//   half sweeps; on level 0 the last one writes the result in natural order;
 //   restriction (one thread per coarse cell);
 //   prolongation fused with the next red half sweep: the red cell reads its black
 //   neighbours corrected on the fly; the black cells are left uncorrected, since the
 //   black half sweep that follows overwrites them without reading them.
-// Levels with odd nz use one thread per cell.
+// Levels with odd nz are stored in natural order and use one thread per cell.  The
+// arithmetic of every cell update (terms, weights, order) is the same in either layout.
 #include "cfdmx_internal.hpp"
 
 namespace cfdmx {
@@ -92,9 +95,71 @@ __device__ __forceinline__ T mg_nbsum(const MGLevelT<T> &L, int n, int i, int j,
     return s;
 }
 
-// ---- levels with even nz: one thread per z-pair ----
+// storage index of cell (i, j, k) of a level (natural or by colour, see MGLevelT)
+template <typename T>
+__device__ __forceinline__ int mg_at(const MGLevelT<T> &L, int i, int j, int k)
+{
+    const int row = i * L.ny + j;
+    if (!L.split) return row * L.nz + k;
+    return ((i + j + k) & 1) * (L.n >> 1) + row * (L.nz >> 1) + (k >> 1);
+}
 
-// red-black Gauss-Seidel half sweep over the cells with (i + j + k) & 1 == colour
+// mg_nbsum by coordinates: v(a, b, c) gives the value of cell (a, b, c); the same
+// terms, weights and order, and the cell itself with weight 0 for a missing neighbour
+template <typename T, class V>
+__device__ __forceinline__ T mg_nbsum_c(const MGLevelT<T> &L, int i, int j, int k, V &&v)
+{
+    const bool xm = i > 0, xp = i < L.nx - 1, ym = j > 0, yp = j < L.ny - 1;
+    T s = (T)0;
+    s += (xm ? L.cx : (T)0) * v(xm ? i - 1 : i, j, k);
+    s += (xp ? L.cx : (T)0) * v(xp ? i + 1 : i, j, k);
+    s += (ym ? L.cy : (T)0) * v(i, ym ? j - 1 : j, k);
+    s += (yp ? L.cy : (T)0) * v(i, yp ? j + 1 : j, k);
+    if (L.nz > 1) {
+        const bool zm = L.zper || k > 0, zp = L.zper || k < L.nz - 1;
+        const int km = k > 0 ? k - 1 : (zm ? L.nz - 1 : k);
+        const int kp = k < L.nz - 1 ? k + 1 : (zp ? 0 : k);
+        s += (zm ? L.cz : (T)0) * v(i, j, km);
+        s += (zp ? L.cz : (T)0) * v(i, j, kp);
+    }
+    return s;
+}
+
+// ---- levels with even nz: stored by colour, one thread per z-pair ----
+// Thread m is the z-pair (2kk, 2kk+1) of row (i, j); its cell of colour c is
+// k = 2kk + ((i + j + c) & 1), at c*n/2 + m.  The neighbours of that cell are the
+// other colour's cells at the same kk of the rows (i +- 1, j) and (i, j +- 1), and at
+// kk - 1 + p and kk + p of its own row (p = (i + j + c) & 1; periodic z wraps kk).  A
+// half sweep so reads the other colour's half and its own colour's f, and writes its
+// own colour's half: whole cache lines of the values it uses.
+
+// half sweep over colour c: the sum of mg_nbsum (same terms and order; a missing
+// neighbour reads the cell itself with weight 0)
+template <typename T>
+__device__ __forceinline__ T mg_split_update(const MGLevelT<T> &L, const T *u, int m, int row,
+                                             int i, int j, int kk, int c, int &own, int &k)
+{
+    const int half = L.nz >> 1, hn = L.n >> 1;
+    const int pc = (i + j + c) & 1;
+    k = 2 * kk + pc;
+    own = c * hn + m;
+    const int ob = (1 - c) * hn;
+    const int sxh = L.ny * half;
+    const bool xm = i > 0, xp = i < L.nx - 1, ym = j > 0, yp = j < L.ny - 1;
+    T s = (T)0;
+    s += (xm ? L.cx : (T)0) * u[xm ? ob + m - sxh : own];
+    s += (xp ? L.cx : (T)0) * u[xp ? ob + m + sxh : own];
+    s += (ym ? L.cy : (T)0) * u[ym ? ob + m - half : own];
+    s += (yp ? L.cy : (T)0) * u[yp ? ob + m + half : own];
+    // nz is even, so nz > 1
+    const bool zm = L.zper || k > 0, zp = L.zper || k < L.nz - 1;
+    const int kkm = (k > 0) ? kk - 1 + pc : half - 1;  // k - 1, or nz - 1 wrapped
+    const int kkp = (k < L.nz - 1) ? kk + pc : 0;      // k + 1, or 0 wrapped
+    s += (zm ? L.cz : (T)0) * u[zm ? ob + row * half + kkm : own];
+    s += (zp ? L.cz : (T)0) * u[zp ? ob + row * half + kkp : own];
+    return s;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_mg_smooth(MGLevelT<T> L, int colour)
 {
@@ -102,14 +167,30 @@ __global__ void __launch_bounds__(256) k_mg_smooth(MGLevelT<T> L, int colour)
     if (m >= (L.n >> 1)) return;
     int i, j, k0, row;
     mg_pair(L, m, i, j, k0, row);
-    const int k = k0 + ((i + j + colour) & 1);
-    const int n = row * L.nz + k;
-    const T *u = L.u;
-    const T s = mg_nbsum(L, n, i, j, k, [&](int q, int, int, int) { return u[q]; });
-    L.u[n] = (L.f[n] + s) / mg_diag(L, i, j, k);
+    int own, k;
+    const T s = mg_split_update(L, L.u, m, row, i, j, k0 >> 1, colour, own, k);
+    L.u[own] = (L.f[own] + s) / mg_diag(L, i, j, k);
 }
 
-// first sweep from u = 0: red u = f/d, then black u = (f + sum c (f/d)(red nb)) / d
+// the last half sweep of level 0 (black), writing the result of both colours to the
+// natural-order output vector
+template <typename T>
+__global__ void __launch_bounds__(256) k_mg_smooth_out(MGLevelT<T> L)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= (L.n >> 1)) return;
+    int i, j, k0, row;
+    mg_pair(L, m, i, j, k0, row);
+    int own, k;
+    const T s = mg_split_update(L, L.u, m, row, i, j, k0 >> 1, 1, own, k);
+    const T ub = (L.f[own] + s) / mg_diag(L, i, j, k);
+    const int kr = k0 + ((i + j) & 1);
+    L.uout[row * L.nz + k] = ub;
+    L.uout[row * L.nz + kr] = L.u[m];
+}
+
+// first sweep from u = 0: red u = f/d, then black u = (f + sum c (f/d)(red nb)) / d.
+// On level 0 f comes in natural order (fin) and is stored by colour here.
 template <typename T>
 __global__ void __launch_bounds__(256) k_mg_smooth_first(MGLevelT<T> L)
 {
@@ -117,13 +198,25 @@ __global__ void __launch_bounds__(256) k_mg_smooth_first(MGLevelT<T> L)
     if (m >= (L.n >> 1)) return;
     int i, j, k0, row;
     mg_pair(L, m, i, j, k0, row);
+    const int hn = L.n >> 1;
     const int kr = k0 + ((i + j) & 1), kb = k0 + ((i + j + 1) & 1);
-    const int nr = row * L.nz + kr, nb = row * L.nz + kb;
-    const T *f = L.f;
-    L.u[nr] = f[nr] / mg_diag(L, i, j, kr);
-    const T s = mg_nbsum(L, nb, i, j, kb,
-                         [&](int q, int a, int b, int c) { return f[q] / mg_diag(L, a, b, c); });
-    L.u[nb] = (f[nb] + s) / mg_diag(L, i, j, kb);
+    T fr, fb;
+    if (L.fin) {
+        fr = L.fin[row * L.nz + kr];
+        fb = L.fin[row * L.nz + kb];
+        L.f[m] = fr;
+        L.f[hn + m] = fb;
+    } else {
+        fr = L.f[m];
+        fb = L.f[hn + m];
+    }
+    L.u[m] = fr / mg_diag(L, i, j, kr);
+    const T *fin = L.fin, *f = L.f;
+    const T s = mg_nbsum_c(L, i, j, kb, [&](int a, int b, int c) {
+        const T fv = fin ? fin[(a * L.ny + b) * L.nz + c] : f[mg_at(L, a, b, c)];
+        return fv / mg_diag(L, a, b, c);
+    });
+    L.u[hn + m] = (fb + s) / mg_diag(L, i, j, kb);
 }
 
 // prolongation fused with the red half sweep (see the file comment)
@@ -135,20 +228,24 @@ __global__ void __launch_bounds__(256) k_mg_prolong_smooth(MGLevelT<T> F, MGLeve
     int i, j, k0, row;
     mg_pair(F, m, i, j, k0, row);
     const int k = k0 + ((i + j) & 1);
-    const int n = row * F.nz + k;
     const int shx = C.fx >> 1, shy = C.fy >> 1, shz = C.fz >> 1;
     const T *u = F.u, *e = C.u;
-    const T s = mg_nbsum(F, n, i, j, k, [&](int q, int a, int b, int c) {
-        return u[q] + e[((a >> shx) * C.ny + (b >> shy)) * C.nz + (c >> shz)];
+    const T s = mg_nbsum_c(F, i, j, k, [&](int a, int b, int c) {
+        return u[mg_at(F, a, b, c)] + e[mg_at(C, a >> shx, b >> shy, c >> shz)];
     });
-    F.u[n] = (F.f[n] + s) / mg_diag(F, i, j, k);
+    F.u[m] = (F.f[m] + s) / mg_diag(F, i, j, k);
 }
 
 // ---- levels with odd nz: one thread per cell ----
 
-// half sweep; zero_first: u starts at zero (the other colour is written 0)
+// half sweep; zero_first: u starts at zero (the other colour is written 0).  With
+// periodic z and odd nz > 1 the planes k = 0 and k = nz-1 hold the same colour and are
+// neighbours, so a half sweep there runs in two launches, planes = 1 (k < nz-1) and then
+// planes = 2 (k = nz-1): the first reads the old k = nz-1 values, the second the new
+// k = 0 values, and no cell is read while another thread writes it.  planes = 0: all.
 template <typename T>
-__global__ void __launch_bounds__(256) k_mg_smooth_cell(MGLevelT<T> L, int colour, int zero_first)
+__global__ void __launch_bounds__(256) k_mg_smooth_cell(MGLevelT<T> L, int colour, int zero_first,
+                                                        int planes)
 {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= L.n) return;
@@ -158,6 +255,7 @@ __global__ void __launch_bounds__(256) k_mg_smooth_cell(MGLevelT<T> L, int colou
         if (zero_first) L.u[n] = (T)0;
         return;
     }
+    if ((planes == 1 && k == L.nz - 1) || (planes == 2 && k != L.nz - 1)) return;
     const T *u = L.u;
     const T s = zero_first ? (T)0 : mg_nbsum(L, n, i, j, k, [&](int q, int, int, int) { return u[q]; });
     L.u[n] = (L.f[n] + s) / mg_diag(L, i, j, k);
@@ -170,7 +268,7 @@ __global__ void __launch_bounds__(256) k_mg_prolong(MGLevelT<T> F, MGLevelT<T> C
     if (n >= F.n) return;
     int i, j, k;
     mg_coords(F, n, i, j, k);
-    F.u[n] += C.u[((i >> (C.fx >> 1)) * C.ny + (j >> (C.fy >> 1))) * C.nz + (k >> (C.fz >> 1))];
+    F.u[n] += C.u[mg_at(C, i >> (C.fx >> 1), j >> (C.fy >> 1), k >> (C.fz >> 1))];
 }
 
 // coarse right-hand side: mean over the children of the fine residual f - A u
@@ -192,11 +290,13 @@ __global__ void __launch_bounds__(256) k_mg_restrict(MGLevelT<T> F, MGLevelT<T> 
             for (int c = 0; c < 2; ++c) {
                 if (a >= C.fx || b >= C.fy || c >= C.fz) continue;
                 const int i = I * C.fx + a, j = J * C.fy + b, k = K * C.fz + c;
-                const int n = (i * F.ny + j) * F.nz + k;
-                const T s = mg_nbsum(F, n, i, j, k, [&](int q, int, int, int) { return u[q]; });
+                const int n = mg_at(F, i, j, k);
+                const T s = mg_nbsum_c(F, i, j, k, [&](int x, int y, int z) {
+                    return u[mg_at(F, x, y, z)];
+                });
                 acc += F.f[n] - (mg_diag(F, i, j, k) * u[n] - s);
             }
-    C.f[m] = acc / (T)(C.fx * C.fy * C.fz);
+    C.f[mg_at(C, I, J, K)] = acc / (T)(C.fx * C.fy * C.fz);
 }
 
 // coarsest level: u = M^-1 (f - mean f).  One wave per row: lane l takes the columns
@@ -208,10 +308,16 @@ __global__ void __launch_bounds__(256) k_mg_coarse(MGLevelT<T> L)
 {
     __shared__ T g[kMGCoarseMax];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // natural cell order n; the level may be stored by colour
+    auto at = [&](int n) {
+        int i, j, k;
+        mg_coords(L, n, i, j, k);
+        return mg_at(L, i, j, k);
+    };
     double part = 0.0;
-    for (int n = lane; n < L.n; n += 64) part += (double)L.f[n];
+    for (int n = lane; n < L.n; n += 64) part += (double)L.f[at(n)];
     const double mean = __shfl(wave_fold(part), 0, 64) / (double)L.n;
-    for (int n = threadIdx.x; n < L.n; n += 256) g[n] = (T)((double)L.f[n] - mean);
+    for (int n = threadIdx.x; n < L.n; n += 256) g[n] = (T)((double)L.f[at(n)] - mean);
     __syncthreads();
     const int r = blockIdx.x * 4 + wave;
     if (r >= L.n) return;
@@ -223,7 +329,7 @@ __global__ void __launch_bounds__(256) k_mg_coarse(MGLevelT<T> L)
         if (c < L.n) acc += row[c] * g[c];
     }
     for (int sft = 32; sft >= 1; sft >>= 1) acc += __shfl_down(acc, sft, 64);
-    if (lane == 0) L.u[r] = acc;
+    if (lane == 0) L.u[at(r)] = acc;
 }
 
 // ---- preconditioned BiCGSTAB pieces (linear cell order, canonical block sums) ----
@@ -370,20 +476,34 @@ void launch_mg_vcycle(hipStream_t s, const MGLevelT<T> *L, int nlev, int l)
         hipLaunchKernelGGL(k_mg_prolong_smooth<T>, dim3(nb), dim3(256), 0, s, F, C);
         hipLaunchKernelGGL(k_mg_smooth<T>, dim3(nb), dim3(256), 0, s, F, 1);
         hipLaunchKernelGGL(k_mg_smooth<T>, dim3(nb), dim3(256), 0, s, F, 0);
-        hipLaunchKernelGGL(k_mg_smooth<T>, dim3(nb), dim3(256), 0, s, F, 1);
+        if (F.uout)
+            hipLaunchKernelGGL(k_mg_smooth_out<T>, dim3(nb), dim3(256), 0, s, F);
+        else
+            hipLaunchKernelGGL(k_mg_smooth<T>, dim3(nb), dim3(256), 0, s, F, 1);
         return;
     }
     const int nb = cdiv(F.n, 256);
-    hipLaunchKernelGGL(k_mg_smooth_cell<T>, dim3(nb), dim3(256), 0, s, F, 0, 1);
-    hipLaunchKernelGGL(k_mg_smooth_cell<T>, dim3(nb), dim3(256), 0, s, F, 1, 0);
-    hipLaunchKernelGGL(k_mg_smooth_cell<T>, dim3(nb), dim3(256), 0, s, F, 0, 0);
-    hipLaunchKernelGGL(k_mg_smooth_cell<T>, dim3(nb), dim3(256), 0, s, F, 1, 0);
+    // a half sweep that reads neighbours; two launches where the periodic z wrap joins
+    // two cells of one colour (see k_mg_smooth_cell)
+    const bool wrap_same = F.zper && F.nz > 1;
+    auto half_sweep = [&](int colour) {
+        if (wrap_same) {
+            hipLaunchKernelGGL(k_mg_smooth_cell<T>, dim3(nb), dim3(256), 0, s, F, colour, 0, 1);
+            hipLaunchKernelGGL(k_mg_smooth_cell<T>, dim3(nb), dim3(256), 0, s, F, colour, 0, 2);
+        } else {
+            hipLaunchKernelGGL(k_mg_smooth_cell<T>, dim3(nb), dim3(256), 0, s, F, colour, 0, 0);
+        }
+    };
+    hipLaunchKernelGGL(k_mg_smooth_cell<T>, dim3(nb), dim3(256), 0, s, F, 0, 1, 0);
+    half_sweep(1);
+    half_sweep(0);
+    half_sweep(1);
     hipLaunchKernelGGL(k_mg_restrict<T>, dim3(cdiv(C.n, 256)), dim3(256), 0, s, F, C);
     launch_mg_vcycle(s, L, nlev, l + 1);
     hipLaunchKernelGGL(k_mg_prolong<T>, dim3(nb), dim3(256), 0, s, F, C);
     for (int q = 0; q < 2; ++q) {
-        hipLaunchKernelGGL(k_mg_smooth_cell<T>, dim3(nb), dim3(256), 0, s, F, 0, 0);
-        hipLaunchKernelGGL(k_mg_smooth_cell<T>, dim3(nb), dim3(256), 0, s, F, 1, 0);
+        half_sweep(0);
+        half_sweep(1);
     }
 }
 
