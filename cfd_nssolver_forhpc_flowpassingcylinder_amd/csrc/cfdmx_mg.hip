@@ -236,6 +236,69 @@ __global__ void __launch_bounds__(256) k_mg_prolong_smooth(MGLevelT<T> F, MGLeve
     F.u[m] = (F.f[m] + s) / mg_diag(F, i, j, k);
 }
 
+// ---- z-only coarsening (fx = fy = 1, fz = 2) of a level stored by colour: the
+// children of coarse cell (i, j, kk) are fine z-pair m, so these run one thread per
+// z-pair with the fast neighbour indexing ----
+
+// coarse right-hand side: the mean of the two children's residuals f - A u (the same
+// expression and order as k_mg_restrict)
+template <typename T>
+__global__ void __launch_bounds__(256) k_mg_restrict_z(MGLevelT<T> F, MGLevelT<T> C)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= (F.n >> 1)) return;
+    int i, j, k0, row;
+    mg_pair(F, m, i, j, k0, row);
+    const int kk = k0 >> 1;
+    T r[2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        int own, k;
+        const T s = mg_split_update(F, F.u, m, row, i, j, kk, c, own, k);
+        r[c] = F.f[own] - (mg_diag(F, i, j, k) * F.u[own] - s);
+    }
+    // children in z order: cell 2kk has colour (i + j) & 1
+    const bool even_red = ((i + j) & 1) == 0;
+    T acc = (T)0;
+    acc += even_red ? r[0] : r[1];
+    acc += even_red ? r[1] : r[0];
+    C.f[mg_at(C, i, j, kk)] = acc / (T)2;
+}
+
+// prolongation fused with the red half sweep: each black neighbour is corrected by its
+// parent's value, the parent of fine cell (a, b, c) being coarse cell (a, b, c/2)
+template <typename T>
+__global__ void __launch_bounds__(256) k_mg_prolong_smooth_z(MGLevelT<T> F, MGLevelT<T> C)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= (F.n >> 1)) return;
+    int i, j, k0, row;
+    mg_pair(F, m, i, j, k0, row);
+    const int kk = k0 >> 1;
+    const int half = F.nz >> 1, hn = F.n >> 1, sxh = F.ny * half;
+    const int pr = (i + j) & 1;
+    const int k = k0 + pr;
+    const T *u = F.u, *e = C.u;
+    // fine black index (other half) and coarse index of each neighbour, in the order of
+    // mg_nbsum; a missing neighbour is the red cell itself, weight 0
+    const bool xm = i > 0, xp = i < F.nx - 1, ym = j > 0, yp = j < F.ny - 1;
+    const bool zm = F.zper || k > 0, zp = F.zper || k < F.nz - 1;
+    const int kzm = k > 0 ? k - 1 : F.nz - 1, kzp = k < F.nz - 1 ? k + 1 : 0;
+    const int kkm = (k > 0) ? kk - 1 + pr : half - 1;
+    const int kkp = (k < F.nz - 1) ? kk + pr : 0;
+    auto val = [&](bool ok, int fine, int a, int b, int c) {
+        return ok ? u[hn + fine] + e[mg_at(C, a, b, c >> 1)] : u[m] + e[mg_at(C, i, j, kk)];
+    };
+    T s = (T)0;
+    s += (xm ? F.cx : (T)0) * val(xm, m - sxh, i - 1, j, k);
+    s += (xp ? F.cx : (T)0) * val(xp, m + sxh, i + 1, j, k);
+    s += (ym ? F.cy : (T)0) * val(ym, m - half, i, j - 1, k);
+    s += (yp ? F.cy : (T)0) * val(yp, m + half, i, j + 1, k);
+    s += (zm ? F.cz : (T)0) * val(zm, row * half + kkm, i, j, kzm);
+    s += (zp ? F.cz : (T)0) * val(zp, row * half + kkp, i, j, kzp);
+    F.u[m] = (F.f[m] + s) / mg_diag(F, i, j, k);
+}
+
 // ---- levels with odd nz: one thread per cell ----
 
 // half sweep; zero_first: u starts at zero (the other colour is written 0).  With
@@ -471,9 +534,16 @@ void launch_mg_vcycle(hipStream_t s, const MGLevelT<T> *L, int nlev, int l)
         hipLaunchKernelGGL(k_mg_smooth_first<T>, dim3(nb), dim3(256), 0, s, F);
         hipLaunchKernelGGL(k_mg_smooth<T>, dim3(nb), dim3(256), 0, s, F, 0);
         hipLaunchKernelGGL(k_mg_smooth<T>, dim3(nb), dim3(256), 0, s, F, 1);
-        hipLaunchKernelGGL(k_mg_restrict<T>, dim3(cdiv(C.n, 256)), dim3(256), 0, s, F, C);
+        const bool zonly = C.fx == 1 && C.fy == 1 && C.fz == 2;
+        if (zonly)
+            hipLaunchKernelGGL(k_mg_restrict_z<T>, dim3(nb), dim3(256), 0, s, F, C);
+        else
+            hipLaunchKernelGGL(k_mg_restrict<T>, dim3(cdiv(C.n, 256)), dim3(256), 0, s, F, C);
         launch_mg_vcycle(s, L, nlev, l + 1);
-        hipLaunchKernelGGL(k_mg_prolong_smooth<T>, dim3(nb), dim3(256), 0, s, F, C);
+        if (zonly)
+            hipLaunchKernelGGL(k_mg_prolong_smooth_z<T>, dim3(nb), dim3(256), 0, s, F, C);
+        else
+            hipLaunchKernelGGL(k_mg_prolong_smooth<T>, dim3(nb), dim3(256), 0, s, F, C);
         hipLaunchKernelGGL(k_mg_smooth<T>, dim3(nb), dim3(256), 0, s, F, 1);
         hipLaunchKernelGGL(k_mg_smooth<T>, dim3(nb), dim3(256), 0, s, F, 0);
         if (F.uout)
