@@ -344,21 +344,22 @@ __global__ void __launch_bounds__(256) k_mg_restrict(MGLevelT<T> F, MGLevelT<T> 
     mg_coords(C, m, I, J, K);
     const T *u = F.u;
     T acc = (T)0;
-    // at most 2 x 2 x 2 children, unrolled so that their loads are in flight together
+    // at most 2 x 2 x 2 children in the order (a, b, c), unrolled and branch-free so that
+    // their loads are in flight together: a child that does not exist evaluates child
+    // (0, 0, 0) and adds +0, an identity (the sum starts at +0 and so never holds -0)
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b)
-#pragma unroll
-            for (int c = 0; c < 2; ++c) {
-                if (a >= C.fx || b >= C.fy || c >= C.fz) continue;
-                const int i = I * C.fx + a, j = J * C.fy + b, k = K * C.fz + c;
-                const int n = mg_at(F, i, j, k);
-                const T s = mg_nbsum_c(F, i, j, k, [&](int x, int y, int z) {
-                    return u[mg_at(F, x, y, z)];
-                });
-                acc += F.f[n] - (mg_diag(F, i, j, k) * u[n] - s);
-            }
+    for (int q = 0; q < 8; ++q) {
+        const int a = q >> 2, b = (q >> 1) & 1, c = q & 1;
+        const bool ok = a < C.fx && b < C.fy && c < C.fz;
+        const int i = I * C.fx + (ok ? a : 0), j = J * C.fy + (ok ? b : 0),
+                  k = K * C.fz + (ok ? c : 0);
+        const int n = mg_at(F, i, j, k);
+        const T s = mg_nbsum_c(F, i, j, k, [&](int x, int y, int z) {
+            return u[mg_at(F, x, y, z)];
+        });
+        const T r = F.f[n] - (mg_diag(F, i, j, k) * u[n] - s);
+        acc += ok ? r : (T)0;
+    }
     C.f[mg_at(C, I, J, K)] = acc / (T)(C.fx * C.fy * C.fz);
 }
 
