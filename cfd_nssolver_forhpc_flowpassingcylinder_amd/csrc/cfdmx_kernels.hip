@@ -1264,9 +1264,7 @@ k_mixed_resid(Geo g, int pre, const double *__restrict__ b, const double *__rest
     const int L = logical_block(g);
     if (L >= g.nblocks) return;
     double acc[3] = {0.0, 0.0, 0.0};
-    for (int c = 0; c < kCells; ++c) {
-        const int n = L * kPerBlock + c * kThreads + threadIdx.x;
-        if (n >= g.N) break;
+    auto cell = [&](int n) {
         int i, j, k;
         decode(g, n, i, j, k);
         RowCoef rc;
@@ -1282,6 +1280,17 @@ k_mixed_resid(Geo g, int pre, const double *__restrict__ b, const double *__rest
         acc[0] += bn * bn;
         acc[1] += rr * rr;
         acc[2] += (double)rf * (double)rf;
+    };
+    // a full block has no bounds test, so its cells' loads overlap
+    if ((L + 1) * kPerBlock <= g.N) {
+#pragma unroll
+        for (int c = 0; c < kCells; ++c) cell(L * kPerBlock + c * kThreads + threadIdx.x);
+    } else {
+        for (int c = 0; c < kCells; ++c) {
+            const int n = L * kPerBlock + c * kThreads + threadIdx.x;
+            if (n >= g.N) break;
+            cell(n);
+        }
     }
     block_sums<3>(acc, part, g.nblocks, L);
 }

@@ -409,9 +409,8 @@ k_pb_spmv(Geo g, MGLevelT<T> L, const T *__restrict__ in, T *__restrict__ out,
     if (B >= g.nblocks) return;
     double acc[NS];
     for (int q = 0; q < NS; ++q) acc[q] = 0.0;
-    for (int c = 0; c < kCells; ++c) {
-        const int n = B * kPerBlock + c * kThreads + threadIdx.x;
-        if (n < g.N) {
+    auto cell = [&](int n) {
+        {
             int i, j, k;
             mg_coords(L, n, i, j, k);
             // branch-free: a missing neighbour reads the cell itself with weight 0
@@ -435,6 +434,16 @@ k_pb_spmv(Geo g, MGLevelT<T> L, const T *__restrict__ in, T *__restrict__ out,
             out[n] = o;
             acc[0] += (double)o * (double)w[n];
             if (NS == 2) acc[NS - 1] += (double)o * (double)o;
+        }
+    };
+    // a full block has no bounds test, so its cells' loads overlap
+    if ((B + 1) * kPerBlock <= g.N) {
+#pragma unroll
+        for (int c = 0; c < kCells; ++c) cell(B * kPerBlock + c * kThreads + threadIdx.x);
+    } else {
+        for (int c = 0; c < kCells; ++c) {
+            const int n = B * kPerBlock + c * kThreads + threadIdx.x;
+            if (n < g.N) cell(n);
         }
     }
     block_sums<NS>(acc, part, g.nblocks, B);
