@@ -62,7 +62,7 @@ class Config(C.Structure):
         ("convective_speed", C.c_double), ("flux_correction", C.c_int),
         ("solver", C.c_int), ("precond", C.c_int), ("tol", C.c_double), ("maxit", C.c_int),
         ("sor_omega", C.c_double), ("skip_first_pressure", C.c_int), ("precision", C.c_int),
-        ("inner_tol", C.c_double), ("xcd_remap", C.c_int),
+        ("inner_tol", C.c_double), ("xcd_remap", C.c_int), ("mg_graphs", C.c_int),
     ]
 
 

@@ -37,6 +37,14 @@ struct cfdmx_ctx {
     std::vector<MGLevelT<float>> mg32;
     std::vector<MGLevelT<double>> mg64;
     PBState *pbs = nullptr, *pbs_h = nullptr;
+    // V-cycles captured as HIP graphs, one per element type and level-0 binding
+    struct VGraph {
+        hipGraphExec_t exec;
+        size_t tsize;
+        const void *rhs;
+        void *out;
+    };
+    std::vector<VGraph> vgraphs;
     double *pb_part = nullptr;
     double *mix = nullptr, *mix_h = nullptr;
     int mix_outer = 0;
@@ -339,6 +347,46 @@ template <typename T> static void mg_bind_level0(MGLevelT<T> &L0, const T *rhs, 
         L0.f = const_cast<T *>(rhs);
         L0.u = out;
     }
+}
+
+// one V-cycle on rhs -> out (natural-order level-0 vectors).  With cfg.mg_graphs the
+// cycle's launches are captured once per (type, rhs, out) into a HIP graph and replayed;
+// the cycle has no memory sets or library calls, only kernels.
+template <typename T> static int run_vcycle(cfdmx_ctx *ctx, const T *rhs, T *out)
+{
+    std::vector<MGLevelT<T>> &L = mg_levels<T>(ctx);
+    const int nlev = (int)L.size();
+    hipStream_t s = ctx->stream;
+    mg_bind_level0(L[0], rhs, out);
+    if (!ctx->cfg.mg_graphs) {
+        launch_mg_vcycle<T>(s, L.data(), nlev, 0);
+        LAUNCHCHK();
+        return CFDMX_OK;
+    }
+    for (const auto &v : ctx->vgraphs)
+        if (v.tsize == sizeof(T) && v.rhs == rhs && v.out == out) {
+            HIPCHK(hipGraphLaunch(v.exec, s));
+            return CFDMX_OK;
+        }
+    hipGraph_t graph = nullptr;
+    HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    launch_mg_vcycle<T>(s, L.data(), nlev, 0);
+    const hipError_t le = hipGetLastError();
+    const hipError_t ce = hipStreamEndCapture(s, &graph);
+    if (le != hipSuccess || ce != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        FAIL(CFDMX_ERR_HIP, "multigrid: capturing the V-cycle failed: %s",
+             hipGetErrorString(le != hipSuccess ? le : ce));
+    }
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ie != hipSuccess)
+        FAIL(CFDMX_ERR_HIP, "multigrid: instantiating the V-cycle graph failed: %s",
+             hipGetErrorString(ie));
+    ctx->vgraphs.push_back({exec, sizeof(T), rhs, out});
+    HIPCHK(hipGraphLaunch(exec, s));
+    return CFDMX_OK;
 }
 
 template <typename T> static int setup_multigrid(cfdmx_ctx *ctx)
@@ -697,6 +745,7 @@ int cfdmx_config_default(cfdmx_config *cfg)
     cfg->precision = CFDMX_PRECISION_FP64;
     cfg->inner_tol = 1e-5;
     cfg->xcd_remap = 1;
+    cfg->mg_graphs = 1;
     return CFDMX_OK;
 }
 
@@ -715,6 +764,7 @@ int cfdmx_destroy(cfdmx_ctx *ctx)
     if (ctx->p3d.ready) (void)hipEventDestroy(ctx->p3d.ready);
     if (ctx->p3d.host) (void)hipHostFree(ctx->p3d.host);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto &v : ctx->vgraphs) (void)hipGraphExecDestroy(v.exec);
     for (void *p : ctx->allocs) (void)hipFree(p);
     prof_harvest(ctx);
     for (hipEvent_t e : ctx->prof_pool) (void)hipEventDestroy(e);
@@ -1104,6 +1154,7 @@ static int solve_pb_inner(cfdmx_ctx *ctx, double rr, double tol, int maxit, int 
     T *r = pv[0], *r0 = pv[1], *p = pv[2], *v = pv[3], *sv = pv[4], *t = pv[5], *ph = pv[6],
       *sh = pv[7];
     T *e = static_cast<T *>(ctx->ecorr);
+    int rc = CFDMX_OK;
     PBState ps = {};
     ps.rho = rr; // (r, r0) with r0 = r
     ps.norm0 = std::sqrt(rr);
@@ -1113,15 +1164,14 @@ static int solve_pb_inner(cfdmx_ctx *ctx, double rr, double tol, int maxit, int 
     *ctx->pbs_h = ps;
     HIPCHK(hipMemcpyAsync(ctx->pbs, ctx->pbs_h, sizeof ps, hipMemcpyHostToDevice, s));
     std::vector<MGLevelT<T>> &L = mg_levels<T>(ctx);
-    const int nlev = (int)L.size();
     for (int it = 0; !ctx->pbs_h->done; ++it) {
         PROF(CFDMX_K_PB, launch_pb_update_p<T>(g, s, it == 0, r, v, p, ctx->pbs));
-        mg_bind_level0(L[0], p, ph);
-        PROF(CFDMX_K_MG, launch_mg_vcycle<T>(s, L.data(), nlev, 0));
+        PROF(CFDMX_K_MG, rc = run_vcycle<T>(ctx, p, ph));
+        if (rc) return rc;
         PROF(CFDMX_K_PB, launch_pb_spmv<T>(g, s, L[0], 1, ph, v, r0, ctx->part, ctx->pbs));
         PROF(CFDMX_K_PB, launch_pb_update_s<T>(g, s, ctx->part, r, v, sv, ctx->pbs));
-        mg_bind_level0(L[0], sv, sh);
-        PROF(CFDMX_K_MG, launch_mg_vcycle<T>(s, L.data(), nlev, 0));
+        PROF(CFDMX_K_MG, rc = run_vcycle<T>(ctx, sv, sh));
+        if (rc) return rc;
         PROF(CFDMX_K_PB, launch_pb_spmv<T>(g, s, L[0], 2, sh, t, sv, ctx->part, ctx->pbs));
         PROF(CFDMX_K_PB, launch_pb_update_xr<T>(g, s, ctx->part, ph, sh, sv, t, r0, e, r,
                                                 ctx->pb_part, ctx->pbs));

@@ -74,6 +74,8 @@ typedef struct cfdmx_config {
                          (default 1e-5) */
     /* launch shaping */
     int xcd_remap; /* map consecutive logical blocks to one XCD (0 = hardware order) */
+    int mg_graphs; /* multigrid modes: replay each V-cycle from a captured HIP graph
+                      (default 1; 0 launches its kernels one by one, same results) */
 } cfdmx_config;
 
 typedef struct cfdmx_ctx cfdmx_ctx;

@@ -17,7 +17,7 @@ BC_PRESETS = {
 def make_pair(nxc, nyc, nzc, *, lengths=(20.0, 10.0, 0.5), kind=0, re=100.0, dt=0.002,
               solver=0, precond=0, tol=1e-6, maxit=5000, omega=1.8, bc="flowpassing",
               seed=20261202, amp=1e-3, cyl=(5.0, 5.0, 0.5), xcd_remap=1, smoothed_eps=None,
-              skip_first=1, inflow=None, precision=0, inner_tol=1e-5):
+              skip_first=1, inflow=None, precision=0, inner_tol=1e-5, mg_graphs=1):
     num, dirr, zper, conv, flux = BC_PRESETS[bc]
     cfg = pkg.default_config(nx=nxc, ny=nyc, nz=nzc, lx=lengths[0], ly=lengths[1],
                              lz=lengths[2], grid_kind=kind, re=re, dt=dt, solver=solver,
@@ -25,7 +25,7 @@ def make_pair(nxc, nyc, nzc, *, lengths=(20.0, 10.0, 0.5), kind=0, re=100.0, dt=
                              bc_num=num, bc_dir=dirr, z_periodic=zper, outflow_convective=conv,
                              convective_speed=1.0, flux_correction=flux,
                              skip_first_pressure=skip_first, xcd_remap=xcd_remap,
-                             precision=precision, inner_tol=inner_tol)
+                             precision=precision, inner_tol=inner_tol, mg_graphs=mg_graphs)
     gpu = pkg.FlowSolver(cfg)
     o = op.Oracle()
     g = o.grid(nxc, nyc, nzc, *lengths, kind=kind)
