@@ -46,6 +46,10 @@ BYTES_PER_CELL = {
 
 MIXED_INNER_TOL = 1e-5
 
+# multigrid V-cycles from captured HIP graphs (cfdmx_config.mg_graphs); --no-mg-graphs
+# launches them kernel by kernel (the same bits and, at these sizes, the same time)
+MG_GRAPHS = 1
+
 # the same phases on float Krylov vectors (mixed-precision inner solve); the FP64
 # residual/update passes of the refinement are counted under bicg_init
 BYTES_PER_CELL_FP32 = {
@@ -72,7 +76,7 @@ def make_solver(wl, precond, solver, seed, maxit=5000, precision=0):
         solver=SOLVER_RBSOR if solver == "rbsor" else SOLVER_BICGSTAB, sor_omega=1.8,
         bc_num=slip_num, bc_dir=slip_dir, z_periodic=1, outflow_convective=1,
         convective_speed=1.0, flux_correction=1, skip_first_pressure=1,
-        precision=precision, inner_tol=MIXED_INNER_TOL)
+        precision=precision, inner_tol=MIXED_INNER_TOL, mg_graphs=MG_GRAPHS)
     s = pkg.FlowSolver(cfg)
     s.init_flow(1.0, 1e-3, seed)
     s.set_eta_cylinder_z(5.0, 5.0, 0.5, 100)
@@ -123,6 +127,17 @@ def stream_ceiling(n_cells, launches=60):
     del bufs
     torch.cuda.empty_cache()
     return out
+
+
+def stream_ceiling_child(n_cells):
+    """stream_ceiling in a child process: torch brings a HIP runtime of its own, which
+    does not share a process with the library's."""
+    import subprocess
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--stream-ceiling",
+                        str(n_cells)], capture_output=True, text=True, timeout=300)
+    if r.returncode != 0:
+        return {"error": r.stderr.strip().splitlines()[-1:] or ["failed"]}
+    return json.loads(r.stdout.strip().splitlines()[-1])
 
 
 def measured_traffic(workload, kernel):
@@ -257,7 +272,8 @@ def case_solver(a, seed, precision=0, tol=None, inflow_amp=0.0):
         nx=a["nx"], ny=a["ny"], nz=a["nz"], lx=20.0, ly=10.0, lz=0.5, re=a["re"], dt=a["dt"],
         tol=tol if tol else a["tol"], maxit=5000, bc_num=SLIP_NUM, bc_dir=SLIP_DIR,
         z_periodic=1, outflow_convective=1, convective_speed=1.0, flux_correction=1,
-        skip_first_pressure=1, precision=precision, inner_tol=MIXED_INNER_TOL)
+        skip_first_pressure=1, precision=precision, inner_tol=MIXED_INNER_TOL,
+        mg_graphs=MG_GRAPHS)
     s = pkg.FlowSolver(cfg)
     prof = None
     if inflow_amp:
@@ -541,13 +557,27 @@ def main():
                     help="steps of --part large (default: the configuration's 100)")
     ap.add_argument("--no-fd", action="store_true",
                     help="--part large / adjoint: skip the finite-difference gradient checks")
+    ap.add_argument("--no-mg-graphs", action="store_true",
+                    help="launch the multigrid V-cycles kernel by kernel instead of "
+                         "replaying captured HIP graphs (for profiler runs)")
+    ap.add_argument("--no-stream-ceiling", action="store_true",
+                    help="skip the streaming-ceiling child process (under a profiler, whose "
+                         "preloaded library would also wrap the child's own HIP runtime)")
+    ap.add_argument("--stream-ceiling", type=int, default=0, metavar="CELLS",
+                    help="only measure the cache-flushed copy and triad rates over arrays "
+                         "of CELLS doubles, print them as JSON and exit")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB three times (the "
                          "calibration kernel for HBM byte counters) and exit")
     args = ap.parse_args()
+    global MG_GRAPHS
+    MG_GRAPHS = 0 if args.no_mg_graphs else 1
 
     if args.copy_probe:
         copy_probe(args.copy_probe)
+        return
+    if args.stream_ceiling:
+        print(json.dumps(stream_ceiling(args.stream_ceiling)), flush=True)
         return
 
     if args.gpus != 1:
@@ -594,7 +624,7 @@ def main():
                 "avg_launch_us": round(avg_s * 1e6, 2),
                 "bytes_per_launch": alg_bytes}
     kernels = kernel_table(prof, n_cells, BYTES_PER_CELL)
-    ceiling = stream_ceiling(n_cells)
+    ceiling = None if args.no_stream_ceiling else stream_ceiling_child(n_cells)
 
     mixed = None
     if not args.no_mixed and args.solver == "bicgstab":
