@@ -210,3 +210,30 @@ def test_xcd_remap_does_not_change_results():
             assert np.array_equal(x, y)
     a.close()
     b.close()
+
+
+def test_full_size_convdiff_rhs_and_bicgstab_bitwise():
+    """The metric's grid, 512x256x64 with the cylinder BCs, so the kernels run at their
+    production launch shapes: conv-diff tiles of 32 x 8 cells over 8-plane chunks (1024
+    blocks), 1024 full BiCGSTAB march tiles with the XCD remap.  Conv-diff, its CFL
+    maximum, the right-hand side and a BiCGSTAB solve capped at 12 iterations (it ends at
+    the cap, which keeps the oracle to seconds) against the oracle, bit for bit."""
+    maxit = 12
+    gpu, sim, o, g = make_pair(512, 256, 64, precond=0, tol=1e-10, maxit=maxit, bc="cylinder")
+    gpu.convdiff()
+    o.convdiff(g, 0.002, 1.0 / 100.0, sim.field("u0"), sim.field("v0"), sim.field("w0"),
+               sim.field("u1"), sim.field("v1"), sim.field("w1"), z_periodic=1)
+    for got, name in zip(gpu.download_velocity(1), ("u1", "v1", "w1")):
+        assert np.array_equal(got, sim.field(name)), name
+    assert gpu.cfl() == o.cfl(g, 0.002, sim.field("u1"), sim.field("v1"), sim.field("w1"),
+                              z_periodic=1)
+    gpu.pressure_rhs()
+    b = o.rhs(g, 0.002, sim.field("u1"), sim.field("v1"), sim.field("w1"), 0, 1)
+    assert np.array_equal(gpu.rhs(), b)
+    it_g, res_g = gpu.solve_bicgstab()
+    xo = sim.field("x")
+    it_o, res_o = op.BiCG(o, g).solve(b, xo, 1e-10, 0, 1, op.RED_BLOCKED, maxit=maxit)
+    print(f"512x256x64: {it_g} iterations, residual {res_g:.6e}")
+    assert it_g == it_o and _same(res_g, res_o), (it_g, it_o, res_g, res_o)
+    assert np.array_equal(gpu.xresult(), xo, equal_nan=True)
+    gpu.close()
