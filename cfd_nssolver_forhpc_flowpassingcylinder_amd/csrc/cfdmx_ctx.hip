@@ -273,67 +273,46 @@ template <typename T> static std::vector<MGLevelT<T>> &mg_levels(cfdmx_ctx *ctx)
 template <> std::vector<MGLevelT<float>> &mg_levels<float>(cfdmx_ctx *ctx) { return ctx->mg32; }
 template <> std::vector<MGLevelT<double>> &mg_levels<double>(cfdmx_ctx *ctx) { return ctx->mg64; }
 
-// (A + 1 1^T / n)^-1 of a level's Laplacian (the operator of the multigrid kernels:
-// c_d per existing neighbour, periodic z wrapped), by Cholesky on the host in FP64.
-// A's null space is the constants, so the matrix is positive definite and its inverse
-// maps a mean-free f to the mean-free solution of A u = f.  Symmetric, so the layout
-// is both row- and column-major.
-static bool coarse_inverse(int nx, int ny, int nz, double cx, double cy, double cz, int zper,
-                           std::vector<double> &inv)
+// Eigenpairs of the 1-D operator of one axis of a multigrid level: c (2 u_i - u_(i-1) -
+// u_(i+1)) over n cells, with the end faces folded into the diagonal (Neumann:
+// q_m(i) = cos(pi m (i + 1/2) / n), lambda_m = c (2 - 2 cos(pi m / n))) or wrapped
+// (periodic: the real Fourier modes, lambda_m = c (2 - 2 cos(2 pi m / n))).  Q holds
+// the orthonormal modes as rows (Q[m n + i]); a single cell has the one mode 1, with 0.
+static void axis_eigen(int n, double c, bool periodic, std::vector<double> &Q,
+                       std::vector<double> &lam)
 {
-    const int n = nx * ny * nz;
-    std::vector<double> M((size_t)n * n, 1.0 / n);
-    for (int i = 0; i < nx; ++i)
-        for (int j = 0; j < ny; ++j)
-            for (int k = 0; k < nz; ++k) {
-                const int c = (i * ny + j) * nz + k;
-                auto add = [&](int q, double w) {
-                    M[(size_t)c * n + c] += w;
-                    M[(size_t)c * n + q] -= w;
-                };
-                if (i > 0) add(c - ny * nz, cx);
-                if (i < nx - 1) add(c + ny * nz, cx);
-                if (j > 0) add(c - nz, cy);
-                if (j < ny - 1) add(c + nz, cy);
-                if (nz > 1) {
-                    if (zper) {
-                        add(k > 0 ? c - 1 : c + nz - 1, cz);
-                        add(k < nz - 1 ? c + 1 : c - (nz - 1), cz);
-                    } else {
-                        if (k > 0) add(c - 1, cz);
-                        if (k < nz - 1) add(c + 1, cz);
-                    }
-                }
-            }
-    // M = L L^T, L in the lower triangle of M
-    for (int j = 0; j < n; ++j) {
-        double d = M[(size_t)j * n + j];
-        for (int k = 0; k < j; ++k) d -= M[(size_t)j * n + k] * M[(size_t)j * n + k];
-        if (!(d > 0.0)) return false;
-        const double ljj = std::sqrt(d);
-        M[(size_t)j * n + j] = ljj;
-        for (int i = j + 1; i < n; ++i) {
-            double v = M[(size_t)i * n + j];
-            for (int k = 0; k < j; ++k) v -= M[(size_t)i * n + k] * M[(size_t)j * n + k];
-            M[(size_t)i * n + j] = v / ljj;
-        }
+    const double pi = 3.14159265358979323846;
+    Q.assign((size_t)n * n, 0.0);
+    lam.assign(n, 0.0);
+    if (n == 1) {
+        Q[0] = 1.0;
+        return;
     }
-    inv.assign((size_t)n * n, 0.0);
-    std::vector<double> y(n);
-    for (int col = 0; col < n; ++col) {
+    if (!periodic) {
+        for (int m = 0; m < n; ++m) {
+            const double a = m == 0 ? std::sqrt(1.0 / n) : std::sqrt(2.0 / n);
+            for (int i = 0; i < n; ++i) Q[(size_t)m * n + i] = a * std::cos(pi * m * (i + 0.5) / n);
+            lam[m] = c * (2.0 - 2.0 * std::cos(pi * m / n));
+        }
+        return;
+    }
+    int row = 0;
+    auto put = [&](int m, bool sine, double a) {
         for (int i = 0; i < n; ++i) {
-            double v = i == col ? 1.0 : 0.0;
-            for (int k = 0; k < i; ++k) v -= M[(size_t)i * n + k] * y[k];
-            y[i] = v / M[(size_t)i * n + i];
+            const double t = 2.0 * pi * m * i / n;
+            Q[(size_t)row * n + i] = a * (sine ? std::sin(t) : std::cos(t));
         }
-        for (int i = n - 1; i >= 0; --i) {
-            double v = y[i];
-            for (int k = i + 1; k < n; ++k) v -= M[(size_t)k * n + i] * inv[(size_t)col * n + k];
-            inv[(size_t)col * n + i] = v / M[(size_t)i * n + i];
-        }
+        lam[row] = c * (2.0 - 2.0 * std::cos(2.0 * pi * m / n));
+        ++row;
+    };
+    put(0, false, std::sqrt(1.0 / n));
+    for (int m = 1; 2 * m < n; ++m) {
+        put(m, false, std::sqrt(2.0 / n));
+        put(m, true, std::sqrt(2.0 / n));
     }
-    return true;
+    if (n % 2 == 0) put(n / 2, false, std::sqrt(1.0 / n));
 }
+
 
 // Levels of the multigrid preconditioner (level 0 is the SPE operator of the uniform
 // grid), the inner-solve vectors and state, in arithmetic type T.
@@ -434,7 +413,7 @@ template <typename T> static int setup_multigrid(cfdmx_ctx *ctx)
     lev.push_back(L);
     for (;;) {
         const MGLevelT<T> &P = lev.back();
-        if (P.n <= 512) break;
+        if (P.n <= kMGCoarseCells) break;
         const double hmin = std::min(h[0], std::min(h[1], h[2]));
         const int fx = (P.nx % 2 == 0 && P.nx >= 4 && h[0] <= 2.0 * hmin) ? 2 : 1;
         const int fy = (P.ny % 2 == 0 && P.ny >= 4 && h[1] <= 2.0 * hmin) ? 2 : 1;
@@ -463,21 +442,28 @@ template <typename T> static int setup_multigrid(cfdmx_ctx *ctx)
         lev.push_back(C);
     }
     MGLevelT<T> &B = lev.back();
-    if (B.n > 512)
-        FAIL(CFDMX_ERR_ARG, "multigrid: the hierarchy of %dx%dx%d stops at %d cells (the "
-                            "coarse solve takes at most 512); use sizes with more factors of "
-                            "two", g.nxc, g.nyc, g.nzc, B.n);
-    if (lev.size() < 2)
-        FAIL(CFDMX_ERR_ARG, "multigrid: the grid %dx%dx%d needs at least one coarse level",
-             g.nxc, g.nyc, g.nzc);
-    std::vector<double> inv;
-    if (!coarse_inverse(B.nx, B.ny, B.nz, cd[0], cd[1], cd[2], B.zper, inv))
-        FAIL(CFDMX_ERR_ARG, "multigrid: the coarsest operator is not positive definite");
-    std::vector<T> invT(inv.begin(), inv.end());
-    T *dm = nullptr;
-    ALLOC(dm, invT.size());
-    HIPCHK(hipMemcpy(dm, invT.data(), invT.size() * sizeof(T), hipMemcpyHostToDevice));
-    B.minv = dm;
+    if (B.n > kMGCoarseCells || B.nx > kMGCoarseAxis || B.ny > kMGCoarseAxis ||
+        B.nz > kMGCoarseAxis)
+        FAIL(CFDMX_ERR_ARG, "multigrid: the hierarchy of %dx%dx%d stops at %dx%dx%d (the exact "
+                            "coarse solve takes at most %d cells and %d per axis); use sizes "
+                            "with more factors of two",
+             g.nxc, g.nyc, g.nzc, B.nx, B.ny, B.nz, kMGCoarseCells, kMGCoarseAxis);
+    // the coarsest level is solved in natural order (a single level is then bound to the
+    // Krylov vectors directly); its operator is the Kronecker sum of three 1-D chains
+    B.split = 0;
+    {
+        std::vector<double> Qx, Qy, Qz, lx, ly, lz;
+        axis_eigen(B.nx, (double)B.cx, false, Qx, lx);
+        axis_eigen(B.ny, (double)B.cy, false, Qy, ly);
+        axis_eigen(B.nz, (double)B.cz, B.zper != 0, Qz, lz);
+        std::vector<T> pack;
+        for (const auto *v : {&Qx, &Qy, &Qz, &lx, &ly, &lz})
+            for (double d : *v) pack.push_back((T)d);
+        T *dm = nullptr;
+        ALLOC(dm, pack.size());
+        HIPCHK(hipMemcpy(dm, pack.data(), pack.size() * sizeof(T), hipMemcpyHostToDevice));
+        B.fd = dm;
+    }
     ALLOC(ctx->pbs, 1);
     ALLOC(ctx->pb_part, 2 * (size_t)g.nblocks + 64);
     ALLOC(ctx->mix, 4);

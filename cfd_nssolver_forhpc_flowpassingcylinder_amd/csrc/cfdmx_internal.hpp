@@ -95,8 +95,15 @@ struct MGLevelT {
     // Krylov solve (read by the first sweep, written by the last)
     const T *fin;
     T *uout;
-    const T *minv;      // coarsest level: (A + 1 1^T / n)^-1 (symmetric), n x n
+    // coarsest level (stored in natural order): the orthonormal eigenvectors of the 1-D
+    // operators, Qx (nx x nx, row m = mode m), Qy, Qz, then their eigenvalues (nx, ny, nz
+    // values); the level's operator is their Kronecker sum
+    const T *fd;
 };
+
+// the hierarchy stops at the first level of at most this many cells, solved exactly
+constexpr int kMGCoarseCells = 512;
+constexpr int kMGCoarseAxis = 512; // largest axis length of that level
 
 struct PBState {
     double rho, alpha, omega, beta, norm0, res, tol;

@@ -10,8 +10,8 @@
 // at most 512 cells or no even size left; its operator is the same Laplacian on the
 // coarse spacings.  Cycle: V(2,2) with red-black Gauss-Seidel, restriction = mean of the
 // children's residuals, prolongation = the parent's value; the coarsest level is solved
-// exactly by a dense (A + 1 1^T / n)^-1 computed on the host, applied to f minus its
-// mean.  Every inner product accumulates in double.
+// exactly by fast diagonalisation (the 1-D eigenbases of its three axes, computed on the
+// host).  Every inner product accumulates in double.
 //
 // A level with even nz stores u and f by colour (red cells, then black cells), and its
 // kernels take one thread per z-pair (cells 2kk and 2kk+1 of an (i, j) row), so a half
@@ -363,37 +363,74 @@ __global__ void __launch_bounds__(256) k_mg_restrict(MGLevelT<T> F, MGLevelT<T> 
     C.f[mg_at(C, I, J, K)] = acc / (T)(C.fx * C.fy * C.fz);
 }
 
-// coarsest level: u = M^-1 (f - mean f).  One wave per row: lane l takes the columns
-// c = l + 64 t (M is symmetric, so row r is contiguous and the loads coalesce), the
-// wave folds its lanes; every wave forms the mean of f itself, in double.
-constexpr int kMGCoarseMax = 512;
+// coarsest level: the exact solve by fast diagonalisation.  The level's operator is the
+// Kronecker sum of three 1-D chains with orthonormal eigenvectors Qx, Qy, Qz, so
+// u = Q diag(1 / lambda) Q^T f with Q = Qx (x) Qy (x) Qz and lambda = lx + ly + lz; the
+// constant mode (lambda = 0) is dropped, which returns the mean-free solution of A u = f
+// for the mean-free part of f.  One workgroup: the six 1-D transforms run through LDS,
+// a barrier apart.
+constexpr int kMGCoarseThreads = 1024;
 template <typename T>
-__global__ void __launch_bounds__(256) k_mg_coarse(MGLevelT<T> L)
+__global__ void __launch_bounds__(kMGCoarseThreads) k_mg_coarse(MGLevelT<T> L)
 {
-    __shared__ T g[kMGCoarseMax];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    // natural cell order n; the level may be stored by colour
-    auto at = [&](int n) {
+    extern __shared__ double coarse_lds[];
+    T *a = reinterpret_cast<T *>(coarse_lds), *b = a + L.n;
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    const T *Qx = L.fd, *Qy = Qx + nx * nx, *Qz = Qy + ny * ny;
+    const T *lx = Qz + nz * nz, *ly = lx + nx, *lz = ly + ny;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int n = tid; n < L.n; n += nt) a[n] = L.f[n];
+    __syncthreads();
+    // forward: coefficients along x, then y, then z (divided by the eigenvalue)
+    for (int n = tid; n < L.n; n += nt) {
+        int m, j, k;
+        mg_coords(L, n, m, j, k);
+        T acc = (T)0;
+        for (int i = 0; i < nx; ++i) acc += Qx[m * nx + i] * a[(i * ny + j) * nz + k];
+        b[n] = acc;
+    }
+    __syncthreads();
+    for (int n = tid; n < L.n; n += nt) {
+        int mx, m, k;
+        mg_coords(L, n, mx, m, k);
+        T acc = (T)0;
+        for (int j = 0; j < ny; ++j) acc += Qy[m * ny + j] * b[(mx * ny + j) * nz + k];
+        a[n] = acc;
+    }
+    __syncthreads();
+    for (int n = tid; n < L.n; n += nt) {
+        int mx, my, m;
+        mg_coords(L, n, mx, my, m);
+        T acc = (T)0;
+        for (int k = 0; k < nz; ++k) acc += Qz[m * nz + k] * a[(mx * ny + my) * nz + k];
+        const T lam = lx[mx] + ly[my] + lz[m];
+        b[n] = (mx == 0 && my == 0 && m == 0) ? (T)0 : acc / lam;
+    }
+    __syncthreads();
+    // inverse: back along z, y, x
+    for (int n = tid; n < L.n; n += nt) {
+        int mx, my, k;
+        mg_coords(L, n, mx, my, k);
+        T acc = (T)0;
+        for (int m = 0; m < nz; ++m) acc += Qz[m * nz + k] * b[(mx * ny + my) * nz + m];
+        a[n] = acc;
+    }
+    __syncthreads();
+    for (int n = tid; n < L.n; n += nt) {
+        int mx, j, k;
+        mg_coords(L, n, mx, j, k);
+        T acc = (T)0;
+        for (int m = 0; m < ny; ++m) acc += Qy[m * ny + j] * a[(mx * ny + m) * nz + k];
+        b[n] = acc;
+    }
+    __syncthreads();
+    for (int n = tid; n < L.n; n += nt) {
         int i, j, k;
         mg_coords(L, n, i, j, k);
-        return mg_at(L, i, j, k);
-    };
-    double part = 0.0;
-    for (int n = lane; n < L.n; n += 64) part += (double)L.f[at(n)];
-    const double mean = __shfl(wave_fold(part), 0, 64) / (double)L.n;
-    for (int n = threadIdx.x; n < L.n; n += 256) g[n] = (T)((double)L.f[at(n)] - mean);
-    __syncthreads();
-    const int r = blockIdx.x * 4 + wave;
-    if (r >= L.n) return;
-    const T *row = L.minv + (size_t)r * L.n;
-    T acc = (T)0;
-#pragma unroll
-    for (int t = 0; t < kMGCoarseMax / 64; ++t) {
-        const int c = lane + 64 * t;
-        if (c < L.n) acc += row[c] * g[c];
+        T acc = (T)0;
+        for (int m = 0; m < nx; ++m) acc += Qx[m * nx + i] * b[(m * ny + j) * nz + k];
+        L.u[n] = acc;
     }
-    for (int sft = 32; sft >= 1; sft >>= 1) acc += __shfl_down(acc, sft, 64);
-    if (lane == 0) L.u[at(r)] = acc;
 }
 
 // ---- preconditioned BiCGSTAB pieces (linear cell order, canonical block sums) ----
@@ -535,7 +572,8 @@ void launch_mg_vcycle(hipStream_t s, const MGLevelT<T> *L, int nlev, int l)
 {
     const MGLevelT<T> &F = L[l];
     if (l == nlev - 1) {
-        hipLaunchKernelGGL(k_mg_coarse<T>, dim3(cdiv(F.n, 4)), dim3(256), 0, s, F);
+        hipLaunchKernelGGL(k_mg_coarse<T>, dim3(1), dim3(kMGCoarseThreads),
+                           2 * (size_t)F.n * sizeof(T), s, F);
         return;
     }
     const MGLevelT<T> &C = L[l + 1];

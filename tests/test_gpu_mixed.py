@@ -111,9 +111,9 @@ def test_mixed_configuration_limits(precision):
 
 
 def test_odd_nz_levels_take_the_per_cell_kernels():
-    """nz = 5 (lz chosen so that all spacings are equal): the levels 48x24x5 and 24x12x5
-    have odd nz, so the V-cycle runs its one-thread-per-cell smoother and prolongation
-    (coarse level 12x6x5); the solve must still reach the FP64 tolerance."""
+    """nz = 5 (lz chosen so that all spacings are equal): level 48x24x5 has odd nz, so the
+    V-cycle runs its one-thread-per-cell smoother and prolongation (coarsest level
+    12x6x5, solved exactly); the solve must still reach the FP64 tolerance."""
     tol = 1e-10
     gpu, sim, o, g = make_pair(48, 24, 5, lengths=(20.0, 10.0, 20.0 * 5 / 48), bc="cylinder",
                                tol=tol, precision=PRECISION_MIXED)
@@ -175,3 +175,27 @@ def test_graph_replay_matches_launched_cycles(precision, n, lz):
         for a, b in zip(ua, ub):
             assert np.array_equal(a, b)
         assert np.array_equal(xa, xb)
+
+
+@pytest.mark.parametrize("precision,max_inner", [(PRECISION_FP64_MG, 2), (PRECISION_MIXED, 6)])
+@pytest.mark.parametrize("n,lengths", [((16, 8, 4), (20.0, 10.0, 5.0)),
+                                       ((9, 7, 5), (9.0, 7.0, 5.0))])
+def test_single_level_is_the_exact_solve(precision, max_inner, n, lengths):
+    """A grid of at most 512 cells is its own coarsest level, which the V-cycle solves
+    exactly by fast diagonalisation (cosine modes in x and y, Fourier modes in periodic z,
+    even and odd sizes).  The preconditioned BiCGSTAB then needs one iteration per
+    refinement pass, and the FP64 residual, recomputed with the oracle's operator, meets
+    the tolerance."""
+    tol = 1e-10
+    gpu, sim, o, g = make_pair(*n, lengths=lengths, bc="cylinder", tol=tol, precision=precision)
+    gpu.convdiff()
+    gpu.pressure_rhs()
+    o.convdiff(g, 0.002, 1.0 / 100.0, sim.field("u0"), sim.field("v0"), sim.field("w0"),
+               sim.field("u1"), sim.field("v1"), sim.field("w1"), z_periodic=1)
+    b = o.rhs(g, 0.002, sim.field("u1"), sim.field("v1"), sim.field("w1"), 0, 1)
+    it, res = gpu.solve_bicgstab()
+    r = b - o.apply_A(g, gpu.xresult(), 0, 1)
+    rel = np.linalg.norm(r) / np.linalg.norm(b)
+    print(f"{n}: {it} inner iterations, residual {rel:.3e}")
+    assert rel < tol and it <= max_inner, (it, rel)
+    gpu.close()
