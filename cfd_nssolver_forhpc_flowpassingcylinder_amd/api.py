@@ -34,7 +34,8 @@ EXPORTS = [
     "cfdmx_apply_operator", "cfdmx_step", "cfdmx_synchronize", "cfdmx_profile",
     "cfdmx_profile_read", "cfdmx_set_tolerance", "cfdmx_set_physics", "cfdmx_upload_force",
     "cfdmx_drag_objective", "cfdmx_adjoint_drag", "cfdmx_plot3d_q", "cfdmx_plot3d_flush",
-    "cfdmx_plot3d_x", "cfdmx_checkpoint_save", "cfdmx_checkpoint_load",
+    "cfdmx_plot3d_x", "cfdmx_checkpoint_save", "cfdmx_checkpoint_load", "cfdmx_smagorinsky",
+    "cfdmx_download_viseff",
 ]
 
 # kernel classes of cfdmx_profile_read
@@ -63,6 +64,7 @@ class Config(C.Structure):
         ("solver", C.c_int), ("precond", C.c_int), ("tol", C.c_double), ("maxit", C.c_int),
         ("sor_omega", C.c_double), ("skip_first_pressure", C.c_int), ("precision", C.c_int),
         ("inner_tol", C.c_double), ("xcd_remap", C.c_int), ("mg_graphs", C.c_int),
+        ("smagorinsky_cs", C.c_double),
     ]
 
 
@@ -108,6 +110,8 @@ def load_library(path=LIB_PATH):
     L.cfdmx_plot3d_flush.argtypes = [vp]
     L.cfdmx_plot3d_x.argtypes = [vp, C.c_char_p]
     L.cfdmx_checkpoint_save.argtypes = [vp, C.c_char_p]
+    L.cfdmx_smagorinsky.argtypes = [vp]
+    L.cfdmx_download_viseff.argtypes = [vp, DP]
     L.cfdmx_checkpoint_load.argtypes = [vp, C.c_char_p]
     L.cfdmx_set_eta_cylinder_z.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_int]
     L.cfdmx_set_eta_smoothed.argtypes = [vp] + [C.c_double] * 4
@@ -257,6 +261,13 @@ class FlowSolver:
 
     def plot3d_x(self, path):
         self._chk(self.L.cfdmx_plot3d_x(self.h, os.fsencode(path)), "plot3d_x")
+
+    def smagorinsky(self):
+        """Eddy viscosity from T0 (SmagorinskyModel); needs smagorinsky_cs > 0."""
+        self._chk(self.L.cfdmx_smagorinsky(self.h), "smagorinsky")
+
+    def viseff(self):
+        return self._get("cfdmx_download_viseff", self.N)
 
     def checkpoint_save(self, path):
         """FP64 state of the run (fields, X_result history, step, time, solver state)."""

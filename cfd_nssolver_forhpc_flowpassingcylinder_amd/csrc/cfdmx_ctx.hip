@@ -79,6 +79,9 @@ struct cfdmx_ctx {
         double *apart = nullptr, *out = nullptr, *out_h = nullptr;
     } adj;
     double *part = nullptr, *cfl_part = nullptr, *flux_part = nullptr;
+    // Smagorinsky model (cfg.smagorinsky_cs > 0): eddy viscosity per interior cell and its
+    // factor (Cs delta)^2
+    double *viseff = nullptr, *smag_coef = nullptr;
     double *bpart[3] = {}; // per-phase BiCGSTAB partials (A, B, C)
     double *scal = nullptr; // [0] cfl, [1] cd, [2] cl
     double *inflow = nullptr;
@@ -595,6 +598,24 @@ static int setup_device(cfdmx_ctx *ctx)
     ALLOC(ctx->B.ap[1], N);
     ALLOC(ctx->part, 4 * (size_t)(g.nblocks > g.nmarch ? g.nblocks : g.nmarch) + 64);
     ALLOC(ctx->cfl_part, (size_t)g.cd_nb + 8);
+    if (ctx->cfg.smagorinsky_cs > 0.0) {
+        // (Cs delta)^2 with delta = pow(Dx Dy Dz, 1/3) (SmagorinskyModel.hpp:74-76), on the
+        // host with the C library's pow
+        std::vector<double> coef(N);
+        for (int i = 0; i < g.nxc; ++i)
+            for (int j = 0; j < g.nyc; ++j)
+                for (int k = 0; k < g.nzc; ++k) {
+                    const double delta = std::pow(ctx->Dx[i + kGC] * ctx->Dy[j + kGC] *
+                                                      ctx->Dz[k + kGC],
+                                                  1.0 / 3.0);
+                    const double cd = ctx->cfg.smagorinsky_cs * delta;
+                    coef[((size_t)i * g.nyc + j) * g.nzc + k] = cd * cd;
+                }
+        ALLOC(ctx->smag_coef, N);
+        ALLOC(ctx->viseff, N);
+        HIPCHK(hipMemcpy(ctx->smag_coef, coef.data(), N * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemset(ctx->viseff, 0, N * sizeof(double)));
+    }
     for (int q = 0; q < 3; ++q) ALLOC(ctx->bpart[q], 2 * (size_t)g.nmarch + 64);
     ALLOC(ctx->flux_part, 3 * ((size_t)g.cyz / kPerBlock + 2));
     ALLOC(ctx->scal, 16);
@@ -670,6 +691,8 @@ static int validate(cfdmx_ctx *ctx)
     if (c.precond != CFDMX_PRECOND_NONE && c.precond != CFDMX_PRECOND_JACOBI)
         FAIL(CFDMX_ERR_ARG, "unknown precond %d", c.precond);
     if (c.maxit < 0) FAIL(CFDMX_ERR_ARG, "maxit must be >= 0");
+    if (!(c.smagorinsky_cs >= 0.0 && c.smagorinsky_cs < 1.0))
+        FAIL(CFDMX_ERR_ARG, "smagorinsky_cs must be in [0, 1) (got %g)", c.smagorinsky_cs);
     if (c.precision != CFDMX_PRECISION_FP64 && c.precision != CFDMX_PRECISION_MIXED &&
         c.precision != CFDMX_PRECISION_FP64_MG)
         FAIL(CFDMX_ERR_ARG, "unknown precision %d", c.precision);
@@ -732,6 +755,7 @@ int cfdmx_config_default(cfdmx_config *cfg)
     cfg->inner_tol = 1e-5;
     cfg->xcd_remap = 1;
     cfg->mg_graphs = 1;
+    cfg->smagorinsky_cs = 0.0;
     return CFDMX_OK;
 }
 
@@ -1017,12 +1041,32 @@ int cfdmx_set_tolerance(cfdmx_ctx *ctx, double tol, int maxit)
     return CFDMX_OK;
 }
 
+int cfdmx_smagorinsky(cfdmx_ctx *ctx)
+{
+    if (!ctx) return CFDMX_ERR_ARG;
+    if (!ctx->viseff)
+        FAIL(CFDMX_ERR_STATE, "the Smagorinsky model is off (smagorinsky_cs = 0)");
+    launch_smagorinsky(ctx->g, ctx->stream, ctx->t0[0], ctx->t0[1], ctx->t0[2], ctx->smag_coef,
+                       ctx->viseff);
+    LAUNCHCHK();
+    return CFDMX_OK;
+}
+
+int cfdmx_download_viseff(cfdmx_ctx *ctx, double *viseff)
+{
+    if (!ctx) return CFDMX_ERR_ARG;
+    if (!ctx->viseff)
+        FAIL(CFDMX_ERR_STATE, "the Smagorinsky model is off (smagorinsky_cs = 0)");
+    return copy_d2h(ctx, viseff, ctx->viseff, ctx->g.N);
+}
+
 int cfdmx_convdiff(cfdmx_ctx *ctx)
 {
     if (!ctx) return CFDMX_ERR_ARG;
     PROF(CFDMX_K_CONVDIFF,
          launch_convdiff(ctx->g, ctx->stream, ctx->t0[0], ctx->t0[1], ctx->t0[2], ctx->t1[0],
-                         ctx->t1[1], ctx->t1[2], ctx->cfg.dt, 1.0 / ctx->cfg.re, ctx->cfl_part));
+                         ctx->t1[1], ctx->t1[2], ctx->cfg.dt, 1.0 / ctx->cfg.re, ctx->cfl_part,
+                         ctx->viseff));
     launch_max_fin(ctx->g, ctx->stream, ctx->cfl_part, ctx->scal + 0);
     LAUNCHCHK();
     return CFDMX_OK;
@@ -1352,7 +1396,10 @@ static int step_once(cfdmx_ctx *ctx, int *iters, double *err, int *solved)
     hipStream_t s = ctx->stream;
     const Geo &g = ctx->g;
     const cfdmx_config &c = ctx->cfg;
-    int rc = cfdmx_convdiff(ctx);
+    int rc = CFDMX_OK;
+    // the eddy viscosity from T0 (RUN_CPU.cpp:261-263)
+    if (ctx->viseff && (rc = cfdmx_smagorinsky(ctx))) return rc;
+    rc = cfdmx_convdiff(ctx);
     if (rc) return rc;
     *iters = 0;
     *err = 0.0;
@@ -1799,6 +1846,9 @@ static int adj_check(cfdmx_ctx *ctx, int nsteps, int K)
         c.solver != CFDMX_SOLVER_BICGSTAB)
         FAIL(CFDMX_ERR_ARG, "the drag adjoint needs the uniform grid, the SPE operator "
                             "(precond none) and BiCGSTAB");
+    if (c.smagorinsky_cs > 0.0)
+        FAIL(CFDMX_ERR_ARG, "the drag adjoint does not differentiate the Smagorinsky eddy "
+                            "viscosity (smagorinsky_cs must be 0)");
     if (!ctx->started) FAIL(CFDMX_ERR_STATE, "call cfdmx_init_flow or cfdmx_start first");
     return CFDMX_OK;
 }

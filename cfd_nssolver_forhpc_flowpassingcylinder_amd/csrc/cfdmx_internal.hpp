@@ -53,7 +53,9 @@ using BicgBuffers = BicgBuf<double>;
 
 void launch_convdiff(const Geo &g, hipStream_t s, const double *u, const double *v,
                      const double *w, double *us, double *vs, double *ws, double dt,
-                     double nu, double *cfl_part);
+                     double nu, double *cfl_part, const double *viseff = nullptr);
+void launch_smagorinsky(const Geo &g, hipStream_t s, const double *u, const double *v,
+                        const double *w, const double *coef, double *viseff);
 void launch_bc(const Geo &g, hipStream_t s, const BCParams &bc, int advance, double *u,
                double *v, double *w, double *p, double *u1, double *v1, double *w1,
                double *flux_part);

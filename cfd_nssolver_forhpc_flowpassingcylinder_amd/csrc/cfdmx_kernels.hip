@@ -135,12 +135,12 @@ __host__ __device__ inline size_t cd_lds_bytes(const Geo &g)
            sizeof(double);
 }
 
-template <int NE>
+template <int NE, bool LES>
 __global__ void __launch_bounds__(kThreads)
 k_convdiff(Geo g, const double *__restrict__ u, const double *__restrict__ v,
            const double *__restrict__ w, double *__restrict__ us, double *__restrict__ vs,
            double *__restrict__ ws, double mdt, double nudt, double dt,
-           double *__restrict__ cfl_part)
+           double *__restrict__ cfl_part, double nu, const double *__restrict__ viseff)
 {
     const int b = logical_block(g);
     if (b >= g.cd_nb) return;
@@ -236,6 +236,11 @@ k_convdiff(Geo g, const double *__restrict__ u, const double *__restrict__ v,
         if (own) {
             const int ic = gidx(g, I, J, K);
             double cfl = 0.0;
+            // the diffusion's nu*dt; with the Smagorinsky model nu = 1/Re + Viseff of the
+            // cell (Convection_Difussion.hpp:251-255)
+            const double nudt_c =
+                LES ? (nu + viseff[((I - kGC) * g.nyc + (J - kGC)) * g.nzc + (K - kGC)]) * dt
+                    : nudt;
             // ---------------- u (:205-273) ----------------
             if (I < g.nx - 3) {
                 const double IT = X.IT[I];
@@ -259,7 +264,7 @@ k_convdiff(Geo g, const double *__restrict__ u, const double *__restrict__ v,
                            divr(uPy * Vp - uNy * Vn, Y.D[J], Y.rD[J]) +
                            divr(uPz * Wp - uNz * Wn, Z.D[K], Z.rD[K]));
                 // ((u[xp]-u)/Dx[i+1] - (u-u[xm])/Dx[i]) = ax1 - ax0, etc.
-                const double difussion = nudt * (divr(ax1 - ax0, X.Ds[I], X.rDs[I]) +
+                const double difussion = nudt_c * (divr(ax1 - ax0, X.Ds[I], X.rDs[I]) +
                                                  divr(ay1 - ay0, Y.D[J], Y.rD[J]) +
                                                  divr(az1 - az0, Z.D[K], Z.rD[K]));
                 const double r = uc[c] + convection + difussion;
@@ -288,7 +293,7 @@ k_convdiff(Geo g, const double *__restrict__ u, const double *__restrict__ v,
                     mdt * (divr(Up * vPx - Un * vNx, X.D[I], X.rD[I]) +
                            divr(vPy * vPy - vNy * vNy, Y.Ds[J], Y.rDs[J]) +
                            divr(Wp * vPz - Wn * vNz, Z.D[K], Z.rD[K]));
-                const double difussion = nudt * (divr(ax1 - ax0, X.D[I], X.rD[I]) +
+                const double difussion = nudt_c * (divr(ax1 - ax0, X.D[I], X.rD[I]) +
                                                  divr(ay1 - ay0, Y.Ds[J], Y.rDs[J]) +
                                                  divr(az1 - az0, Z.D[K], Z.rD[K]));
                 const double r = vc[c] + convection + difussion;
@@ -317,7 +322,7 @@ k_convdiff(Geo g, const double *__restrict__ u, const double *__restrict__ v,
                     mdt * (divr(Up * wPx - Un * wNx, X.D[I], X.rD[I]) +
                            divr(Vp * wPy - Vn * wNy, Y.D[J], Y.rD[J]) +
                            divr(wPz * wPz - wNz * wNz, Z.Ds[K], Z.rDs[K]));
-                const double difussion = nudt * (divr(ax1 - ax0, X.D[I], X.rD[I]) +
+                const double difussion = nudt_c * (divr(ax1 - ax0, X.D[I], X.rD[I]) +
                                                  divr(ay1 - ay0, Y.D[J], Y.rD[J]) +
                                                  divr(az1 - az0, Z.Ds[K], Z.rDs[K]));
                 const double r = wc[c] + convection + difussion;
@@ -366,6 +371,40 @@ k_convdiff(Geo g, const double *__restrict__ u, const double *__restrict__ v,
         for (int q = 1; q < 4; ++q) m = red[q] > m ? red[q] : m;
         cfl_part[b] = m * dt;
     }
+}
+
+// ---------------------------------------------------------------------------
+// SmagorinskyModel (SmagorinskyModel.hpp:30-79, GradientGauss :6-27): per interior cell,
+// central differences 0.5/Dm * (phi[m] - phi[p]) of u over (Dxs, Dy, Dz), v over
+// (Dx, Dys, Dz) and w over (Dx, Dy, Dzs); temp = |2 ux^2 + vy^2 + wz^2 + (uy + vx)^2 +
+// (uz + wx)^2 + (vz + wy)^2| (the factor 2 on the first term only, as written there);
+// Viseff = coef * sqrt(temp) with coef = (Cs delta)^2, delta = pow(Dx Dy Dz, 1/3),
+// tabulated on the host.  One thread per cell, icelCal order.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kThreads)
+k_smagorinsky(Geo g, const double *__restrict__ u, const double *__restrict__ v,
+              const double *__restrict__ w, const double *__restrict__ coef,
+              double *__restrict__ viseff)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= g.N) return;
+    int i, j, k;
+    decode(g, n, i, j, k);
+    const int I = i + kGC, J = j + kGC, K = k + kGC;
+    const int c = gidx(g, I, J, K), sx = g.sx, sy = g.sy;
+    const Axis &X = g.ax[0], &Y = g.ax[1], &Z = g.ax[2];
+    const double ux = 0.5 / X.Ds[I] * (u[c - sx] - u[c + sx]);
+    const double uy = 0.5 / Y.D[J] * (u[c - sy] - u[c + sy]);
+    const double uz = 0.5 / Z.D[K] * (u[c - 1] - u[c + 1]);
+    const double vx = 0.5 / X.D[I] * (v[c - sx] - v[c + sx]);
+    const double vy = 0.5 / Y.Ds[J] * (v[c - sy] - v[c + sy]);
+    const double vz = 0.5 / Z.D[K] * (v[c - 1] - v[c + 1]);
+    const double wx = 0.5 / X.D[I] * (w[c - sx] - w[c + sx]);
+    const double wy = 0.5 / Y.D[J] * (w[c - sy] - w[c + sy]);
+    const double wz = 0.5 / Z.Ds[K] * (w[c - 1] - w[c + 1]);
+    const double a = uy + vx, b = uz + wx, d = vz + wy;
+    const double temp = fabs(2.0 * (ux * ux) + vy * vy + wz * wz + a * a + b * b + d * d);
+    viseff[n] = coef[n] * sqrt(temp);
 }
 
 // ---------------------------------------------------------------------------
@@ -1686,18 +1725,35 @@ static inline int grid_blocks(const Geo &g)
 }
 static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 
+void launch_smagorinsky(const Geo &g, hipStream_t s, const double *u, const double *v,
+                        const double *w, const double *coef, double *viseff)
+{
+    hipLaunchKernelGGL(k_smagorinsky, dim3((g.N + kThreads - 1) / kThreads), dim3(kThreads), 0,
+                       s, g, u, v, w, coef, viseff);
+}
+
 void launch_convdiff(const Geo &g, hipStream_t s, const double *u, const double *v,
                      const double *w, double *us, double *vs, double *ws, double dt,
-                     double nu, double *cfl_part)
+                     double nu, double *cfl_part, const double *viseff)
 {
     const int nb = g.xcd_remap ? ((g.cd_nb + 7) / 8) * 8 : g.cd_nb;
     const int P = (g.cd_tj + 4) * (g.cd_tk + 4);
-    if (P <= 2 * kThreads)
-        hipLaunchKernelGGL(k_convdiff<2>, dim3(nb), dim3(kThreads), cd_lds_bytes(g), s, g, u, v,
-                           w, us, vs, ws, -dt, nu * dt, dt, cfl_part);
-    else
-        hipLaunchKernelGGL(k_convdiff<3>, dim3(nb), dim3(kThreads), cd_lds_bytes(g), s, g, u, v,
-                           w, us, vs, ws, -dt, nu * dt, dt, cfl_part);
+    const size_t lds = cd_lds_bytes(g);
+    if (P <= 2 * kThreads) {
+        if (viseff)
+            hipLaunchKernelGGL((k_convdiff<2, true>), dim3(nb), dim3(kThreads), lds, s, g, u, v,
+                               w, us, vs, ws, -dt, nu * dt, dt, cfl_part, nu, viseff);
+        else
+            hipLaunchKernelGGL((k_convdiff<2, false>), dim3(nb), dim3(kThreads), lds, s, g, u, v,
+                               w, us, vs, ws, -dt, nu * dt, dt, cfl_part, nu, viseff);
+    } else {
+        if (viseff)
+            hipLaunchKernelGGL((k_convdiff<3, true>), dim3(nb), dim3(kThreads), lds, s, g, u, v,
+                               w, us, vs, ws, -dt, nu * dt, dt, cfl_part, nu, viseff);
+        else
+            hipLaunchKernelGGL((k_convdiff<3, false>), dim3(nb), dim3(kThreads), lds, s, g, u, v,
+                               w, us, vs, ws, -dt, nu * dt, dt, cfl_part, nu, viseff);
+    }
 }
 
 void launch_bc(const Geo &g, hipStream_t s, const BCParams &bc, int advance, double *u,

@@ -76,6 +76,10 @@ typedef struct cfdmx_config {
     int xcd_remap; /* map consecutive logical blocks to one XCD (0 = hardware order) */
     int mg_graphs; /* multigrid modes: replay each V-cycle from a captured HIP graph
                       (default 1; 0 launches its kernels one by one, same results) */
+    /* Smagorinsky LES (SmagorinskyModel.hpp:38, built with TERBULENCE_SMAGORINSKY): the
+     * constant Cs of the eddy viscosity (Cs delta)^2 |S| that each step adds to 1/Re in the
+     * diffusion; 0 = off (the reference's default build; its constant is 0.18) */
+    double smagorinsky_cs;
 } cfdmx_config;
 
 typedef struct cfdmx_ctx cfdmx_ctx;
@@ -178,6 +182,14 @@ int cfdmx_set_tolerance(cfdmx_ctx *ctx, double tol, int maxit);
 /* ---- stages of one time step (RUN_CPU.cpp:257-436) ---- */
 /* ConvectionDifussion, Convection_Difussion.hpp:174-450: T0 -> T1, QUICK, explicit Euler */
 int cfdmx_convdiff(cfdmx_ctx *ctx);
+/* SmagorinskyModel, SmagorinskyModel.hpp:30-79 (GradientGauss :6-27): the eddy viscosity
+ * Viseff of every interior cell from T0, which cfdmx_convdiff then adds to 1/Re
+ * (Convection_Difussion.hpp:251-255,334-338,414-418).  Requires smagorinsky_cs > 0;
+ * cfdmx_step runs it at the start of each step (RUN_CPU.cpp:261-263).  The reference's
+ * source does not compile (SmagorinskyModel.hpp:61) and is switched off; this computes
+ * the formulas it states. */
+int cfdmx_smagorinsky(cfdmx_ctx *ctx);
+int cfdmx_download_viseff(cfdmx_ctx *ctx, double *viseff); /* N values, icelCal order */
 /* BC_staggered_main, Boundary_Condition.hpp:37-176, on T0 and p */
 int cfdmx_bc_main(cfdmx_ctx *ctx);
 /* BC_staggered_copy, updateBC.hpp:6-110: ghosts T0 -> T1 */

@@ -51,6 +51,9 @@ inline cfdmx_config &config()
     static cfdmx_config c = [] {
         cfdmx_config x;
         cfdmx_config_default(&x);
+#ifdef TERBULENCE_SMAGORINSKY
+        x.smagorinsky_cs = 0.18; // SmagorinskyModel.hpp:38
+#endif
         return x;
     }();
     return c;
@@ -131,6 +134,20 @@ bool ConvectionDifussion(Simu &simu, Vel &oldVel, Vel &curtVel, const Lo &, Grid
     check(cfdmx_convdiff(c), "ConvectionDifussion");
     get_vel(1, curtVel);
     return true;
+}
+
+// SmagorinskyModel, SmagorinskyModel.hpp:30-37 (called at RUN_CPU.cpp:261-263 when the
+// reference is built with TERBULENCE_SMAGORINSKY): Viseff of TX from its velocity.  The
+// context must have been created with config().smagorinsky_cs > 0 (the reference's
+// constant is 0.18); the following ConvectionDifussion adds Viseff to 1/Re.
+template <class Share, class Simu, class Vel, class Pre, class Lo, class Grid>
+void SmagorinskyModel(Share &, Simu &simu, Vel &TX, Pre &, Lo &, Grid &gA)
+{
+    cfdmx_ctx *c = context(gA, simu);
+    put_vel(0, TX);
+    check(cfdmx_smagorinsky(c), "SmagorinskyModel");
+    fit(TX.Viseff, session().N);
+    check(cfdmx_download_viseff(c, TX.Viseff.data()), "download Viseff");
 }
 
 // BC_staggered_main, Boundary_Condition.hpp:37-42

@@ -291,6 +291,15 @@ void orc_convdiff(const orc_grid *g, double dt, double nu, int z_periodic,
                   const double *u, const double *v, const double *w,
                   double *us, double *vs, double *ws)
 {
+    orc_convdiff_les(g, dt, nu, NULL, z_periodic, u, v, w, us, vs, ws);
+}
+
+/* with TERBULENCE_SMAGORINSKY the diffusion uses nu = simu.nu + Viseff[icelCal(i,j,k)] at
+ * each face (Convection_Difussion.hpp:251-255,334-338,414-418) */
+void orc_convdiff_les(const orc_grid *g, double dt, double nu0, const double *viseff,
+                      int z_periodic, const double *u, const double *v, const double *w,
+                      double *us, double *vs, double *ws)
+{
     const int nx = g->nx, ny = g->ny, nz = g->nz;
     const long sx = (long)ny * nz, sy = nz, sz = 1;
     const double *Dx = g->Dx, *Dy = g->Dy, *Dz = g->Dz;
@@ -320,6 +329,7 @@ void orc_convdiff(const orc_grid *g, double dt, double nu, int z_periodic,
                 const double convection = -dt * ((uPx * uPx - uNx * uNx) / Dxs[i] +
                                                  (uPy * Vp - uNy * Vn) / Dy[j] +
                                                  (uPz * Wp - uNz * Wn) / Dz[k]);
+                const double nu = viseff ? nu0 + viseff[ICAL(g, i, j, k)] : nu0;
                 const double difussion =
                     nu * dt *
                     (((u[xp] - u[ic]) / Dx[i + 1] - (u[ic] - u[xm]) / Dx[i]) / Dxs[i] +
@@ -352,6 +362,7 @@ void orc_convdiff(const orc_grid *g, double dt, double nu, int z_periodic,
                 const double convection = -dt * ((Up * vPx - Un * vNx) / Dx[i] +
                                                  (vPy * vPy - vNy * vNy) / Dys[j] +
                                                  (Wp * vPz - Wn * vNz) / Dz[k]);
+                const double nu = viseff ? nu0 + viseff[ICAL(g, i, j, k)] : nu0;
                 const double difussion =
                     nu * dt *
                     (((v[xp] - v[ic]) / Dxs[i] - (v[ic] - v[xm]) / Dxs[i - 1]) / Dx[i] +
@@ -385,6 +396,7 @@ void orc_convdiff(const orc_grid *g, double dt, double nu, int z_periodic,
                 const double convection = -dt * ((Up * wPx - Un * wNx) / Dx[i] +
                                                  (Vp * wPy - Vn * wNy) / Dy[j] +
                                                  (wPz * wPz - wNz * wNz) / Dzs[k]);
+                const double nu = viseff ? nu0 + viseff[ICAL(g, i, j, k)] : nu0;
                 const double difussion =
                     nu * dt *
                     (((w[xp] - w[ic]) / Dxs[i] - (w[ic] - w[xm]) / Dxs[i - 1]) / Dx[i] +
@@ -399,6 +411,47 @@ void orc_convdiff(const orc_grid *g, double dt, double nu, int z_periodic,
         periodic_z_fill(g, vs, 0);
         periodic_z_fill(g, ws, 0);
     }
+}
+
+/* SmagorinskyModel.hpp:30-79 with GradientGauss :6-27.  The reference's source does not
+ * compile (`auot`, :61) and is disabled (TERBULENCE_SMAGORINSKY undefined); this restates
+ * its formulas.  Per interior cell: central differences 0.5/Dm * (phi[m] - phi[p]) of u
+ * over (Dxs, Dy, Dz), v over (Dx, Dys, Dz) and w over (Dx, Dy, Dzs); temp = |2 ux^2 + vy^2
+ * + wz^2 + (uy + vx)^2 + (uz + wx)^2 + (vz + wy)^2| (the factor 2 on the first term only,
+ * as written); Viseff = (Cs delta)^2 sqrt(temp), delta = pow(Dx Dy Dz, 1/3). */
+void orc_smagorinsky_coef(const orc_grid *g, double cs, double *coef)
+{
+    for (int i = GC; i < g->nx - GC; ++i)
+        for (int j = GC; j < g->ny - GC; ++j)
+            for (int k = GC; k < g->nz - GC; ++k) {
+                const double delta = pow(g->Dx[i] * g->Dy[j] * g->Dz[k], 1.0 / 3.0);
+                const double cd = cs * delta;
+                coef[ICAL(g, i, j, k)] = cd * cd;
+            }
+}
+
+void orc_smagorinsky(const orc_grid *g, const double *coef, const double *u, const double *v,
+                     const double *w, double *viseff)
+{
+    const long sx = (long)g->ny * g->nz, sy = g->nz;
+    for (int i = GC; i < g->nx - GC; ++i)
+        for (int j = GC; j < g->ny - GC; ++j)
+            for (int k = GC; k < g->nz - GC; ++k) {
+                const long c = IC(g, i, j, k);
+                const double ux = 0.5 / g->Dxs[i] * (u[c - sx] - u[c + sx]);
+                const double uy = 0.5 / g->Dy[j] * (u[c - sy] - u[c + sy]);
+                const double uz = 0.5 / g->Dz[k] * (u[c - 1] - u[c + 1]);
+                const double vx = 0.5 / g->Dx[i] * (v[c - sx] - v[c + sx]);
+                const double vy = 0.5 / g->Dys[j] * (v[c - sy] - v[c + sy]);
+                const double vz = 0.5 / g->Dz[k] * (v[c - 1] - v[c + 1]);
+                const double wx = 0.5 / g->Dx[i] * (w[c - sx] - w[c + sx]);
+                const double wy = 0.5 / g->Dy[j] * (w[c - sy] - w[c + sy]);
+                const double wz = 0.5 / g->Dzs[k] * (w[c - 1] - w[c + 1]);
+                const double a = uy + vx, b = uz + wx, d = vz + wy;
+                const double temp = fabs(2.0 * (ux * ux) + vy * vy + wz * wz + a * a + b * b + d * d);
+                const long n = ICAL(g, i, j, k);
+                viseff[n] = coef[n] * sqrt(temp);
+            }
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1166,6 +1219,7 @@ struct orc_sim {
     orc_bc bc;
     orc_params prm;
     double *f0[3], *f1[3], *f3[3], *p, *eta, *f, *x, *b;
+    double *viseff, *smag_coef; /* Smagorinsky (prm.smag_cs > 0) */
     orc_bicg *bicg;
     int loop;
     double time;
@@ -1190,6 +1244,11 @@ orc_sim *orc_sim_new(const orc_grid *g, const orc_bc *bc, const orc_params *prm)
     s->x = (double *)calloc((size_t)n, sizeof(double));
     s->b = (double *)calloc((size_t)n, sizeof(double));
     s->bicg = orc_bicg_new(n);
+    if (prm->smag_cs > 0.0) {
+        s->viseff = (double *)calloc((size_t)n, sizeof(double));
+        s->smag_coef = (double *)calloc((size_t)n, sizeof(double));
+        orc_smagorinsky_coef(g, prm->smag_cs, s->smag_coef);
+    }
     s->loop = 1;
     s->time = 0.0;
     return s;
@@ -1204,6 +1263,7 @@ void orc_sim_free(orc_sim *s)
         free(s->f3[c]);
     }
     free(s->p); free(s->eta); free(s->f); free(s->x); free(s->b);
+    free(s->viseff); free(s->smag_coef);
     orc_bicg_free(s->bicg);
     free(s);
 }
@@ -1243,8 +1303,12 @@ int orc_sim_step(orc_sim *s, double out[8])
     const long G = (long)g->nx * g->ny * g->nz;
     int iters = 0, solved = 0;
     double err = 0.0;
+    /* 0. eddy viscosity from T0 (RUN_CPU.cpp:261-263) */
+    if (s->viseff)
+        orc_smagorinsky(g, s->smag_coef, s->f0[0], s->f0[1], s->f0[2], s->viseff);
     /* 1. u* (RUN_CPU.cpp:268) */
-    orc_convdiff(g, P->dt, P->nu, zp, s->f0[0], s->f0[1], s->f0[2], s->f1[0], s->f1[1], s->f1[2]);
+    orc_convdiff_les(g, P->dt, P->nu, s->viseff, zp, s->f0[0], s->f0[1], s->f0[2], s->f1[0],
+                     s->f1[1], s->f1[2]);
     const double cfl = orc_cfl(g, P->dt, zp, s->f1[0], s->f1[1], s->f1[2]);
     /* 2. pressure (RUN_CPU.cpp:285-322) */
     if (s->loop > 1 || !P->skip_first_pressure) {

@@ -75,6 +75,7 @@ typedef struct orc_params {
     int reduction;
     int skip_first_pressure; /* RUN_CPU.cpp:303: no pressure solve on step 1 */
     int swap_buffers;        /* 0: copyVel T3->T0 (copyVel.hpp:4-12); 1: pointer swap */
+    double smag_cs;          /* Smagorinsky constant (SmagorinskyModel.hpp:38: 0.18); 0: off */
 } orc_params;
 
 /* ---- grid (structureGrid.hpp) ---- */
@@ -89,6 +90,15 @@ double orc_serial_sum(const double *v, long n);
 void orc_convdiff(const orc_grid *g, double dt, double nu, int z_periodic,
                   const double *u, const double *v, const double *w,
                   double *us, double *vs, double *ws);
+/* the same with the eddy viscosity viseff (per interior cell, icelCal order) added to nu */
+void orc_convdiff_les(const orc_grid *g, double dt, double nu, const double *viseff,
+                      int z_periodic, const double *u, const double *v, const double *w,
+                      double *us, double *vs, double *ws);
+/* SmagorinskyModel (SmagorinskyModel.hpp:6-79): coef[icelCal] = (cs delta)^2 with
+ * delta = cbrt-by-pow(Dx Dy Dz), then viseff = coef * sqrt(|strain sum|) */
+void orc_smagorinsky_coef(const orc_grid *g, double cs, double *coef);
+void orc_smagorinsky(const orc_grid *g, const double *coef, const double *u, const double *v,
+                     const double *w, double *viseff);
 void orc_bc_main(const orc_grid *g, const orc_bc *bc, int advance_outflow, double dt,
                  double *u, double *v, double *w, double *p);
 void orc_bc_copy(const orc_grid *g, const double *u0, const double *v0, const double *w0,

@@ -17,7 +17,8 @@ BC_PRESETS = {
 def make_pair(nxc, nyc, nzc, *, lengths=(20.0, 10.0, 0.5), kind=0, re=100.0, dt=0.002,
               solver=0, precond=0, tol=1e-6, maxit=5000, omega=1.8, bc="flowpassing",
               seed=20261202, amp=1e-3, cyl=(5.0, 5.0, 0.5), xcd_remap=1, smoothed_eps=None,
-              skip_first=1, inflow=None, precision=0, inner_tol=1e-5, mg_graphs=1):
+              skip_first=1, inflow=None, precision=0, inner_tol=1e-5, mg_graphs=1,
+              smag_cs=0.0):
     num, dirr, zper, conv, flux = BC_PRESETS[bc]
     cfg = pkg.default_config(nx=nxc, ny=nyc, nz=nzc, lx=lengths[0], ly=lengths[1],
                              lz=lengths[2], grid_kind=kind, re=re, dt=dt, solver=solver,
@@ -25,7 +26,8 @@ def make_pair(nxc, nyc, nzc, *, lengths=(20.0, 10.0, 0.5), kind=0, re=100.0, dt=
                              bc_num=num, bc_dir=dirr, z_periodic=zper, outflow_convective=conv,
                              convective_speed=1.0, flux_correction=flux,
                              skip_first_pressure=skip_first, xcd_remap=xcd_remap,
-                             precision=precision, inner_tol=inner_tol, mg_graphs=mg_graphs)
+                             precision=precision, inner_tol=inner_tol, mg_graphs=mg_graphs,
+                             smagorinsky_cs=smag_cs)
     gpu = pkg.FlowSolver(cfg)
     o = op.Oracle()
     g = o.grid(nxc, nyc, nzc, *lengths, kind=kind)
@@ -33,7 +35,8 @@ def make_pair(nxc, nyc, nzc, *, lengths=(20.0, 10.0, 0.5), kind=0, re=100.0, dt=
                      conv_speed=1.0, flux_correction=flux)
     osolver = {0: op.SOLVER_BICGSTAB, 1: op.SOLVER_RBSOR}[solver]
     sim = op.Sim(o, g, obc, dt, re, solver=osolver, precond=precond, tol=tol, maxit=maxit,
-                 omega=omega, reduction=op.RED_BLOCKED, skip_first=skip_first, swap=1)
+                 omega=omega, reduction=op.RED_BLOCKED, skip_first=skip_first, swap=1,
+                 smag_cs=smag_cs)
     if inflow is not None:
         gpu.set_inflow(inflow)
     gpu.init_flow(1.0, amp, seed)

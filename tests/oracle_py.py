@@ -45,7 +45,8 @@ class OrcParams(C.Structure):
     _fields_ = [("dt", C.c_double), ("nu", C.c_double), ("solver", C.c_int),
                 ("precond", C.c_int), ("tol", C.c_double), ("maxit", C.c_int),
                 ("omega", C.c_double), ("reduction", C.c_int),
-                ("skip_first_pressure", C.c_int), ("swap_buffers", C.c_int)]
+                ("skip_first_pressure", C.c_int), ("swap_buffers", C.c_int),
+                ("smag_cs", C.c_double)]
 
 
 # Num/Dir table of the reference (FLOWpassing.hpp:6-40): inlet u = 1 Dirichlet with
@@ -116,6 +117,9 @@ class Oracle:
         L.orc_serial_sum.argtypes = [DP, C.c_long]
         G = C.POINTER(OrcGrid)
         L.orc_convdiff.argtypes = [G, C.c_double, C.c_double, C.c_int] + [DP] * 6
+        L.orc_convdiff_les.argtypes = [G, C.c_double, C.c_double, DP, C.c_int] + [DP] * 6
+        L.orc_smagorinsky_coef.argtypes = [G, C.c_double, DP]
+        L.orc_smagorinsky.argtypes = [G] + [DP] * 5
         L.orc_bc_main.argtypes = [G, C.POINTER(OrcBC), C.c_int, C.c_double] + [DP] * 4
         L.orc_bc_copy.argtypes = [G] + [DP] * 6
         L.orc_rhs.argtypes = [G, C.c_double, C.c_int, C.c_int] + [DP] * 4
@@ -187,6 +191,17 @@ class Oracle:
     # ---- stage wrappers (numpy in / numpy out) ----
     def convdiff(self, g, dt, nu, u, v, w, us, vs, ws, z_periodic=0):
         self.lib.orc_convdiff(C.byref(g.c), dt, nu, z_periodic, *map(ptr, (u, v, w, us, vs, ws)))
+
+    def convdiff_les(self, g, dt, nu, viseff, u, v, w, us, vs, ws, z_periodic=0):
+        self.lib.orc_convdiff_les(C.byref(g.c), dt, nu, ptr(viseff), z_periodic,
+                                  *map(ptr, (u, v, w, us, vs, ws)))
+
+    def smagorinsky(self, g, cs, u, v, w):
+        """Eddy viscosity per interior cell (icelCal order) from T0."""
+        coef, vis = np.zeros(g.N), np.zeros(g.N)
+        self.lib.orc_smagorinsky_coef(C.byref(g.c), cs, ptr(coef))
+        self.lib.orc_smagorinsky(C.byref(g.c), ptr(coef), *map(ptr, (u, v, w)), ptr(vis))
+        return vis
 
     def bc_main(self, g, bc, u, v, w, p, advance=0, dt=0.0):
         self.lib.orc_bc_main(C.byref(g.c), C.byref(bc), advance, dt, *map(ptr, (u, v, w, p)))
@@ -289,10 +304,11 @@ class Sim:
               "w3": 8, "p": 9, "eta": 10, "f": 11, "x": 12, "b": 13}
 
     def __init__(self, oracle, g, bc, dt, re, solver=SOLVER_BICGSTAB, precond=PRECOND_NONE,
-                 tol=1e-3, maxit=5000, omega=1.8, reduction=RED_SERIAL, skip_first=1, swap=0):
+                 tol=1e-3, maxit=5000, omega=1.8, reduction=RED_SERIAL, skip_first=1, swap=0,
+                 smag_cs=0.0):
         self.o, self.g, self.bc = oracle, g, bc
         self.prm = OrcParams(dt, 1.0 / re, solver, precond, tol, maxit, omega, reduction,
-                             skip_first, swap)
+                             skip_first, swap, smag_cs)
         self.h = oracle.lib.orc_sim_new(C.byref(g.c), C.byref(bc), C.byref(self.prm))
 
     def field(self, name):
