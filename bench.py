@@ -63,7 +63,7 @@ WORKLOADS = {
 }
 
 
-def make_solver(wl, precond, solver, seed, maxit=5000, precision=0):
+def make_solver(wl, precond, solver, seed, maxit=5000, precision=0, smag_cs=0.0):
     import cfd_nssolver_forhpc_flowpassingcylinder_amd as pkg
     from cfd_nssolver_forhpc_flowpassingcylinder_amd.api import (PRECOND_JACOBI, PRECOND_NONE,
                                                                   SOLVER_BICGSTAB, SOLVER_RBSOR)
@@ -76,7 +76,8 @@ def make_solver(wl, precond, solver, seed, maxit=5000, precision=0):
         solver=SOLVER_RBSOR if solver == "rbsor" else SOLVER_BICGSTAB, sor_omega=1.8,
         bc_num=slip_num, bc_dir=slip_dir, z_periodic=1, outflow_convective=1,
         convective_speed=1.0, flux_correction=1, skip_first_pressure=1,
-        precision=precision, inner_tol=MIXED_INNER_TOL, mg_graphs=MG_GRAPHS)
+        precision=precision, inner_tol=MIXED_INNER_TOL, mg_graphs=MG_GRAPHS,
+        smagorinsky_cs=smag_cs)
     s = pkg.FlowSolver(cfg)
     s.init_flow(1.0, 1e-3, seed)
     s.set_eta_cylinder_z(5.0, 5.0, 0.5, 100)
@@ -522,6 +523,32 @@ def plot3d_run(wl, args):
                      "file write to a temporary directory")}
 
 
+def les_run(wl, args):
+    """The mixed-precision step at the metric's grid with the Smagorinsky model off and on
+    (Cs 0.18, the reference's constant): step time, and the eddy viscosity against 1/Re."""
+    import numpy as np
+    out = {}
+    for name, cs in (("off", 0.0), ("on", 0.18)):
+        s = make_solver(wl, args.precond, args.solver, args.seed, args.maxit, 1, smag_cs=cs)
+        for _ in range(3):
+            s.step(1)
+        s.synchronize()
+        n = 5
+        t = time.perf_counter()
+        for _ in range(n):
+            s.step(1)
+        s.synchronize()
+        out[name] = {"step_ms": round((time.perf_counter() - t) / n * 1e3, 3)}
+        if cs > 0:
+            vis = s.viseff()
+            out[name]["viseff_max_over_nu"] = float(vis.max() * wl["re"])
+            out[name]["viseff_mean_over_nu"] = float(vis.mean() * wl["re"])
+        s.close()
+    out["note"] = ("mixed precision, 5 timed steps after 3; 'on' adds the eddy-viscosity "
+                   "kernel and the per-cell viscosity in conv-diff")
+    return out
+
+
 def kernel_table(prof, n_cells, bytes_per_cell):
     return {k: {"ms_total": round(v[0], 3), "launches": v[1],
                 "avg_us": round(v[0] * 1e3 / v[1], 2) if v[1] else None,
@@ -550,7 +577,7 @@ def main():
                     help="BiCGSTAB iteration cap (5000, the reference default; lower only for "
                          "counter-collection passes, which need few launches)")
     ap.add_argument("--part", default="all",
-                    choices=["all", "mixed", "fp64mg", "adjoint", "large", "plot3d"],
+                    choices=["all", "mixed", "fp64mg", "adjoint", "large", "plot3d", "les"],
                     help="run only the mixed-precision or only the adjoint measurement and "
                          "print its JSON object (for profiling one part)")
     ap.add_argument("--large-steps", type=int, default=0,
@@ -592,6 +619,9 @@ def main():
         return
     if args.part == "plot3d":
         print(json.dumps({"plot3d": plot3d_run(wl, args)}), flush=True)
+        return
+    if args.part == "les":
+        print(json.dumps({"les": les_run(wl, args)}), flush=True)
         return
     if args.part == "large":
         print(json.dumps({"large": large_run(args.seed, args.large_steps or None,
