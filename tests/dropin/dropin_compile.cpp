@@ -7,6 +7,15 @@
 
 #include "cfdmx_reference_api.hpp"
 
+// the Smagorinsky stage as the reference calls it under TERBULENCE_SMAGORINSKY
+// (RUN_CPU.cpp:261-263)
+void dropin_les(grid &gA, simpulationVariable &simu, velocity &T0, pressure &t1)
+{
+    shareMenory ShareM;
+    divideLocal Lo;
+    cfdmx_ref::SmagorinskyModel(ShareM, simu, T0, t1, Lo, gA);
+}
+
 bool dropin_loop(grid &gA, clockstruct &timer, simpulationVariable &simu, int argc, char **argv)
 {
     DfibArray Dfib;
