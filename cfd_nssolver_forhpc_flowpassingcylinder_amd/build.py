@@ -18,6 +18,7 @@ ARCH = os.environ.get("CFDMX_ARCH", "gfx950")
 
 FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-ffp-contract=off", "-fPIC", "-shared",
+    "-fvisibility=hidden",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,
 ]

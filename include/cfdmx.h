@@ -24,6 +24,9 @@
 #ifndef CFDMX_H
 #define CFDMX_H
 
+/* the library is built with hidden visibility; only these entry points are exported */
+#define CFDMX_API __attribute__((visibility("default")))
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -98,35 +101,35 @@ typedef struct cfdmx_step_info {
 
 /* Defaults: FLOWpassing table, reference solver defaults (tol 1e-3, 5000 iterations,
  * omega 1.8), uniform 20 x 10 x 0.5 domain. */
-int cfdmx_config_default(cfdmx_config *cfg);
+CFDMX_API int cfdmx_config_default(cfdmx_config *cfg);
 
 /* Context lifecycle; replaces the allocations of runSeqOmp (RUN_CPU.cpp:18-28,64,120,
  * 158-163) and generateGride (Generator.hpp:14-46).  Builds the grid metrics. */
-int cfdmx_create(const cfdmx_config *cfg, cfdmx_ctx **out);
-int cfdmx_destroy(cfdmx_ctx *ctx);
-const char *cfdmx_last_error(const cfdmx_ctx *ctx); /* ctx may be NULL */
-int cfdmx_sizes(const cfdmx_ctx *ctx, long long *G, long long *N);
+CFDMX_API int cfdmx_create(const cfdmx_config *cfg, cfdmx_ctx **out);
+CFDMX_API int cfdmx_destroy(cfdmx_ctx *ctx);
+CFDMX_API const char *cfdmx_last_error(const cfdmx_ctx *ctx); /* ctx may be NULL */
+CFDMX_API int cfdmx_sizes(const cfdmx_ctx *ctx, long long *G, long long *N);
 
 /* Grid metrics as built by the reference generators: X,Y,Z (n+1 each), then
  * Dx,Dy,Dz,Dxs,Dys,Dzs (n each).  structureGrid.hpp:93,336-367,433-486. */
-int cfdmx_grid_metrics(const cfdmx_ctx *ctx, double *out);
+CFDMX_API int cfdmx_grid_metrics(const cfdmx_ctx *ctx, double *out);
 
 /* Field movement.  which: 0 = T0 (u^n), 1 = T1 (u*), 2 = T3 (u^{n+1}).
  * Pressure p is ghosted (pressure::p), x is X_result (MxClass::X_result). */
-int cfdmx_upload_velocity(cfdmx_ctx *ctx, int which, const double *u, const double *v,
+CFDMX_API int cfdmx_upload_velocity(cfdmx_ctx *ctx, int which, const double *u, const double *v,
                           const double *w);
-int cfdmx_download_velocity(cfdmx_ctx *ctx, int which, double *u, double *v, double *w);
-int cfdmx_upload_pressure(cfdmx_ctx *ctx, const double *p);
-int cfdmx_download_pressure(cfdmx_ctx *ctx, double *p);
-int cfdmx_upload_xresult(cfdmx_ctx *ctx, const double *x);
-int cfdmx_download_xresult(cfdmx_ctx *ctx, double *x);
-int cfdmx_download_rhs(cfdmx_ctx *ctx, double *b);
-int cfdmx_upload_rhs(cfdmx_ctx *ctx, const double *b); /* MxClass::matB, N doubles */
-int cfdmx_upload_eta(cfdmx_ctx *ctx, const double *eta);
-int cfdmx_download_eta(cfdmx_ctx *ctx, double *eta);
-int cfdmx_download_force(cfdmx_ctx *ctx, double *f); /* DfibArray::f, 3*G */
-int cfdmx_upload_force(cfdmx_ctx *ctx, const double *f);
-int cfdmx_set_inflow(cfdmx_ctx *ctx, const double *profile); /* u inflow per ghosted j, ny */
+CFDMX_API int cfdmx_download_velocity(cfdmx_ctx *ctx, int which, double *u, double *v, double *w);
+CFDMX_API int cfdmx_upload_pressure(cfdmx_ctx *ctx, const double *p);
+CFDMX_API int cfdmx_download_pressure(cfdmx_ctx *ctx, double *p);
+CFDMX_API int cfdmx_upload_xresult(cfdmx_ctx *ctx, const double *x);
+CFDMX_API int cfdmx_download_xresult(cfdmx_ctx *ctx, double *x);
+CFDMX_API int cfdmx_download_rhs(cfdmx_ctx *ctx, double *b);
+CFDMX_API int cfdmx_upload_rhs(cfdmx_ctx *ctx, const double *b); /* MxClass::matB, N doubles */
+CFDMX_API int cfdmx_upload_eta(cfdmx_ctx *ctx, const double *eta);
+CFDMX_API int cfdmx_download_eta(cfdmx_ctx *ctx, double *eta);
+CFDMX_API int cfdmx_download_force(cfdmx_ctx *ctx, double *f); /* DfibArray::f, 3*G */
+CFDMX_API int cfdmx_upload_force(cfdmx_ctx *ctx, const double *f);
+CFDMX_API int cfdmx_set_inflow(cfdmx_ctx *ctx, const double *profile); /* u inflow per ghosted j, ny */
 
 /* Plot3D output: OutputPLOT3D_Qfile (qfile.hpp:6-108) and OutputPlot3D_Xfile
  * (xfileWrite.hpp:6-64), single-block files in the reference's layout.
@@ -142,9 +145,9 @@ int cfdmx_set_inflow(cfdmx_ctx *ctx, const double *profile); /* u inflow per gho
  * x file: int32 1, nx, ny, nz; the cell-centre coordinates Xc, Yc, Zc
  * (structureGrid.hpp:216-229) as float32 arrays with i fastest.  A file that cannot be
  * written returns CFDMX_ERR_ARG (for q, from the call that writes it). */
-int cfdmx_plot3d_q(cfdmx_ctx *ctx, const char *path, double time);
-int cfdmx_plot3d_flush(cfdmx_ctx *ctx);
-int cfdmx_plot3d_x(cfdmx_ctx *ctx, const char *path);
+CFDMX_API int cfdmx_plot3d_q(cfdmx_ctx *ctx, const char *path, double time);
+CFDMX_API int cfdmx_plot3d_flush(cfdmx_ctx *ctx);
+CFDMX_API int cfdmx_plot3d_x(cfdmx_ctx *ctx, const char *path);
 
 /* FP64 checkpoint / resume.  No reference counterpart: it stands where the reference
  * would resume from a float32 q file (QfileRead, qfileRead.hpp:6-100, called only from
@@ -155,73 +158,73 @@ int cfdmx_plot3d_x(cfdmx_ctx *ctx, const char *path);
  * with the same configuration (size, grid kind, domain, z periodicity: CFDMX_ERR_ARG
  * otherwise), after which steps continue bit for bit as in the saved run.  Both
  * synchronise the context's stream. */
-int cfdmx_checkpoint_save(cfdmx_ctx *ctx, const char *path);
-int cfdmx_checkpoint_load(cfdmx_ctx *ctx, const char *path);
+CFDMX_API int cfdmx_checkpoint_save(cfdmx_ctx *ctx, const char *path);
+CFDMX_API int cfdmx_checkpoint_load(cfdmx_ctx *ctx, const char *path);
 
 /* Seeded initial field on the device: u = u0 + amp*(2U-1) on interior cells with U
  * from splitmix64(seed ^ interior index), v = w = 0, p = 0; then the initial
  * BC_staggered_main + BC_staggered_copy (RUN_CPU.cpp:66-109). */
-int cfdmx_init_flow(cfdmx_ctx *ctx, double u0, double amp, unsigned long long seed);
+CFDMX_API int cfdmx_init_flow(cfdmx_ctx *ctx, double u0, double amp, unsigned long long seed);
 /* Initial T1 = T3 = T0 copy + boundary conditions (RUN_CPU.cpp:104-109) for fields
  * uploaded by the caller. */
-int cfdmx_start(cfdmx_ctx *ctx);
+CFDMX_API int cfdmx_start(cfdmx_ctx *ctx);
 
 /* Immersed-boundary mask.  DFIB_CylinderZ setEta.hpp:136-204 (centre, radius and
  * sub-cell count parameters; the reference fixes 6.5, ly/2, 0.5, 100). */
-int cfdmx_set_eta_cylinder_z(cfdmx_ctx *ctx, double cx, double cy, double radius, int nsub);
+CFDMX_API int cfdmx_set_eta_cylinder_z(cfdmx_ctx *ctx, double cx, double cy, double radius, int nsub);
 /* Smoothed mask of half-width eps for differentiation (no reference counterpart). */
-int cfdmx_set_eta_smoothed(cfdmx_ctx *ctx, double cx, double cy, double radius, double eps);
+CFDMX_API int cfdmx_set_eta_smoothed(cfdmx_ctx *ctx, double cx, double cy, double radius, double eps);
 
 /* simpulationVariable::set_Re / set_dt (0_1_SimulationVariable.hpp:26-31,95-106) after
  * creation */
-int cfdmx_set_physics(cfdmx_ctx *ctx, double re, double dt);
+CFDMX_API int cfdmx_set_physics(cfdmx_ctx *ctx, double re, double dt);
 /* bicgstabRe2::setTolerance (bicgstabRe2.hpp:38) and the iteration limit
  * (iters_max_, :47); also the SOR tolerance/limit (simu.p_criteria, p_sor_iter_max) */
-int cfdmx_set_tolerance(cfdmx_ctx *ctx, double tol, int maxit);
+CFDMX_API int cfdmx_set_tolerance(cfdmx_ctx *ctx, double tol, int maxit);
 
 /* ---- stages of one time step (RUN_CPU.cpp:257-436) ---- */
 /* ConvectionDifussion, Convection_Difussion.hpp:174-450: T0 -> T1, QUICK, explicit Euler */
-int cfdmx_convdiff(cfdmx_ctx *ctx);
+CFDMX_API int cfdmx_convdiff(cfdmx_ctx *ctx);
 /* SmagorinskyModel, SmagorinskyModel.hpp:30-79 (GradientGauss :6-27): the eddy viscosity
  * Viseff of every interior cell from T0, which cfdmx_convdiff then adds to 1/Re
  * (Convection_Difussion.hpp:251-255,334-338,414-418).  Requires smagorinsky_cs > 0;
  * cfdmx_step runs it at the start of each step (RUN_CPU.cpp:261-263).  The reference's
  * source does not compile (SmagorinskyModel.hpp:61) and is switched off; this computes
  * the formulas it states. */
-int cfdmx_smagorinsky(cfdmx_ctx *ctx);
-int cfdmx_download_viseff(cfdmx_ctx *ctx, double *viseff); /* N values, icelCal order */
+CFDMX_API int cfdmx_smagorinsky(cfdmx_ctx *ctx);
+CFDMX_API int cfdmx_download_viseff(cfdmx_ctx *ctx, double *viseff); /* N values, icelCal order */
 /* BC_staggered_main, Boundary_Condition.hpp:37-176, on T0 and p */
-int cfdmx_bc_main(cfdmx_ctx *ctx);
+CFDMX_API int cfdmx_bc_main(cfdmx_ctx *ctx);
 /* BC_staggered_copy, updateBC.hpp:6-110: ghosts T0 -> T1 */
-int cfdmx_bc_copy(cfdmx_ctx *ctx);
+CFDMX_API int cfdmx_bc_copy(cfdmx_ctx *ctx);
 /* createBMatrix_omp, b_matrix.hpp:35-98: b from T1 (kept on the device) */
-int cfdmx_pressure_rhs(cfdmx_ctx *ctx);
+CFDMX_API int cfdmx_pressure_rhs(cfdmx_ctx *ctx);
 /* bicgstabRe2::solve, bicgstabRe2.hpp:82-180, on the device b and X_result; returns
  * iterations and relative residual like std::pair<int,double> */
-int cfdmx_solve_bicgstab(cfdmx_ctx *ctx, int *iters, double *residual);
+CFDMX_API int cfdmx_solve_bicgstab(cfdmx_ctx *ctx, int *iters, double *residual);
 /* Red-black SOR with SorPipeLine_seq's coefficients and update
  * (SorPipeLine.seq.hpp:31-123) on p, RHS from T1; stop on max |dp| < tol */
-int cfdmx_solve_rbsor(cfdmx_ctx *ctx, int *iters, double *err);
+CFDMX_API int cfdmx_solve_rbsor(cfdmx_ctx *ctx, int *iters, double *err);
 /* Pressure_transform_X_result, Transform.hpp:8-25 */
-int cfdmx_transform(cfdmx_ctx *ctx);
+CFDMX_API int cfdmx_transform(cfdmx_ctx *ctx);
 /* update_UandF_omp, updateUandF.hpp:90-180: T1, p, eta -> T3, f */
-int cfdmx_project_ib(cfdmx_ctx *ctx);
+CFDMX_API int cfdmx_project_ib(cfdmx_ctx *ctx);
 /* copyVel(T3, T0), copyVel.hpp:4-12 (a device pointer swap) */
-int cfdmx_swap_t0_t3(cfdmx_ctx *ctx);
+CFDMX_API int cfdmx_swap_t0_t3(cfdmx_ctx *ctx);
 /* CD_CL, virtualForceIntergrator.hpp:92-111 */
-int cfdmx_cd_cl(cfdmx_ctx *ctx, double *cd, double *cl);
+CFDMX_API int cfdmx_cd_cl(cfdmx_ctx *ctx, double *cd, double *cl);
 /* get_Cfl, cfl.hpp:3-79, on T1 */
-int cfdmx_cfl(cfdmx_ctx *ctx, double *cfl);
+CFDMX_API int cfdmx_cfl(cfdmx_ctx *ctx, double *cfl);
 
 /* Pressure operator on host vectors of N doubles: SPE_matrix::multiply_omp
  * (SPE_sparseMatrix.hpp:92-106) or CSR_matrix::multiply_omp (CSR_sparseMatrix.hpp:231-245). */
-int cfdmx_apply_operator(cfdmx_ctx *ctx, const double *x, double *r);
+CFDMX_API int cfdmx_apply_operator(cfdmx_ctx *ctx, const double *x, double *r);
 
 /* n whole time steps (the loop body of runSeqOmp, RUN_CPU.cpp:257-436, without file
  * output).  info (may be NULL) receives the last step's report. */
-int cfdmx_step(cfdmx_ctx *ctx, int nsteps, cfdmx_step_info *info);
+CFDMX_API int cfdmx_step(cfdmx_ctx *ctx, int nsteps, cfdmx_step_info *info);
 
-int cfdmx_synchronize(cfdmx_ctx *ctx);
+CFDMX_API int cfdmx_synchronize(cfdmx_ctx *ctx);
 
 /* ---- adjoint of the time-averaged drag (no reference counterpart: the reference
  * has no adjoint).  Callers: gradient-based design and its checks (the PyTorch
@@ -238,13 +241,13 @@ typedef struct cfdmx_adjoint_info {
 } cfdmx_adjoint_info;
 
 /* J only, by the same forward path. */
-int cfdmx_drag_objective(cfdmx_ctx *ctx, int nsteps, double *J);
+CFDMX_API int cfdmx_drag_objective(cfdmx_ctx *ctx, int nsteps, double *J);
 /* J and dJ/d(inflow profile) (grad_inflow: ny ghosted entries, zero where the profile is
  * not read; all zero without cfdmx_set_inflow) and dJ/d(cx, cy, R) of the smoothed mask
  * set by cfdmx_set_eta_smoothed (grad_mask[3]); either may be NULL.  checkpoint_every
  * = K bounds the tape to K steps (memory: about (4G + N) * 8 bytes per tape step and
  * (4G + 3N) * 8 per checkpoint, plus 63 G * 8 for the conv-diff rows). */
-int cfdmx_adjoint_drag(cfdmx_ctx *ctx, int nsteps, int checkpoint_every, double *J,
+CFDMX_API int cfdmx_adjoint_drag(cfdmx_ctx *ctx, int nsteps, int checkpoint_every, double *J,
                        double *grad_inflow, double *grad_mask, cfdmx_adjoint_info *info);
 
 /* ---- measurement (no reference counterpart: the reference times phases with
@@ -264,9 +267,9 @@ int cfdmx_adjoint_drag(cfdmx_ctx *ctx, int nsteps, int checkpoint_every, double 
 #define CFDMX_K_COUNT 12
 /* Enable (1) or disable (0) HIP-event timing of every launch of the kernels above on
  * the context's stream; resets the totals. */
-int cfdmx_profile(cfdmx_ctx *ctx, int enable);
+CFDMX_API int cfdmx_profile(cfdmx_ctx *ctx, int enable);
 /* Accumulated device time and launch count of one kernel class. */
-int cfdmx_profile_read(cfdmx_ctx *ctx, int kernel, double *total_ms, long long *launches);
+CFDMX_API int cfdmx_profile_read(cfdmx_ctx *ctx, int kernel, double *total_ms, long long *launches);
 
 #ifdef __cplusplus
 }
