@@ -237,3 +237,19 @@ def test_full_size_convdiff_rhs_and_bicgstab_bitwise():
     assert it_g == it_o and _same(res_g, res_o), (it_g, it_o, res_g, res_o)
     assert np.array_equal(gpu.xresult(), xo, equal_nan=True)
     gpu.close()
+
+
+def test_c1_configuration_steps_bitwise():
+    """The CPU-runnable configuration, 128x64x4 over 20x10x0.5 with the cylinder at (5, 5),
+    Re 100, dt 0.002, BiCGSTAB with the Jacobi rows at tol 1e-8: ten whole steps (about 800
+    iterations each, with rho restarts) against the oracle's time loop, every field and
+    reported value bit for bit."""
+    gpu, sim, o, g = make_pair(128, 64, 4, bc="cylinder", precond=1, tol=1e-8)
+    for s in range(10):
+        a = gpu.step(1)
+        b = sim.step()
+        for key in ("iters", "err", "cd", "cl", "cfl", "time"):
+            assert _same(a[key], b[key]), (s, key, a, b)
+        assert fields_equal(gpu, sim) == [], s
+    print(f"128x64x4: step 10 took {a['iters']} iterations, Cd {a['cd']:.12f}")
+    gpu.close()
