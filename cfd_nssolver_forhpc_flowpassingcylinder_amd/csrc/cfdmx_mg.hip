@@ -286,8 +286,11 @@ __global__ void __launch_bounds__(256) k_mg_prolong_smooth_z(MGLevelT<T> F, MGLe
     const int kzm = k > 0 ? k - 1 : F.nz - 1, kzp = k < F.nz - 1 ? k + 1 : 0;
     const int kkm = (k > 0) ? kk - 1 + pr : half - 1;
     const int kkp = (k < F.nz - 1) ? kk + pr : 0;
+    // branch-free: the indices are selected, then both values are loaded unconditionally
     auto val = [&](bool ok, int fine, int a, int b, int c) {
-        return ok ? u[hn + fine] + e[mg_at(C, a, b, c >> 1)] : u[m] + e[mg_at(C, i, j, kk)];
+        const int fi = ok ? hn + fine : m;
+        const int ci = mg_at(C, ok ? a : i, ok ? b : j, (ok ? c : k) >> 1);
+        return u[fi] + e[ci];
     };
     T s = (T)0;
     s += (xm ? F.cx : (T)0) * val(xm, m - sxh, i - 1, j, k);
