@@ -374,6 +374,22 @@ def adjoint_run(seed, fd=True):
                                    "the reverse pass re-runs each checkpointed segment"),
         "forward_iters": info["forward_iters"], "adjoint_iters": info["adjoint_iters"],
     }
+    # the same adjoint with the multigrid pressure solves in FP64 (forward, recompute and
+    # pressure adjoint), solved to the same tolerance
+    s, _ = case_solver(a, seed, precision=2, inflow_amp=0.01)
+    t = time.perf_counter()
+    s.drag_objective(n)
+    plain_mg = time.perf_counter() - t
+    Jm, gim, gmm, infom = s.adjoint_drag(n, a["K"])
+    s.close()
+    out["fp64_multigrid"] = {
+        "forward_ms_per_step": round(plain_mg * 1e3 / n, 3),
+        "reverse_ms_per_step": round(infom["reverse_ms"] / n, 3),
+        "adjoint_step_cost": round(infom["reverse_ms"] / (plain_mg * 1e3), 3),
+        "rel_diff_J": abs(Jm - J) / abs(J),
+        "rel_diff_grad_mask": float(np.max(np.abs(gmm - gm)) / np.max(np.abs(gm))),
+        "rel_diff_grad_inflow": float(np.max(np.abs(gim - gi)) / np.max(np.abs(gi))),
+    }
     if fd:
         out["gradient_check"] = gradient_check(a, seed, n, a["K"],
                                                ["R", "cx", "cy", 2 + 20, 2 + 64, 2 + 100],
