@@ -31,6 +31,19 @@ def test_library_exports_every_declared_symbol():
     assert missing == []
 
 
+def test_library_exports_nothing_else():
+    """Built with hidden visibility: the dynamic symbol table holds the header's entry
+    points and no internal function."""
+    import shutil
+    import subprocess
+    if not os.path.exists(pkg.LIB_PATH) or not shutil.which("nm"):
+        pytest.skip("libcfdmx.so not built or nm missing")
+    out = subprocess.run(["nm", "-D", "--defined-only", pkg.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    text_syms = sorted(l.split()[-1] for l in out.splitlines() if l.split()[1:2] == ["T"])
+    assert text_syms == declared_functions()
+
+
 def test_python_binding_covers_header():
     assert sorted(pkg.EXPORTS) == declared_functions()
 
