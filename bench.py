@@ -624,7 +624,7 @@ def main():
         return
 
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not run "
+        print(f"bench.py: the solver runs on one GPU; --gpus {args.gpus} is not run "
               "(multi-GPU x-slab sharding is design-only)", file=sys.stderr)
         sys.exit(2)
 
