@@ -1022,7 +1022,29 @@ __device__ __forceinline__ float keep_if(float v, int ok)
 // plane two ahead (own cells) and of the next plane's halo are loaded one step before
 // they are formed.  Halo elements per thread: NH = ceil(2 nzc / 256).
 // NH = 1: at most 128 VGPRs, so that four blocks fit each CU at once.
-template <int PHASE, int PRE, int NH, bool FULL, typename T>
+//
+// NT: the accesses read or written once per iteration (phase A's dot operand and its
+// p store, phase B's As store, every phase C operand) take the streaming
+// (nontemporal) hint.  The launcher sets it when the iteration's seven vectors exceed
+// the 256 MiB memory-side cache: there the hint leaves more cache to the stencil
+// inputs, which are reread as halos.  The iteration measured 209 -> 188 us at
+// 512x256x64 (470 MB) and 157 -> 149 us at 384x256x64 (352 MB), but 103 -> 108 us at
+// 256x256x64 (235 MB) and 60 -> 66 us at 256x128x32, where the vectors fit the cache
+// and the hint gives up their reuse.  The values are the same either way.
+template <bool NT, typename T>
+__device__ __forceinline__ T ld_stream(const T *p)
+{
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+template <bool NT, typename T>
+__device__ __forceinline__ void st_stream(T v, T *p)
+{
+    if constexpr (NT) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+
+template <int PHASE, int PRE, int NH, bool FULL, bool NT, typename T>
 __global__ void __launch_bounds__(kThreads, NH == 1 ? 4 : 1)
 k_march(Geo g, MarchIO<T> io, KState *st, int m, const double *part_in)
 {
@@ -1106,13 +1128,14 @@ k_march(Geo g, MarchIO<T> io, KState *st, int m, const double *part_in)
         else v = a - alphaT * c2;
         return keep_if(v, ok);
     };
-    // the second operand of phase A's dot: r0, or r on a restart
+    // the second operand of phase A's dot: r0, or r on a restart (read once: NT)
     auto load_r0 = [&](int i, bool want, T (&o)[C]) {
         const T *src = mode == 1 ? io.r : io.r0;
         const int pk = (int)want;
 #pragma unroll
         for (int c = 0; c < C; ++c)
-            o[c] = src[(i * g.cyz + q0 + c * kThreads + (int)threadIdx.x) & -(pk & own_ok(c))];
+            o[c] = ld_stream<NT>(
+                src + ((i * g.cyz + q0 + c * kThreads + (int)threadIdx.x) & -(pk & own_ok(c))));
     };
     int jc[C], kc[C];
 #pragma unroll
@@ -1211,13 +1234,15 @@ k_march(Geo g, MarchIO<T> io, KState *st, int m, const double *part_in)
             const T own = oc[c];
             const double sd = (double)sum;
             if (PHASE == 0) {
-                io.out_p[n] = own;
+                // p is next read by phase C; Ap (read by phase B) keeps the default
+                // policy, which measured faster
+                st_stream<NT>(own, io.out_p + n);
                 io.out_ap[n] = sum;
                 const T r0v = r0c[c];
                 if (mode == 1) io.r0[n] = r0v;
                 acc[0] += sd * (double)r0v;
             } else {
-                io.out_as[n] = sum;
+                st_stream<NT>(sum, io.out_as + n);
                 acc[0] += sd * (double)own;
                 acc[1] += sd * sd;
             }
@@ -1247,7 +1272,7 @@ k_march(Geo g, MarchIO<T> io, KState *st, int m, const double *part_in)
 
 // phase C in the march mapping (elementwise): x, r update (bicgstabRe2.hpp:123-127),
 // the partials of (r,r), (r,r0), and q = p - omega*Ap in place of p for the next phase A
-template <typename T>
+template <bool NT, typename T>
 __global__ void __launch_bounds__(kThreads)
 k_march_C(Geo g, T *__restrict__ r, const T *__restrict__ ap, T *__restrict__ p,
           const T *__restrict__ as, const T *__restrict__ r0, T *__restrict__ x,
@@ -1274,15 +1299,18 @@ k_march_C(Geo g, T *__restrict__ r, const T *__restrict__ ap, T *__restrict__ p,
             const int tq = c * kThreads + threadIdx.x;
             if (tq < qn) {
                 const int n = i * g.cyz + q0 + tq;
-                const T apn = ap[n], pn = p[n];
-                const T sn = r[n] - alpha * apn;
-                x[n] += alpha * pn + om * sn;
-                const T rn = sn - om * as[n];
+                // every operand is read once (NT: streaming loads, see k_march); plain
+                // stores measured faster here with either setting
+                const T apn = ld_stream<NT>(ap + n);
+                const T pn = ld_stream<NT>(p + n);
+                const T sn = ld_stream<NT>(r + n) - alpha * apn;
+                x[n] = ld_stream<NT>(x + n) + (alpha * pn + om * sn);
+                const T rn = sn - om * ld_stream<NT>(as + n);
                 r[n] = rn;
                 p[n] = pn - om * apn;
                 const double rd = (double)rn;
                 acc[0] += rd * rd;
-                acc[1] += rd * (double)r0[n];
+                acc[1] += rd * (double)ld_stream<NT>(r0 + n);
             }
         }
     }
@@ -1823,16 +1851,32 @@ static inline size_t march_lds(const Geo &g)
 
 // halo cells per thread: ceil(2 nzc / 256); 1 covers nzc <= 128, 5 the largest
 // supported nzc (600)
+// the streaming hint (NT in k_march) when the iteration's seven vectors r, q, r0, p,
+// Ap, As, x exceed the 256 MiB memory-side cache
+template <typename T>
+static bool march_streaming(const Geo &g)
+{
+    return (size_t)7 * (size_t)g.N * sizeof(T) > ((size_t)256 << 20);
+}
+template <int PHASE, int NH, bool FULL, bool NT, typename T>
+static void launch_march_nt(const Geo &g, hipStream_t s, int pre, const MarchIO<T> &io,
+                            KState *st, int m, const double *part_in)
+{
+    if (pre)
+        hipLaunchKernelGGL((k_march<PHASE, 1, NH, FULL, NT, T>), dim3(march_blocks(g)),
+                           dim3(kThreads), march_lds<T>(g), s, g, io, st, m, part_in);
+    else
+        hipLaunchKernelGGL((k_march<PHASE, 0, NH, FULL, NT, T>), dim3(march_blocks(g)),
+                           dim3(kThreads), march_lds<T>(g), s, g, io, st, m, part_in);
+}
 template <int PHASE, int NH, bool FULL, typename T>
 static void launch_march_full(const Geo &g, hipStream_t s, int pre, const MarchIO<T> &io,
                               KState *st, int m, const double *part_in)
 {
-    if (pre)
-        hipLaunchKernelGGL((k_march<PHASE, 1, NH, FULL, T>), dim3(march_blocks(g)),
-                           dim3(kThreads), march_lds<T>(g), s, g, io, st, m, part_in);
+    if (march_streaming<T>(g))
+        launch_march_nt<PHASE, NH, FULL, true>(g, s, pre, io, st, m, part_in);
     else
-        hipLaunchKernelGGL((k_march<PHASE, 0, NH, FULL, T>), dim3(march_blocks(g)),
-                           dim3(kThreads), march_lds<T>(g), s, g, io, st, m, part_in);
+        launch_march_nt<PHASE, NH, FULL, false>(g, s, pre, io, st, m, part_in);
 }
 template <int PHASE, int NH, typename T>
 static void launch_march_nh(const Geo &g, hipStream_t s, int pre, const MarchIO<T> &io,
@@ -1882,9 +1926,14 @@ void launch_bicg_C(const Geo &g, hipStream_t s, BicgBuf<T> &B, T *x, double *con
                    KState *st, int m)
 {
     (void)m;
-    hipLaunchKernelGGL(k_march_C<T>, dim3(march_blocks(g)), dim3(kThreads), 0, s, g, B.r,
-                       B.ap[B.cur], B.p[B.cur], B.as, B.r0, x, (const double *)part[1], part[2],
-                       st);
+    if (march_streaming<T>(g))
+        hipLaunchKernelGGL((k_march_C<true, T>), dim3(march_blocks(g)), dim3(kThreads), 0, s, g,
+                           B.r, B.ap[B.cur], B.p[B.cur], B.as, B.r0, x, (const double *)part[1],
+                           part[2], st);
+    else
+        hipLaunchKernelGGL((k_march_C<false, T>), dim3(march_blocks(g)), dim3(kThreads), 0, s,
+                           g, B.r, B.ap[B.cur], B.p[B.cur], B.as, B.r0, x,
+                           (const double *)part[1], part[2], st);
 }
 #define CFDMX_BICG_INSTANCES(T)                                                               \
     template void launch_bicg_A<T>(const Geo &, hipStream_t, int, BicgBuf<T> &,               \
