@@ -1024,7 +1024,7 @@ __device__ __forceinline__ float keep_if(float v, int ok)
 // NH = 1: at most 128 VGPRs, so that four blocks fit each CU at once.
 //
 // NT: the accesses read or written once per iteration (phase A's dot operand and its
-// p store, phase B's As store, every phase C operand) take the streaming
+// p store, phase B's As store, every phase C load and its x store) take the streaming
 // (nontemporal) hint.  The launcher sets it when the iteration's seven vectors exceed
 // the 256 MiB memory-side cache: there the hint leaves more cache to the stencil
 // inputs, which are reread as halos.  The iteration measured 209 -> 188 us at
@@ -1299,12 +1299,14 @@ k_march_C(Geo g, T *__restrict__ r, const T *__restrict__ ap, T *__restrict__ p,
             const int tq = c * kThreads + threadIdx.x;
             if (tq < qn) {
                 const int n = i * g.cyz + q0 + tq;
-                // every operand is read once (NT: streaming loads, see k_march); plain
-                // stores measured faster here with either setting
+                // every operand is read once (NT: streaming loads, see k_march).  x is
+                // next read by the next phase C, so it is stored with the hint too; r and
+                // q, which the next phase A reads at once, keep plain stores (hinted,
+                // they measured slower)
                 const T apn = ld_stream<NT>(ap + n);
                 const T pn = ld_stream<NT>(p + n);
                 const T sn = ld_stream<NT>(r + n) - alpha * apn;
-                x[n] = ld_stream<NT>(x + n) + (alpha * pn + om * sn);
+                st_stream<NT>(ld_stream<NT>(x + n) + (alpha * pn + om * sn), x + n);
                 const T rn = sn - om * ld_stream<NT>(as + n);
                 r[n] = rn;
                 p[n] = pn - om * apn;
