@@ -1545,6 +1545,12 @@ k_project(Geo g, double dt, double rdt, int convective, const double *__restrict
     const Axis &X = g.ax[0], &Y = g.ax[1], &Z = g.ax[2];
     const double uS = 0.0, vS = 0.0, wS = 0.0;
     double acc[2] = {0.0, 0.0};
+    // Every load is unconditional: where a face is skipped (the last interior x or y
+    // plane, the last z plane without periodicity) its neighbour index falls back to the
+    // cell itself and the face's values are computed and dropped.  Only the stores and
+    // the partial sums depend on the face tests, and each stored value is the same
+    // expression as the reference's, so the result is unchanged; the cell's loads are
+    // then all in flight together instead of one face after another.
     for (int c = 0; c < kCells; ++c) {
         const int n = L * kPerBlock + c * kThreads + threadIdx.x;
         if (n >= g.N) break;
@@ -1552,45 +1558,46 @@ k_project(Geo g, double dt, double rdt, int convective, const double *__restrict
         decode(g, n, i, j, k);
         const int I = i + kGC, J = j + kGC, K = k + kGC;
         const int ic = gidx(g, I, J, K);
-        double pc;
-        if (FROM_X) {
-            pc = xres[n];
-            p[ic] = pc;
-        } else {
-            pc = p[ic];
-        }
-        const double ec = eta[ic];
-        double fx = 0.0, fy = 0.0;
-        if (I < g.nx - 3) {
-            const double pe = FROM_X ? xres[n + g.cyz] : p[ic + g.sx];
-            const double T2u = us[ic] - divr(dt * (pe - pc), X.Ds[I], X.rDs[I]);
-            const double e = ec + (eta[ic + g.sx] - ec) * X.IT[I];
-            const double un = ec * uS + (1.0 - e) * T2u;
+        const bool ox = I < g.nx - 3, oy = J < g.ny - 3, oz = K < g.nz - 3 || g.zper;
+        const bool wrap = (K == g.nz - 3);
+        const int icE = ox ? ic + g.sx : ic, icN = oy ? ic + g.sy : ic;
+        const int icP = oz ? (wrap ? ic - (g.nzc - 1) : ic + 1) : ic;
+        const double pc = FROM_X ? xres[n] : p[ic];
+        const double pe = FROM_X ? xres[ox ? n + g.cyz : n] : p[icE];
+        const double pn = FROM_X ? xres[oy ? n + g.nzc : n] : p[icN];
+        const double pt = FROM_X ? xres[oz ? (wrap ? n - (g.nzc - 1) : n + 1) : n] : p[icP];
+        const double ec = eta[ic], ee = eta[icE], en = eta[icN], et = eta[icP];
+        const double usc = us[ic], vsc = vs[ic], wsc = ws[ic];
+        if (FROM_X) p[ic] = pc;
+        // x face
+        const double T2u = usc - divr(dt * (pe - pc), X.Ds[I], X.rDs[I]);
+        const double eu = ec + (ee - ec) * X.IT[I];
+        const double un = ec * uS + (1.0 - eu) * T2u;
+        const double fxv = divr(un - T2u, dt, rdt);
+        if (ox) {
             u[ic] = un;
-            fx = divr(un - T2u, dt, rdt);
-            f[ic] = fx;
+            f[ic] = fxv;
         } else if (convective) {
-            u[ic] = us[ic];
+            u[ic] = usc;
         }
-        if (J < g.ny - 3) {
-            const double pn = FROM_X ? xres[n + g.nzc] : p[ic + g.sy];
-            const double T2v = vs[ic] - divr(dt * (pn - pc), Y.Ds[J], Y.rDs[J]);
-            const double e = ec + (eta[ic + g.sy] - ec) * Y.IT[J];
-            const double vn = ec * vS + (1.0 - e) * T2v;
+        // y face
+        const double T2v = vsc - divr(dt * (pn - pc), Y.Ds[J], Y.rDs[J]);
+        const double ev = ec + (en - ec) * Y.IT[J];
+        const double vn = ec * vS + (1.0 - ev) * T2v;
+        const double fyv = divr(vn - T2v, dt, rdt);
+        if (oy) {
             v[ic] = vn;
-            fy = divr(vn - T2v, dt, rdt);
-            f[ic + g.G] = fy;
+            f[ic + g.G] = fyv;
         }
-        if (K < g.nz - 3 || g.zper) {
-            const bool wrap = (K == g.nz - 3);
-            const int icP = wrap ? ic - (g.nzc - 1) : ic + 1;
-            const double pt = FROM_X ? xres[wrap ? n - (g.nzc - 1) : n + 1] : p[icP];
-            const double T2w = ws[ic] - divr(dt * (pt - pc), Z.Ds[K], Z.rDs[K]);
-            const double e = ec + (eta[icP] - ec) * Z.IT[K];
-            const double wn = ec * wS + (1.0 - e) * T2w;
+        // z face
+        const double T2w = wsc - divr(dt * (pt - pc), Z.Ds[K], Z.rDs[K]);
+        const double ew = ec + (et - ec) * Z.IT[K];
+        const double wn = ec * wS + (1.0 - ew) * T2w;
+        if (oz) {
             w[ic] = wn;
             f[ic + 2 * g.G] = divr(wn - T2w, dt, rdt);
         }
+        const double fx = ox ? fxv : 0.0, fy = oy ? fyv : 0.0;
         const double vol = X.D[I] * Y.D[J] * Z.D[K];
         acc[0] += fx * vol;
         acc[1] += fy * vol;
