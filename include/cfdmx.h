@@ -73,8 +73,10 @@ typedef struct cfdmx_config {
     double sor_omega;
     int skip_first_pressure; /* RUN_CPU.cpp:303: no pressure solve on step 1 */
     int precision;
-    double inner_tol; /* multigrid modes: relative tolerance of each inner solve
-                         (default 1e-5) */
+    double inner_tol; /* multigrid modes: relative tolerance of each inner solve;
+                         0 (the default) takes the mode's fastest measured value,
+                         3e-3 for MIXED and 1e-5 for FP64_MG; the FP64 refinement
+                         still stops only at tol */
     /* launch shaping */
     int xcd_remap; /* map consecutive logical blocks to one XCD (0 = hardware order) */
     int mg_graphs; /* multigrid modes: replay each V-cycle from a captured HIP graph

@@ -59,4 +59,4 @@ def test_config_struct_matches_header_layout():
     assert cfg.tol == 1e-3 and cfg.maxit == 5000 and cfg.sor_omega == 1.8
     assert [list(r) for r in cfg.bc_dir][0] == [1.0, 0.0, 0.0]
     assert cfg.skip_first_pressure == 1 and cfg.xcd_remap == 1
-    assert cfg.inner_tol == 1e-5
+    assert cfg.inner_tol == 0.0

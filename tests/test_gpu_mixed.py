@@ -59,13 +59,15 @@ def test_mixed_solve_reaches_fp64_residual(n, precision):
 
 # consistent pressure systems only (flux-corrected outlet, periodic z): at tol 1e-10 an
 # inconsistent system ends at the iteration limit in either precision
+# inner_tol 0: the mode's default (3e-3 mixed, 1e-5 FP64 multigrid)
+@pytest.mark.parametrize("inner_tol", [1e-5, 3e-3, 0.0])
 @pytest.mark.parametrize("precision", PRECISIONS)
 @pytest.mark.parametrize("kind,n,precond,steps", [(0, (48, 24, 8), 0, 6),
                                                   (0, (64, 32, 16), 0, 4)])
-def test_mixed_time_steps_against_fp64_oracle(kind, n, precond, steps, precision):
+def test_mixed_time_steps_against_fp64_oracle(kind, n, precond, steps, precision, inner_tol):
     tol = 1e-10
     gpu, sim, o, g = make_pair(*n, bc="cylinder", kind=kind, precond=precond, tol=tol,
-                               precision=precision)
+                               precision=precision, inner_tol=inner_tol)
     for q in range(steps):
         info = gpu.step(1)
         out = sim.step()
