@@ -518,6 +518,7 @@ def plot3d_run(wl, args):
         t = time.perf_counter()
         s.plot3d_q(path, 0.0)
         t_stage = time.perf_counter() - t
+        t = time.perf_counter()
         for _ in range(n):
             s.step(1)
         s.synchronize()
@@ -535,7 +536,9 @@ def plot3d_run(wl, args):
             "stage_call_ms": round(t_stage * 1e3, 3), "flush_ms": round(t_flush * 1e3, 3),
             "x_file_ms": round(t_x * 1e3, 3),
             "note": ("q output started before 3 mixed-precision steps (staging kernel on the "
-                     "step stream, device-to-host copy on a second stream); flush = wait + "
+                     "step stream, device-to-host copy on a second stream); step_ms_with_q_staged "
+                     "times those steps alone, stage_call_ms the call that starts the output "
+                     "(the first call also allocates the staging buffers); flush = wait + "
                      "file write to a temporary directory")}
 
 
