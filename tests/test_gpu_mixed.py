@@ -110,6 +110,13 @@ def test_mixed_configuration_limits(precision):
     # a hierarchy that cannot reach 512 cells is refused at creation
     with pytest.raises(CfdmxError, match="multigrid"):
         pkg.FlowSolver(pkg.default_config(nx=45, ny=27, nz=7, precision=precision))
+    # inner_tol: 0 (the default) takes the mode's own value; outside [0, 1) is refused
+    for bad in (-1e-3, 1.0):
+        with pytest.raises(CfdmxError, match="inner_tol"):
+            pkg.FlowSolver(pkg.default_config(nx=48, ny=24, nz=8, precision=precision,
+                                              inner_tol=bad))
+    pkg.FlowSolver(pkg.default_config(nx=48, ny=24, nz=8, precision=precision,
+                                      inner_tol=0.0)).close()
 
 
 def test_odd_nz_levels_take_the_per_cell_kernels():
