@@ -11,14 +11,17 @@ BC_PRESETS = {
     "slip": (op.SLIP_NUM, op.SLIP_DIR, 0, 0, 0),
     # inflow U=1, convective outflow with flux correction, slip y walls, periodic z
     "cylinder": (op.SLIP_NUM, op.SLIP_DIR, 1, 1, 1),
+    # the same without the flux correction: the convective update alone sets the outlet
+    "convective": (op.SLIP_NUM, op.SLIP_DIR, 1, 1, 0),
 }
 
 
-def make_pair(nxc, nyc, nzc, *, lengths=(20.0, 10.0, 0.5), kind=0, re=100.0, dt=0.002,
-              solver=0, precond=0, tol=1e-6, maxit=5000, omega=1.8, bc="flowpassing",
-              seed=20261202, amp=1e-3, cyl=(5.0, 5.0, 0.5), xcd_remap=1, smoothed_eps=None,
-              skip_first=1, inflow=None, precision=0, inner_tol=1e-5, mg_graphs=1,
-              smag_cs=0.0):
+def make_gpu(nxc, nyc, nzc, *, lengths=(20.0, 10.0, 0.5), kind=0, re=100.0, dt=0.002,
+             solver=0, precond=0, tol=1e-6, maxit=5000, omega=1.8, bc="flowpassing",
+             seed=20261202, amp=1e-3, cyl=(5.0, 5.0, 0.5), xcd_remap=1, smoothed_eps=None,
+             skip_first=1, inflow=None, precision=0, inner_tol=1e-5, mg_graphs=1,
+             smag_cs=0.0):
+    """The device context of a case, with its seeded initial field and mask."""
     num, dirr, zper, conv, flux = BC_PRESETS[bc]
     cfg = pkg.default_config(nx=nxc, ny=nyc, nz=nzc, lx=lengths[0], ly=lengths[1],
                              lz=lengths[2], grid_kind=kind, re=re, dt=dt, solver=solver,
@@ -29,6 +32,29 @@ def make_pair(nxc, nyc, nzc, *, lengths=(20.0, 10.0, 0.5), kind=0, re=100.0, dt=
                              precision=precision, inner_tol=inner_tol, mg_graphs=mg_graphs,
                              smagorinsky_cs=smag_cs)
     gpu = pkg.FlowSolver(cfg)
+    if inflow is not None:
+        gpu.set_inflow(inflow)
+    gpu.init_flow(1.0, amp, seed)
+    if cyl is not None:
+        if smoothed_eps is None:
+            gpu.set_eta_cylinder_z(*cyl, 100)
+        else:
+            gpu.set_eta_smoothed(*cyl, smoothed_eps)
+    return gpu
+
+
+def make_pair(nxc, nyc, nzc, *, lengths=(20.0, 10.0, 0.5), kind=0, re=100.0, dt=0.002,
+              solver=0, precond=0, tol=1e-6, maxit=5000, omega=1.8, bc="flowpassing",
+              seed=20261202, amp=1e-3, cyl=(5.0, 5.0, 0.5), xcd_remap=1, smoothed_eps=None,
+              skip_first=1, inflow=None, precision=0, inner_tol=1e-5, mg_graphs=1,
+              smag_cs=0.0):
+    """The device context of a case and the oracle's run of the same case."""
+    num, dirr, zper, conv, flux = BC_PRESETS[bc]
+    gpu = make_gpu(nxc, nyc, nzc, lengths=lengths, kind=kind, re=re, dt=dt, solver=solver,
+                   precond=precond, tol=tol, maxit=maxit, omega=omega, bc=bc, seed=seed,
+                   amp=amp, cyl=cyl, xcd_remap=xcd_remap, smoothed_eps=smoothed_eps,
+                   skip_first=skip_first, inflow=inflow, precision=precision,
+                   inner_tol=inner_tol, mg_graphs=mg_graphs, smag_cs=smag_cs)
     o = op.Oracle()
     g = o.grid(nxc, nyc, nzc, *lengths, kind=kind)
     obc = op.make_bc(num, dirr, inflow=inflow, z_periodic=zper, outflow_convective=conv,
@@ -37,19 +63,14 @@ def make_pair(nxc, nyc, nzc, *, lengths=(20.0, 10.0, 0.5), kind=0, re=100.0, dt=
     sim = op.Sim(o, g, obc, dt, re, solver=osolver, precond=precond, tol=tol, maxit=maxit,
                  omega=omega, reduction=op.RED_BLOCKED, skip_first=skip_first, swap=1,
                  smag_cs=smag_cs)
-    if inflow is not None:
-        gpu.set_inflow(inflow)
-    gpu.init_flow(1.0, amp, seed)
     u, v, w = o.init_velocity(g, 1.0, amp, seed)
     sim.field("u0")[:] = u
     sim.field("v0")[:] = v
     sim.field("w0")[:] = w
     if cyl is not None:
         if smoothed_eps is None:
-            gpu.set_eta_cylinder_z(*cyl, 100)
             sim.field("eta")[:] = o.eta_cylinder_z(g, *cyl, 100)
         else:
-            gpu.set_eta_smoothed(*cyl, smoothed_eps)
             sim.field("eta")[:] = o.eta_smoothed(g, *cyl, smoothed_eps)
     sim.start()
     return gpu, sim, o, g

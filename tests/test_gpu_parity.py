@@ -66,7 +66,7 @@ def test_convdiff_bitwise(kind, n, L, zper):
     gpu.close()
 
 
-@pytest.mark.parametrize("bc", ["flowpassing", "slip", "cylinder"])
+@pytest.mark.parametrize("bc", ["flowpassing", "slip", "cylinder", "convective"])
 def test_boundary_conditions_bitwise(bc):
     gpu, sim, o, g = make_pair(32, 16, 6, bc=bc)
     assert fields_equal(gpu, sim) == []
