@@ -64,17 +64,19 @@ class Config(C.Structure):
         ("solver", C.c_int), ("precond", C.c_int), ("tol", C.c_double), ("maxit", C.c_int),
         ("sor_omega", C.c_double), ("skip_first_pressure", C.c_int), ("precision", C.c_int),
         ("inner_tol", C.c_double), ("xcd_remap", C.c_int), ("mg_graphs", C.c_int),
-        ("smagorinsky_cs", C.c_double),
+        ("smagorinsky_cs", C.c_double), ("steady_check", C.c_int),
     ]
 
 
 class StepInfo(C.Structure):
     _fields_ = [("cd", C.c_double), ("cl", C.c_double), ("time", C.c_double),
                 ("iters", C.c_int), ("err", C.c_double), ("cfl", C.c_double),
-                ("solved", C.c_int), ("restarts", C.c_int), ("resets", C.c_int)]
+                ("solved", C.c_int), ("restarts", C.c_int), ("resets", C.c_int),
+                ("maxdiff", C.c_double * 3), ("unconverged", C.c_int)]
 
     def as_dict(self):
-        return {n: getattr(self, n) for n, _ in self._fields_}
+        return {n: (tuple(getattr(self, n)) if n == "maxdiff" else getattr(self, n))
+                for n, _ in self._fields_}
 
 
 _lib = None

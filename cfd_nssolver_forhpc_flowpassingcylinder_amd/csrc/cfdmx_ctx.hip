@@ -80,11 +80,12 @@ struct cfdmx_ctx {
         double *apart = nullptr, *out = nullptr, *out_h = nullptr;
     } adj;
     double *part = nullptr, *cfl_part = nullptr, *flux_part = nullptr;
+    double *diff_part = nullptr; // block maxima of the steady-state check (3 * nblocks)
     // Smagorinsky model (cfg.smagorinsky_cs > 0): eddy viscosity per interior cell and its
     // factor (Cs delta)^2
     double *viseff = nullptr, *smag_coef = nullptr;
     double *bpart[3] = {}; // per-phase BiCGSTAB partials (A, B, C)
-    double *scal = nullptr; // [0] cfl, [1] cd, [2] cl
+    double *scal = nullptr; // [0] cfl, [1] cd, [2] cl, [3..5] max(T0 - T3) of u, v, w
     double *inflow = nullptr;
     KState *kst = nullptr;
     SorState *sst = nullptr;
@@ -98,6 +99,7 @@ struct cfdmx_ctx {
     int loop = 1;
     double time = 0.0;
     int resets = 0;
+    int unconverged = 0; // pressure solves that ended with the residual not below tol
     int started = 0;
     // per-kernel HIP-event timing on the context's stream (cfdmx_profile)
     struct ProfRec {
@@ -599,6 +601,7 @@ static int setup_device(cfdmx_ctx *ctx)
     ALLOC(ctx->B.ap[1], N);
     ALLOC(ctx->part, 4 * (size_t)(g.nblocks > g.nmarch ? g.nblocks : g.nmarch) + 64);
     ALLOC(ctx->cfl_part, (size_t)g.cd_nb + 8);
+    ALLOC(ctx->diff_part, 3 * (size_t)g.nblocks + 8);
     if (ctx->cfg.smagorinsky_cs > 0.0) {
         // (Cs delta)^2 with delta = pow(Dx Dy Dz, 1/3) (SmagorinskyModel.hpp:74-76), on the
         // host with the C library's pow
@@ -694,6 +697,8 @@ static int validate(cfdmx_ctx *ctx)
     if (c.maxit < 0) FAIL(CFDMX_ERR_ARG, "maxit must be >= 0");
     if (!(c.smagorinsky_cs >= 0.0 && c.smagorinsky_cs < 1.0))
         FAIL(CFDMX_ERR_ARG, "smagorinsky_cs must be in [0, 1) (got %g)", c.smagorinsky_cs);
+    if (c.steady_check != 0 && c.steady_check != 1)
+        FAIL(CFDMX_ERR_ARG, "steady_check must be 0 or 1 (got %d)", c.steady_check);
     if (c.precision != CFDMX_PRECISION_FP64 && c.precision != CFDMX_PRECISION_MIXED &&
         c.precision != CFDMX_PRECISION_FP64_MG)
         FAIL(CFDMX_ERR_ARG, "unknown precision %d", c.precision);
@@ -757,6 +762,7 @@ int cfdmx_config_default(cfdmx_config *cfg)
     cfg->xcd_remap = 1;
     cfg->mg_graphs = 1;
     cfg->smagorinsky_cs = 0.0;
+    cfg->steady_check = 0;
     return CFDMX_OK;
 }
 
@@ -1244,9 +1250,18 @@ static int solve_refine(cfdmx_ctx *ctx, const double *b, double *x, int *iters, 
         const double bnorm = std::sqrt(ctx->mix_h[0]);
         res64 = std::sqrt(ctx->mix_h[1]) / bnorm;
         if (outer > 0 && !(res64 <= res_prev)) {
+            // take the pass back: x - e is within a rounding of the previous x, so its
+            // residual is formed again and reported, not the previous pass's
             PROF(CFDMX_K_BICG_INIT, launch_mixed_update<T>(g, s, -1.0, x, e));
+            PROF(CFDMX_K_BICG_INIT,
+                 launch_mixed_resid<T>(g, s, 0, b, x, pv[0], pv[1], e, ctx->part));
+            launch_mixed_fold(g, s, ctx->part, ctx->mix);
             LAUNCHCHK();
-            res64 = res_prev;
+            HIPCHK(hipMemcpyAsync(ctx->mix_h, ctx->mix, 3 * sizeof(double),
+                                  hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            prof_harvest(ctx);
+            res64 = std::sqrt(ctx->mix_h[1]) / std::sqrt(ctx->mix_h[0]);
             break;
         }
         res_prev = res64;
@@ -1422,13 +1437,18 @@ static int step_once(cfdmx_ctx *ctx, int *iters, double *err, int *solved)
             rc = solve_rbsor_impl(ctx, iters, err);
         }
         if (rc) return rc;
+        if (!(*err < c.tol)) ++ctx->unconverged;
     }
     const int from_x = (c.solver == CFDMX_SOLVER_BICGSTAB && *solved) ? 1 : 0;
+    // the steady-state check reads T0 (u^n) beside the new field the projection writes
+    const double *const prev[3] = {ctx->t0[0], ctx->t0[1], ctx->t0[2]};
     PROF(CFDMX_K_PROJECT,
          launch_project(g, s, from_x, c.dt, c.outflow_convective, ctx->x, ctx->p, ctx->eta,
                         ctx->t1[0], ctx->t1[1], ctx->t1[2], ctx->t3[0], ctx->t3[1], ctx->t3[2],
-                        ctx->f, ctx->part));
+                        ctx->f, ctx->part, c.steady_check ? prev : nullptr,
+                        c.steady_check ? ctx->diff_part : nullptr));
     launch_cdcl_fin(g, s, ctx->part, c.lz, ctx->scal + 1);
+    if (c.steady_check) launch_max3_fin(g, s, ctx->diff_part, ctx->scal + 3);
     for (int q = 0; q < 3; ++q) std::swap(ctx->t0[q], ctx->t3[q]);
     PROF(CFDMX_K_BC, launch_bc(g, s, ctx->bc, 1, ctx->t0[0], ctx->t0[1], ctx->t0[2], ctx->p,
                                ctx->t1[0], ctx->t1[1], ctx->t1[2], ctx->flux_part));
@@ -1464,6 +1484,9 @@ int cfdmx_step(cfdmx_ctx *ctx, int nsteps, cfdmx_step_info *info)
         HIPCHK(hipStreamSynchronize(ctx->stream));
         info->restarts = ctx->kst_h->restarts;
         info->resets = ctx->resets;
+        for (int q = 0; q < 3; ++q)
+            info->maxdiff[q] = ctx->cfg.steady_check ? ctx->scal_h[3 + q] : 0.0;
+        info->unconverged = ctx->unconverged;
     }
     return CFDMX_OK;
 }
@@ -1558,6 +1581,7 @@ int cfdmx_plot3d_flush(cfdmx_ctx *ctx)
 // a context of the same configuration continues the run bit for bit.
 namespace {
 constexpr char kCkMagic[8] = {'C', 'F', 'D', 'M', 'X', 'C', 'K', '1'};
+constexpr int32_t kCkVersion = 2;
 struct CkHeader {
     char magic[8];
     int32_t version, nx, ny, nz, grid_kind, precision, solver, z_periodic;
@@ -1567,7 +1591,29 @@ struct CkHeader {
     double time, dt, re, mask_cx, mask_cy, mask_r, mask_eps;
     KState kst;
     SorState sst;
+    // the configuration the saved steps ran with, beyond the grid: a load into a context
+    // configured otherwise would not continue the run
+    double bc_num[6][3], bc_dir[6][3];
+    double convective_speed, smagorinsky_cs;
+    int32_t precond, outflow_convective, flux_correction, unconverged;
 };
+
+// the first configuration entry that differs between a checkpoint and a context, or null
+static const char *ck_mismatch(const CkHeader &h, const cfdmx_config &c)
+{
+    if (h.precision != c.precision) return "precision";
+    if (h.solver != c.solver) return "solver";
+    if (h.precond != c.precond) return "precond";
+    if (std::memcmp(h.bc_num, c.bc_num, sizeof h.bc_num) != 0 ||
+        std::memcmp(h.bc_dir, c.bc_dir, sizeof h.bc_dir) != 0)
+        return "boundary-condition table (bc_num, bc_dir)";
+    if (h.outflow_convective != c.outflow_convective) return "outflow_convective";
+    if (h.outflow_convective && h.convective_speed != c.convective_speed)
+        return "convective_speed";
+    if (h.flux_correction != c.flux_correction) return "flux_correction";
+    if (h.smagorinsky_cs != c.smagorinsky_cs) return "smagorinsky_cs";
+    return nullptr;
+}
 
 // the checkpointed arrays and their lengths, in file order
 static std::vector<std::pair<double *, size_t>> ck_arrays(cfdmx_ctx *ctx)
@@ -1596,7 +1642,7 @@ int cfdmx_checkpoint_save(cfdmx_ctx *ctx, const char *path)
     std::memset(&h, 0, sizeof h);
     std::memcpy(h.magic, kCkMagic, sizeof h.magic);
     const cfdmx_config &c = ctx->cfg;
-    h.version = 1;
+    h.version = kCkVersion;
     h.nx = c.nx, h.ny = c.ny, h.nz = c.nz;
     h.grid_kind = c.grid_kind, h.precision = c.precision, h.solver = c.solver;
     h.z_periodic = c.z_periodic;
@@ -1608,6 +1654,14 @@ int cfdmx_checkpoint_save(cfdmx_ctx *ctx, const char *path)
     h.time = ctx->time, h.dt = c.dt, h.re = c.re;
     h.mask_cx = ctx->mask_cx, h.mask_cy = ctx->mask_cy, h.mask_r = ctx->mask_r;
     h.mask_eps = ctx->mask_eps;
+    std::memcpy(h.bc_num, c.bc_num, sizeof h.bc_num);
+    std::memcpy(h.bc_dir, c.bc_dir, sizeof h.bc_dir);
+    h.convective_speed = c.convective_speed;
+    h.smagorinsky_cs = c.smagorinsky_cs;
+    h.precond = c.precond;
+    h.outflow_convective = c.outflow_convective;
+    h.flux_correction = c.flux_correction;
+    h.unconverged = ctx->unconverged;
     HIPCHK(hipMemcpyAsync(ctx->kst_h, ctx->kst, sizeof(KState), hipMemcpyDeviceToHost,
                           ctx->stream));
     HIPCHK(hipMemcpyAsync(ctx->sst_h, ctx->sst, sizeof(SorState), hipMemcpyDeviceToHost,
@@ -1642,9 +1696,10 @@ int cfdmx_checkpoint_load(cfdmx_ctx *ctx, const char *path)
     if (!fp) FAIL(CFDMX_ERR_ARG, "checkpoint: cannot open %s", path);
     CkHeader h;
     if (std::fread(&h, sizeof h, 1, fp) != 1 ||
-        std::memcmp(h.magic, kCkMagic, sizeof h.magic) != 0 || h.version != 1) {
+        std::memcmp(h.magic, kCkMagic, sizeof h.magic) != 0 || h.version != kCkVersion) {
         std::fclose(fp);
-        FAIL(CFDMX_ERR_ARG, "checkpoint: %s is not a cfdmx checkpoint", path);
+        FAIL(CFDMX_ERR_ARG, "checkpoint: %s is not a cfdmx checkpoint of version %d", path,
+             kCkVersion);
     }
     const cfdmx_config &c = ctx->cfg;
     if (h.nx != c.nx || h.ny != c.ny || h.nz != c.nz || h.grid_kind != c.grid_kind ||
@@ -1655,6 +1710,12 @@ int cfdmx_checkpoint_load(cfdmx_ctx *ctx, const char *path)
              "checkpoint: %s holds a %dx%dx%d grid (kind %d) of another size or domain than "
              "this context's %dx%dx%d (kind %d)",
              path, h.nx, h.ny, h.nz, h.grid_kind, c.nx, c.ny, c.nz, c.grid_kind);
+    }
+    if (const char *what = ck_mismatch(h, c)) {
+        std::fclose(fp);
+        FAIL(CFDMX_ERR_ARG,
+             "checkpoint: %s was saved with another %s than this context's configuration",
+             path, what);
     }
     const auto arrays = ck_arrays(ctx);
     size_t most = 0, total = sizeof h;
@@ -1690,6 +1751,7 @@ int cfdmx_checkpoint_load(cfdmx_ctx *ctx, const char *path)
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->loop = h.loop;
     ctx->resets = h.resets;
+    ctx->unconverged = h.unconverged;
     ctx->started = h.started;
     ctx->time = h.time;
     ctx->cfg.dt = h.dt;

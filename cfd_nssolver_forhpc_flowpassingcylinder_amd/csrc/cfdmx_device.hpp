@@ -199,4 +199,28 @@ __device__ __forceinline__ double wave_max(double v)
     return v;
 }
 
+// NS block maxima (starting from the values given) into out_partials[s * stride + L];
+// a maximum is exact in any order, so no fixed order is kept
+template <int NS>
+__device__ __forceinline__ void block_maxes(double (&v)[NS], double *out_partials, int stride,
+                                            int L)
+{
+    __shared__ double red[NS][4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        const double w = wave_max(v[s]);
+        if (lane == 0) red[s][wave] = w;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            double m = red[s][0];
+            for (int q = 1; q < 4; ++q) m = red[s][q] > m ? red[s][q] : m;
+            out_partials[s * stride + L] = m;
+        }
+    }
+}
+
 } // namespace cfdmx

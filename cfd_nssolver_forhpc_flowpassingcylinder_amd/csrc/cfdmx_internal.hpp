@@ -147,10 +147,14 @@ void launch_sor_rhs(const Geo &g, hipStream_t s, const double *us, const double 
 void launch_sor_half(const Geo &g, hipStream_t s, int colour, double omega, const double *cf6,
                      double *p, double *part, SorState *st);
 void launch_sor_fin(const Geo &g, hipStream_t s, double *part, SorState *st);
+// prev (T0 of the step) and dpart (3 * nblocks doubles) add the steady-state check;
+// both null leaves it out
 void launch_project(const Geo &g, hipStream_t s, int from_x, double dt, int convective,
                     const double *xres, double *p, const double *eta, const double *us,
                     const double *vs, const double *ws, double *u, double *v, double *w,
-                    double *f, double *part);
+                    double *f, double *part, const double *const *prev = nullptr,
+                    double *dpart = nullptr);
+void launch_max3_fin(const Geo &g, hipStream_t s, const double *dpart, double *out);
 void launch_cdcl_fin(const Geo &g, hipStream_t s, const double *part, double lz, double *out);
 void launch_force_partial(const Geo &g, hipStream_t s, const double *f, double *part);
 void launch_max_fin(const Geo &g, hipStream_t s, const double *part, double *out);

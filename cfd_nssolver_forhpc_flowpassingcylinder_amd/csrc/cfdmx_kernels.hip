@@ -1532,19 +1532,25 @@ k_sor_fin(const double *part, int nb, SorState *st)
 // (Transform.hpp:8-25) and the drag/lift partial sums of virtualF_Int
 // (virtualForceIntergrator.hpp:79-86).  FROM_X: pressure read from X_result.
 // ---------------------------------------------------------------------------
-template <int FROM_X>
+// DIFF: also the steady-state check of CheckSteadyStatebyMaxVal_omp
+// (CheckSteadystate.hpp:247-307), max(T0 - T3) per component starting at 0, over the
+// faces written here, from the previous field u0, v0, w0: block maxima into dpart
+template <int FROM_X, int DIFF>
 __global__ void __launch_bounds__(kThreads)
 k_project(Geo g, double dt, double rdt, int convective, const double *__restrict__ xres,
           double *__restrict__ p, const double *__restrict__ eta,
           const double *__restrict__ us, const double *__restrict__ vs,
           const double *__restrict__ ws, double *__restrict__ u, double *__restrict__ v,
-          double *__restrict__ w, double *__restrict__ f, double *__restrict__ part)
+          double *__restrict__ w, double *__restrict__ f, double *__restrict__ part,
+          const double *__restrict__ u0, const double *__restrict__ v0,
+          const double *__restrict__ w0, double *__restrict__ dpart)
 {
     const int L = logical_block(g);
     if (L >= g.nblocks) return;
     const Axis &X = g.ax[0], &Y = g.ax[1], &Z = g.ax[2];
     const double uS = 0.0, vS = 0.0, wS = 0.0;
     double acc[2] = {0.0, 0.0};
+    double dm[3] = {0.0, 0.0, 0.0};
     // Every load is unconditional: where a face is skipped (the last interior x or y
     // plane, the last z plane without periodicity) its neighbour index falls back to the
     // cell itself and the face's values are computed and dropped.  Only the stores and
@@ -1601,8 +1607,35 @@ k_project(Geo g, double dt, double rdt, int convective, const double *__restrict
         const double vol = X.D[I] * Y.D[J] * Z.D[K];
         acc[0] += fx * vol;
         acc[1] += fy * vol;
+        if (DIFF) {
+            const double du = u0[ic] - un, dv = v0[ic] - vn, dw = w0[ic] - wn;
+            if (ox && du > dm[0]) dm[0] = du;
+            if (oy && dv > dm[1]) dm[1] = dv;
+            if (oz && dw > dm[2]) dm[2] = dw;
+        }
     }
     block_sums<2>(acc, part, g.nblocks, L);
+    if (DIFF) block_maxes<3>(dm, dpart, g.nblocks, L);
+}
+
+// the three maxima of the steady-state check from the block maxima
+__global__ void __launch_bounds__(kThreads) k_max3_fin(const double *part, int nb, double *out)
+{
+    __shared__ double red[3][4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int c = 0; c < 3; ++c) {
+        double m = 0.0;
+        for (int q = threadIdx.x; q < nb; q += kThreads) m = part[c * nb + q] > m ? part[c * nb + q] : m;
+        const double w = wave_max(m);
+        if (lane == 0) red[c][wave] = w;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int c = 0; c < 3; ++c) {
+            double m = red[c][0];
+            for (int q = 1; q < 4; ++q) m = red[c][q] > m ? red[c][q] : m;
+            out[c] = m;
+        }
 }
 
 __global__ void __launch_bounds__(kThreads)
@@ -2027,15 +2060,32 @@ void launch_sor_fin(const Geo &g, hipStream_t s, double *part, SorState *st)
 void launch_project(const Geo &g, hipStream_t s, int from_x, double dt, int convective,
                     const double *xres, double *p, const double *eta, const double *us,
                     const double *vs, const double *ws, double *u, double *v, double *w,
-                    double *f, double *part)
+                    double *f, double *part, const double *const *prev, double *dpart)
 {
     const double rdt = 1.0 / dt;
-    if (from_x)
-        hipLaunchKernelGGL(k_project<1>, dim3(grid_blocks(g)), dim3(kThreads), 0, s, g, dt, rdt,
-                           convective, xres, p, eta, us, vs, ws, u, v, w, f, part);
-    else
-        hipLaunchKernelGGL(k_project<0>, dim3(grid_blocks(g)), dim3(kThreads), 0, s, g, dt, rdt,
-                           convective, xres, p, eta, us, vs, ws, u, v, w, f, part);
+    const dim3 grid(grid_blocks(g)), block(kThreads);
+    const double *u0 = prev ? prev[0] : nullptr, *v0 = prev ? prev[1] : nullptr,
+                 *w0 = prev ? prev[2] : nullptr;
+    if (dpart) {
+        if (from_x)
+            hipLaunchKernelGGL((k_project<1, 1>), grid, block, 0, s, g, dt, rdt, convective, xres,
+                               p, eta, us, vs, ws, u, v, w, f, part, u0, v0, w0, dpart);
+        else
+            hipLaunchKernelGGL((k_project<0, 1>), grid, block, 0, s, g, dt, rdt, convective, xres,
+                               p, eta, us, vs, ws, u, v, w, f, part, u0, v0, w0, dpart);
+    } else {
+        if (from_x)
+            hipLaunchKernelGGL((k_project<1, 0>), grid, block, 0, s, g, dt, rdt, convective, xres,
+                               p, eta, us, vs, ws, u, v, w, f, part, u0, v0, w0, dpart);
+        else
+            hipLaunchKernelGGL((k_project<0, 0>), grid, block, 0, s, g, dt, rdt, convective, xres,
+                               p, eta, us, vs, ws, u, v, w, f, part, u0, v0, w0, dpart);
+    }
+}
+
+void launch_max3_fin(const Geo &g, hipStream_t s, const double *dpart, double *out)
+{
+    hipLaunchKernelGGL(k_max3_fin, dim3(1), dim3(kThreads), 0, s, dpart, g.nblocks, out);
 }
 
 void launch_cdcl_fin(const Geo &g, hipStream_t s, const double *part, double lz, double *out)

@@ -75,8 +75,10 @@ typedef struct cfdmx_config {
     int precision;
     double inner_tol; /* multigrid modes: relative tolerance of each inner solve;
                          0 (the default) takes the mode's fastest measured value,
-                         3e-3 for MIXED and 1e-5 for FP64_MG; the FP64 refinement
-                         still stops only at tol */
+                         3e-3 for MIXED and 1e-5 for FP64_MG.  The FP64 refinement stops
+                         at tol, or unconverged (cfdmx_step_info.unconverged) after 40
+                         passes, at maxit inner iterations in all, on a pass that raises
+                         the residual (taken back) or on an inner divergence */
     /* launch shaping */
     int xcd_remap; /* map consecutive logical blocks to one XCD (0 = hardware order) */
     int mg_graphs; /* multigrid modes: replay each V-cycle from a captured HIP graph
@@ -85,6 +87,11 @@ typedef struct cfdmx_config {
      * constant Cs of the eddy viscosity (Cs delta)^2 |S| that each step adds to 1/Re in the
      * diffusion; 0 = off (the reference's default build; its constant is 0.18) */
     double smagorinsky_cs;
+    /* CheckSteadyStatebyMaxVal_omp (CheckSteadystate.hpp:247-307, called every step at
+     * RUN_CPU.cpp:380-384): 1 = each step also reports max(T0 - T3) per component in
+     * cfdmx_step_info.maxdiff, folded into the projection (it then reads T0 as well:
+     * 24 more bytes per cell); 0 (the default) = off */
+    int steady_check;
 } cfdmx_config;
 
 typedef struct cfdmx_ctx cfdmx_ctx;
@@ -99,6 +106,16 @@ typedef struct cfdmx_step_info {
     int solved;       /* 1 if a pressure solve ran this step */
     int restarts;     /* cumulative BiCGSTAB rho restarts */
     int resets;       /* cumulative BiCGSTAB NaN resets */
+    /* steady_check: max(T0 - T3) of u, v, w over the faces the projection writes,
+     * starting at 0 as the reference's loop does (CheckSteadystate.hpp:253-301); the
+     * faces it skips are left out, because the reference's T3 holds its initial values
+     * there (copyVel copies T3 into T0, never back).  0 when steady_check is off. */
+    double maxdiff[3];
+    /* cumulative pressure solves that ended with err not below tol: the iteration
+     * limit, and in the multigrid modes also the 40-pass limit of the refinement, a
+     * refinement pass that raised the residual (taken back) or an inner solve that
+     * diverged; err then holds the residual reached */
+    int unconverged;
 } cfdmx_step_info;
 
 /* Defaults: FLOWpassing table, reference solver defaults (tol 1e-3, 5000 iterations,
@@ -157,9 +174,12 @@ CFDMX_API int cfdmx_plot3d_x(cfdmx_ctx *ctx, const char *path);
  * later step reads (T0, T1, T3, p, eta, the forcing, X_result and its two history
  * copies, the inflow profile) with the step counter, time, dt, Re, the mask parameters
  * and the solver states; cfdmx_checkpoint_load restores them into a context created
- * with the same configuration (size, grid kind, domain, z periodicity: CFDMX_ERR_ARG
- * otherwise), after which steps continue bit for bit as in the saved run.  Both
- * synchronise the context's stream. */
+ * with the same configuration (size, grid kind, domain, z periodicity, precision,
+ * solver, operator, boundary-condition table, outflow and flux-correction settings and
+ * Smagorinsky constant: CFDMX_ERR_ARG otherwise), after which steps continue bit for
+ * bit as in the saved run as long as the tolerances (tol, maxit, sor_omega, inner_tol)
+ * are also the saved run's, which the load does not check.  Both synchronise the
+ * context's stream. */
 CFDMX_API int cfdmx_checkpoint_save(cfdmx_ctx *ctx, const char *path);
 CFDMX_API int cfdmx_checkpoint_load(cfdmx_ctx *ctx, const char *path);
 

@@ -517,7 +517,7 @@ double orc_adjoint_drag(const orc_grid *g, const orc_bc *bc, const orc_params *p
         for (int c = 0; c < 3; ++c)
             memcpy(tapeU + (3 * (size_t)s + c) * G, orc_sim_field(sim, c), sizeof(double) * G);
         memcpy(tapeP + (size_t)s * G, orc_sim_field(sim, 9), sizeof(double) * G);
-        double out[8];
+        double out[11];
         orc_sim_step(sim, out);
         J += out[0];
         solved[s] = (int)out[7];
@@ -616,7 +616,7 @@ double orc_drag_objective(const orc_grid *g, const orc_bc *bc, const orc_params 
     orc_sim_start(sim);
     double J = 0.0;
     for (int s = 0; s < nsteps; ++s) {
-        double out[8];
+        double out[11];
         orc_sim_step(sim, out);
         J += out[0];
     }

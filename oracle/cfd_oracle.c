@@ -1182,6 +1182,31 @@ void orc_maxdiff(const orc_grid *g, const double *u0, const double *v0, const do
     out[2] = m[2];
 }
 
+/* The same maximum over the faces the projection computes this step (orc_update_uf's
+ * ranges: u for i < nx-3, v for j < ny-3, w for k < nz-3, or every k when z is
+ * periodic).  The reference's T3 keeps, on the faces its update skips, values no step
+ * refreshes (the initial field: copyVel copies T3 into T0, never back), so those faces
+ * are left out here. */
+void orc_maxdiff_faces(const orc_grid *g, int zp, const double *u0, const double *v0,
+                       const double *w0, const double *u3, const double *v3, const double *w3,
+                       double out[3])
+{
+    double m[3] = {0.0, 0.0, 0.0};
+    const int kend = zp ? g->nz - GC : g->nz - GC - 1;
+    for (int i = GC; i < g->nx - GC; ++i)
+        for (int j = GC; j < g->ny - GC; ++j)
+            for (int k = GC; k < g->nz - GC; ++k) {
+                const long ic = IC(g, i, j, k);
+                const double du = u0[ic] - u3[ic], dv = v0[ic] - v3[ic], dw = w0[ic] - w3[ic];
+                if (i < g->nx - GC - 1 && du > m[0]) m[0] = du;
+                if (j < g->ny - GC - 1 && dv > m[1]) m[1] = dv;
+                if (k < kend && dw > m[2]) m[2] = dw;
+            }
+    out[0] = m[0];
+    out[1] = m[1];
+    out[2] = m[2];
+}
+
 /* Seeded initial field (no reference counterpart; the reference starts from
  * u = 1, v = w = 0, RUN_CPU.cpp:66-76): u = u0 + amp*(2U - 1) on interior cells,
  * U from splitmix64(seed ^ icelCal). */
@@ -1295,7 +1320,7 @@ void orc_sim_start(orc_sim *s)
     orc_bc_copy(&s->g, s->f0[0], s->f0[1], s->f0[2], s->f1[0], s->f1[1], s->f1[2]);
 }
 
-int orc_sim_step(orc_sim *s, double out[8])
+int orc_sim_step(orc_sim *s, double out[11])
 {
     const orc_grid *g = &s->g;
     const orc_params *P = &s->prm;
@@ -1326,6 +1351,9 @@ int orc_sim_step(orc_sim *s, double out[8])
     /* 3. projection + forcing (RUN_CPU.cpp:369) */
     orc_update_uf(g, P->dt, zp, s->bc.outflow_convective, s->p, s->eta, s->f1[0], s->f1[1],
                   s->f1[2], s->f3[0], s->f3[1], s->f3[2], s->f);
+    /* steady-state check max(T0 - T3) (RUN_CPU.cpp:380-384) */
+    double dmax[3];
+    orc_maxdiff_faces(g, zp, s->f0[0], s->f0[1], s->f0[2], s->f3[0], s->f3[1], s->f3[2], dmax);
     /* 4. T3 -> T0 (RUN_CPU.cpp:391) */
     if (P->swap_buffers) {
         for (int c = 0; c < 3; ++c) {
@@ -1352,6 +1380,9 @@ int orc_sim_step(orc_sim *s, double out[8])
         out[5] = cfl;
         out[6] = s->bicg->restarts;
         out[7] = solved;
+        out[8] = dmax[0];
+        out[9] = dmax[1];
+        out[10] = dmax[2];
     }
     s->time += P->dt;
     s->loop++;

@@ -133,6 +133,9 @@ double orc_cfl(const orc_grid *g, double dt, int z_periodic,
                const double *u, const double *v, const double *w);
 void orc_maxdiff(const orc_grid *g, const double *u0, const double *v0, const double *w0,
                  const double *u3, const double *v3, const double *w3, double out[3]);
+void orc_maxdiff_faces(const orc_grid *g, int zp, const double *u0, const double *v0,
+                       const double *w0, const double *u3, const double *v3, const double *w3,
+                       double out[3]);
 void orc_init_velocity(const orc_grid *g, double u0, double amp, unsigned long long seed,
                        double *u, double *v, double *w);
 
@@ -143,7 +146,8 @@ void orc_sim_free(orc_sim *s);
 /* which: 0..2 T0 u,v,w; 3..5 T1; 6..8 T3; 9 p; 10 eta; 11 f (3G); 12 X_result (N) */
 double *orc_sim_field(orc_sim *s, int which);
 void orc_sim_start(orc_sim *s); /* initial BC_main + BC_copy, T1 = T3 = T0 */
-int orc_sim_step(orc_sim *s, double out[8]);
+/* out: cd, cl, time, iters, err, cfl, restarts, solved, max(T0 - T3) of u, v, w */
+int orc_sim_step(orc_sim *s, double out[11]);
 /* out: cd, cl, time, iters, err, cfl, restarts, solve_flag */
 
 /* ---- adjoint of the time-averaged drag (cfd_adjoint.c; no reference counterpart) ----

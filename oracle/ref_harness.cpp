@@ -16,7 +16,9 @@
 #include "0_General.hpp"
 #include "run/general.hpp"
 
+#include <cstdio>
 #include <memory>
+#include <sstream>
 
 namespace {
 
@@ -295,6 +297,26 @@ int ref_plot3d_x(void *h)
 {
     auto *c = static_cast<RefCtx *>(h);
     return OutputPlot3D_Xfile(c->simu, *c->g);
+}
+
+// CheckSteadyStatebyMaxVal_omp (CheckSteadystate.hpp:247-307) on the harness's T0 and
+// T3.  The reference prints the three maxima and returns nothing; its line is captured
+// at 17 significant digits and read back, which restores the doubles exactly.
+int ref_maxdiff(void *h, double *out)
+{
+    auto *c = static_cast<RefCtx *>(h);
+    std::ostringstream buf;
+    std::streambuf *old = std::cout.rdbuf(buf.rdbuf());
+    const std::streamsize prec = std::cout.precision(17);
+    CheckSteadyStatebyMaxVal_omp(c->simu, c->ShareM, c->T0, c->T3, c->Lo, *c->g);
+    std::cout.precision(prec);
+    std::cout.rdbuf(old);
+    const std::string line = buf.str();
+    const size_t eq = line.find('=');
+    if (eq == std::string::npos) return -1;
+    return std::sscanf(line.c_str() + eq + 1, " %lf , %lf , %lf", &out[0], &out[1], &out[2]) == 3
+               ? 0
+               : -1;
 }
 
 int ref_threads(void) { return omp_get_max_threads(); }

@@ -141,6 +141,7 @@ class Oracle:
         L.orc_cfl.restype = C.c_double
         L.orc_cfl.argtypes = [G, C.c_double, C.c_int] + [DP] * 3
         L.orc_maxdiff.argtypes = [G] + [DP] * 7
+        L.orc_maxdiff_faces.argtypes = [G, C.c_int] + [DP] * 7
         L.orc_init_velocity.argtypes = [G, C.c_double, C.c_double, C.c_ulonglong, DP, DP, DP]
         L.orc_sim_new.restype = C.c_void_p
         L.orc_sim_new.argtypes = [G, C.POINTER(OrcBC), C.POINTER(OrcParams)]
@@ -265,6 +266,19 @@ class Oracle:
     def cfl(self, g, dt, u, v, w, z_periodic=0):
         return self.lib.orc_cfl(C.byref(g.c), dt, z_periodic, ptr(u), ptr(v), ptr(w))
 
+    def maxdiff(self, g, u0, v0, w0, u3, v3, w3):
+        """max(0, max over interior cells of T0 - T3) per component"""
+        out = np.zeros(3)
+        self.lib.orc_maxdiff(C.byref(g.c), *map(ptr, (u0, v0, w0, u3, v3, w3)), ptr(out))
+        return tuple(out)
+
+    def maxdiff_faces(self, g, u0, v0, w0, u3, v3, w3, z_periodic=0):
+        """the same over the faces the projection computes"""
+        out = np.zeros(3)
+        self.lib.orc_maxdiff_faces(C.byref(g.c), z_periodic,
+                                   *map(ptr, (u0, v0, w0, u3, v3, w3)), ptr(out))
+        return tuple(out)
+
     def init_velocity(self, g, u0=1.0, amp=1e-3, seed=20261202):
         u, v, w = np.zeros(g.G), np.zeros(g.G), np.zeros(g.G)
         self.lib.orc_init_velocity(C.byref(g.c), u0, amp, seed, ptr(u), ptr(v), ptr(w))
@@ -321,10 +335,11 @@ class Sim:
         self.o.lib.orc_sim_start(self.h)
 
     def step(self):
-        out = np.zeros(8)
+        out = np.zeros(11)
         self.o.lib.orc_sim_step(self.h, ptr(out))
         return dict(cd=out[0], cl=out[1], time=out[2], iters=int(out[3]), err=out[4],
-                    cfl=out[5], restarts=int(out[6]), solved=int(out[7]))
+                    cfl=out[5], restarts=int(out[6]), solved=int(out[7]),
+                    maxdiff=tuple(out[8:11]))
 
     def __del__(self):
         try:
@@ -354,6 +369,7 @@ class RefHarness:
         L.ref_apply_A.argtypes = [C.c_void_p, DP, DP]
         L.ref_sor_seq.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int, DP]
         L.ref_step.argtypes = [C.c_void_p, C.c_int, DP]
+        L.ref_maxdiff.argtypes = [C.c_void_p, DP]
         L.ref_set_maxit.argtypes = [C.c_void_p, C.c_int]
         L.ref_set_tol.argtypes = [C.c_void_p, C.c_double]
         L.ref_set_threads.argtypes = [C.c_int]
@@ -413,6 +429,12 @@ class RefHarness:
 
     def set_threads(self, threads):
         self.lib.ref_set_threads(threads)
+
+    def maxdiff(self):
+        """CheckSteadyStatebyMaxVal_omp on the harness's T0 and T3"""
+        out = np.zeros(3)
+        assert self.lib.ref_maxdiff(self.h, ptr(out)) == 0
+        return tuple(out)
 
     def step(self, loop):
         out = np.zeros(6)

@@ -105,3 +105,30 @@ def test_checkpoint_errors(tmp_path):
     with pytest.raises(CfdmxError, match="before cfdmx_start"):
         fresh.checkpoint_save(os.fspath(tmp_path / "b.ck"))
     fresh.close()
+
+
+@pytest.mark.parametrize("field,value,what", [
+    ("precision", PRECISION_FP64_MG, "precision"), ("solver", 1, "solver"),
+    ("precond", 1, "precond"), ("flux_correction", 0, "flux_correction"),
+    ("smagorinsky_cs", 0.1, "smagorinsky_cs")])
+def test_checkpoint_of_another_configuration_is_refused(field, value, what, tmp_path):
+    """A load into a context whose precision, solver, operator, boundary conditions or
+    Smagorinsky constant differ from the saved run's would not continue it: refused,
+    and the context is left as it was."""
+    gpu, *_ = make_pair(32, 16, 8, bc="cylinder", tol=1e-8)
+    gpu.step(2)
+    path = os.fspath(tmp_path / "a.ck")
+    gpu.checkpoint_save(path)
+    cfg = pkg.default_config()
+    for name, _ in cfg._fields_:
+        setattr(cfg, name, getattr(gpu.cfg, name))
+    setattr(cfg, field, value)
+    other = pkg.FlowSolver(cfg)
+    other.init_flow(1.0, 1e-3, 5)
+    before = other.download_velocity(0)
+    with pytest.raises(CfdmxError, match="another " + what):
+        other.checkpoint_load(path)
+    for a, b in zip(other.download_velocity(0), before):
+        assert np.array_equal(a, b)
+    other.close()
+    gpu.close()

@@ -208,3 +208,21 @@ def test_single_level_is_the_exact_solve(precision, max_inner, n, lengths):
     print(f"{n}: {it} inner iterations, residual {rel:.3e}")
     assert rel < tol and it <= max_inner, (it, rel)
     gpu.close()
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_refinement_that_cannot_converge_is_reported(precision):
+    """Without the flux correction the singular system is inconsistent: the FP64
+    refinement cannot reach tol 1e-10, ends (pass limit, residual rise or maxit) with
+    err >= tol, and the step report counts the solve as unconverged; the reported err
+    is the FP64 residual of the x the solve returns (recomputed on the CPU)."""
+    gpu, sim, o, g = make_pair(48, 24, 8, bc="convective", tol=1e-10, precision=precision,
+                               maxit=200)
+    gpu.step(1)
+    info = gpu.step(1)
+    print(info)
+    assert info["err"] >= 1e-10 and info["unconverged"] == 1
+    b = gpu.rhs()
+    r = b - o.apply_A(g, gpu.xresult(), 0, 1)
+    assert np.isclose(np.linalg.norm(r) / np.linalg.norm(b), info["err"], rtol=1e-6)
+    gpu.close()

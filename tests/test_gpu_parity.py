@@ -258,3 +258,31 @@ def test_c1_configuration_steps_bitwise():
         assert fields_equal(gpu, sim) == [], s
     print(f"128x64x4: step 10 took {a['iters']} iterations, Cd {a['cd']:.12f}")
     gpu.close()
+
+
+@pytest.mark.parametrize("bc,solver,precond", [("cylinder", 0, 1), ("flowpassing", 1, 0)])
+def test_steady_check_bitwise(bc, solver, precond):
+    """steady_check = 1: every step also reports max(T0 - T3) over the projected faces
+    (CheckSteadyStatebyMaxVal_omp), equal to the oracle's; the fields are unchanged by
+    the extra reads."""
+    gpu, sim, o, g = make_pair(40, 20, 6, bc=bc, solver=solver, precond=precond, tol=1e-6,
+                               maxit=400, steady_check=1)
+    for s in range(4):
+        a = gpu.step(1)
+        b = sim.step()
+        assert a["maxdiff"] == b["maxdiff"], (s, a["maxdiff"], b["maxdiff"])
+        for key in ("iters", "err", "cd", "cl", "cfl"):
+            assert _same(a[key], b[key]), (s, key, a, b)
+        assert fields_equal(gpu, sim) == [], s
+    assert max(a["maxdiff"]) > 0.0
+    gpu.close()
+
+
+def test_unconverged_solves_are_counted():
+    """An inconsistent system (zero-gradient outlet, no flux correction) at tol 1e-10
+    stops at the iteration limit: err stays above tol and each such solve is counted."""
+    gpu, sim, o, g = make_pair(24, 12, 4, bc="flowpassing", tol=1e-10, maxit=30)
+    infos = [gpu.step(1) for _ in range(3)]
+    assert infos[0]["solved"] == 0 and infos[0]["unconverged"] == 0
+    assert infos[2]["err"] >= 1e-10 and infos[2]["unconverged"] == 2
+    gpu.close()

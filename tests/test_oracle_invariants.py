@@ -463,3 +463,27 @@ def test_initial_field_splitmix64(o):
     inner = u.reshape(g.shape())[2:-2, 2:-2, 2:-2]
     assert np.all(np.abs(inner - 1.0) <= 1e-3) and np.all(v == 0) and np.all(w == 0)
     assert abs(inner.mean() - 1.0) < 2e-4
+
+
+def test_steady_check_over_the_projected_faces(o):
+    """The per-step steady-state check covers the faces the projection writes: u for
+    i < nx-3, v for j < ny-3, w for k < nz-3 (every k with periodic z); a brute-force
+    maximum over those index ranges (starting at 0, as the reference's loop does)."""
+    g = o.grid(12, 10, 6)
+    rng = np.random.default_rng(9)
+    f = [rng.standard_normal(g.G) for _ in range(6)]
+    shape = g.shape()
+    nx, ny, nz = g.nx, g.ny, g.nz
+    for zper in (0, 1):
+        d = [(f[c] - f[c + 3]).reshape(shape) for c in range(3)]
+        want = (max(0.0, d[0][2:nx - 3, 2:-2, 2:-2].max()),
+                max(0.0, d[1][2:-2, 2:ny - 3, 2:-2].max()),
+                max(0.0, d[2][2:-2, 2:-2, 2:(nz - 2 if zper else nz - 3)].max()))
+        assert o.maxdiff_faces(g, *f, z_periodic=zper) == want
+    # a change only on a skipped face does not count
+    u3 = f[0].copy().reshape(shape)
+    u0 = u3.copy()
+    u0[nx - 3, 4, 4] += 5.0
+    z = np.zeros(g.G)
+    assert o.maxdiff_faces(g, u0.ravel(), z, z, u3.ravel(), z, z) == (0.0, 0.0, 0.0)
+    assert o.maxdiff(g, u0.ravel(), z, z, u3.ravel(), z, z)[0] == 5.0

@@ -161,3 +161,24 @@ def test_time_loop(oracle, c):
         bad = [nm for nm in names
                if not np.array_equal(sim.field(nm), r.field(nm), equal_nan=True)]
         assert bad == [], (s, bad)
+
+
+@pytest.mark.parametrize("gd", GRIDS, ids=["uniform", "sml"])
+def test_steady_state_check(oracle, gd):
+    """CheckSteadyStatebyMaxVal_omp (CheckSteadystate.hpp:247-307): the signed maximum
+    of T0 - T3 per component over the interior cells, starting at 0.  The reference
+    prints it; the harness reads its line back at 17 digits, so the values compare
+    exactly."""
+    n, L, kind = gd["n"], gd["L"], gd["kind"]
+    g = oracle.grid(*n, *L, kind=kind)
+    r = op.RefHarness(*n, *L, sml=kind)
+    rng = np.random.default_rng(4)
+    fields = [1.0 + 0.1 * rng.standard_normal(g.G) for _ in range(6)]
+    # make w one-sided (T0 below T3 everywhere) so that its maximum is the starting 0
+    fields[5] = fields[2] + 1.0
+    for nm, a in zip(("u0", "v0", "w0", "u3", "v3", "w3"), fields):
+        r.field(nm)[:] = a
+    want = r.maxdiff()
+    got = oracle.maxdiff(g, *fields)
+    assert got == want, (got, want)
+    assert want[2] == 0.0 and want[0] > 0.0 and want[1] > 0.0
