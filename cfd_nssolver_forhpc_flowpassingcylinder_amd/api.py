@@ -35,7 +35,7 @@ EXPORTS = [
     "cfdmx_profile_read", "cfdmx_set_tolerance", "cfdmx_set_physics", "cfdmx_upload_force",
     "cfdmx_drag_objective", "cfdmx_adjoint_drag", "cfdmx_plot3d_q", "cfdmx_plot3d_flush",
     "cfdmx_plot3d_x", "cfdmx_checkpoint_save", "cfdmx_checkpoint_load", "cfdmx_smagorinsky",
-    "cfdmx_download_viseff",
+    "cfdmx_download_viseff", "cfdmx_cd_cl_noalloc",
 ]
 
 # kernel classes of cfdmx_profile_read
@@ -124,6 +124,7 @@ def load_library(path=LIB_PATH):
     L.cfdmx_solve_bicgstab.argtypes = [vp, C.POINTER(C.c_int), DP]
     L.cfdmx_solve_rbsor.argtypes = [vp, C.POINTER(C.c_int), DP]
     L.cfdmx_cd_cl.argtypes = [vp, DP, DP]
+    L.cfdmx_cd_cl_noalloc.argtypes = [vp, vp, vp]
     L.cfdmx_cfl.argtypes = [vp, DP]
     L.cfdmx_apply_operator.argtypes = [vp, DP, DP]
     L.cfdmx_step.argtypes = [vp, C.c_int, C.POINTER(StepInfo)]
@@ -335,6 +336,13 @@ class FlowSolver:
         cd, cl = C.c_double(), C.c_double()
         self._chk(self.L.cfdmx_cd_cl(self.h, C.byref(cd), C.byref(cl)), "cd_cl")
         return cd.value, cl.value
+
+    def cd_cl_noalloc(self, f_dev_ptr, cdcl_dev_ptr):
+        """Cd, Cl of a force field already in device memory (address f_dev_ptr) into two
+        doubles of device memory at cdcl_dev_ptr, on the context's stream, without any
+        allocation; call synchronize() before reading the result on another stream."""
+        self._chk(self.L.cfdmx_cd_cl_noalloc(self.h, C.c_void_p(f_dev_ptr),
+                                             C.c_void_p(cdcl_dev_ptr)), "cd_cl_noalloc")
 
     def cfl(self):
         c = C.c_double()

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <utility>
@@ -106,6 +107,10 @@ struct cfdmx_ctx {
         int id;
         hipEvent_t a, b;
     };
+    // this context's device and that device's address of the file-scope scratch array
+    // of cfdmx_cd_cl_noalloc
+    int device = -1;
+    double *scratch_dev = nullptr;
     int prof_on = 0;
     double prof_ms[CFDMX_K_COUNT] = {};
     long long prof_n[CFDMX_K_COUNT] = {};
@@ -114,6 +119,20 @@ struct cfdmx_ctx {
 };
 
 static thread_local std::string g_create_error;
+
+// ---------------------------------------------------------------------------
+// Scratch memory of cfdmx_cd_cl_noalloc: the block partials of the drag/lift
+// reduction (two sums per 1024-cell block) in a file-scope device array instead of
+// context memory.  Every GPU holds its own copy for the life of the process; host code
+// uses only the per-device address that hipGetSymbolAddress returns (kept in
+// g_scratch at the first context creation on that device), never the array's name.
+// Calls on one GPU share the array, so calls that overlap on one GPU race.
+// ---------------------------------------------------------------------------
+constexpr int kScratchBlocks = 65536; // 64 Mi interior cells: 1024 x 512 x 128
+static __device__ __align__(256) double scratch[2 * kScratchBlocks];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch[kMaxDevices] = {};
 
 #define FAIL(code, ...)                                                                        \
     do {                                                                                       \
@@ -799,6 +818,26 @@ int cfdmx_destroy(cfdmx_ctx *ctx)
     return CFDMX_OK;
 }
 
+// the scratch array's address on the current device, looked up once per device
+static int scratch_setup(cfdmx_ctx *ctx)
+{
+    int dev = -1;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices)
+        FAIL(CFDMX_ERR_HIP, "device %d outside the %d devices the library tracks", dev,
+             kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void *addr = nullptr;
+        HIPCHK(hipGetSymbolAddress(&addr, HIP_SYMBOL(scratch)));
+        if (!addr) FAIL(CFDMX_ERR_HIP, "hipGetSymbolAddress returned no address for scratch");
+        g_scratch[dev] = static_cast<double *>(addr);
+    }
+    ctx->device = dev;
+    ctx->scratch_dev = g_scratch[dev];
+    return CFDMX_OK;
+}
+
 int cfdmx_create(const cfdmx_config *cfg, cfdmx_ctx **out)
 {
     if (!cfg || !out) {
@@ -818,6 +857,7 @@ int cfdmx_create(const cfdmx_config *cfg, cfdmx_ctx **out)
         }
     }
     if (!rc) rc = setup_device(ctx);
+    if (!rc) rc = scratch_setup(ctx);
     if (rc) {
         g_create_error = ctx->err;
         cfdmx_destroy(ctx);
@@ -1391,6 +1431,23 @@ int cfdmx_cd_cl(cfdmx_ctx *ctx, double *cd, double *cl)
     if (rc) return rc;
     if (cd) *cd = ctx->scal_h[1];
     if (cl) *cl = ctx->scal_h[2];
+    return CFDMX_OK;
+}
+
+int cfdmx_cd_cl_noalloc(cfdmx_ctx *ctx, const double *f_dev, double *cdcl_dev)
+{
+    if (!ctx) return CFDMX_ERR_ARG;
+    if (!f_dev || !cdcl_dev) FAIL(CFDMX_ERR_ARG, "cfdmx_cd_cl_noalloc: null device pointer");
+    if (ctx->g.nblocks > kScratchBlocks)
+        FAIL(CFDMX_ERR_ARG,
+             "cfdmx_cd_cl_noalloc: %d interior cells exceed the scratch limit of %d "
+             "(%d reduction blocks of 1024 cells)",
+             ctx->g.N, kScratchBlocks * 1024, kScratchBlocks);
+    if (!ctx->scratch_dev) FAIL(CFDMX_ERR_STATE, "cfdmx_cd_cl_noalloc: no scratch address");
+    // every block writes its two partials, so nothing left by an earlier call is read
+    launch_force_partial(ctx->g, ctx->stream, f_dev, ctx->scratch_dev);
+    launch_cdcl_fin(ctx->g, ctx->stream, ctx->scratch_dev, ctx->cfg.lz, cdcl_dev);
+    LAUNCHCHK();
     return CFDMX_OK;
 }
 

@@ -235,6 +235,16 @@ CFDMX_API int cfdmx_project_ib(cfdmx_ctx *ctx);
 CFDMX_API int cfdmx_swap_t0_t3(cfdmx_ctx *ctx);
 /* CD_CL, virtualForceIntergrator.hpp:92-111 */
 CFDMX_API int cfdmx_cd_cl(cfdmx_ctx *ctx, double *cd, double *cl);
+/* CD_CL as above, for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  No reference counterpart.
+ * Reads a caller's force field f_dev (device memory, 3*G doubles in the DfibArray::f
+ * layout) and writes cd, cl to cdcl_dev[0..1] (device memory), ordered on the context's
+ * stream; it neither synchronises nor allocates: the reduction's block partials live in
+ * a file-scope device array (1 MiB per GPU, held until the process ends), which limits
+ * the grid to 64 Mi interior cells (CFDMX_ERR_ARG beyond).  Every call on one GPU shares
+ * that array: calls that overlap on one GPU (contexts on different streams) race.  The
+ * result equals cfdmx_cd_cl's on the same field bit for bit. */
+CFDMX_API int cfdmx_cd_cl_noalloc(cfdmx_ctx *ctx, const double *f_dev, double *cdcl_dev);
 /* get_Cfl, cfl.hpp:3-79, on T1 */
 CFDMX_API int cfdmx_cfl(cfdmx_ctx *ctx, double *cfl);
 
