@@ -41,8 +41,12 @@ BYTES_PER_CELL = {
     "convdiff": 48,  # r u, v, w;  w u*, v*, w*
     "rhs": 32,       # r u*, v*, w*;  w b
     "project": 96,   # r u*, v*, w*, X, eta;  w u, v, w, p, f(3)
-    "sor_half": 24,  # r p (all), rhs (half);  w p (half)
+    "sor_half": 16,  # colour-split half sweep: r p (other colour), p, rhs (own);  w p (own)
 }
+
+# the BiCGSTAB iteration (phases A + B + C) is the FP64 step's dominant work: 136 B/cell
+# per iteration (40 + 24 + 72)
+ITERATION_PHASES = ("bicg_A", "bicg_B", "bicg_C")
 
 MIXED_INNER_TOL = 0.0  # the library's per-mode default: 3e-3 mixed, 1e-5 FP64 multigrid
 
@@ -63,7 +67,8 @@ WORKLOADS = {
 }
 
 
-def make_solver(wl, precond, solver, seed, maxit=5000, precision=0, smag_cs=0.0):
+def make_solver(wl, precond, solver, seed, maxit=5000, precision=0, smag_cs=0.0,
+                inner_tol=None):
     import cfd_nssolver_forhpc_flowpassingcylinder_amd as pkg
     from cfd_nssolver_forhpc_flowpassingcylinder_amd.api import (PRECOND_JACOBI, PRECOND_NONE,
                                                                   SOLVER_BICGSTAB, SOLVER_RBSOR)
@@ -76,8 +81,8 @@ def make_solver(wl, precond, solver, seed, maxit=5000, precision=0, smag_cs=0.0)
         solver=SOLVER_RBSOR if solver == "rbsor" else SOLVER_BICGSTAB, sor_omega=1.8,
         bc_num=slip_num, bc_dir=slip_dir, z_periodic=1, outflow_convective=1,
         convective_speed=1.0, flux_correction=1, skip_first_pressure=1,
-        precision=precision, inner_tol=MIXED_INNER_TOL, mg_graphs=MG_GRAPHS,
-        smagorinsky_cs=smag_cs)
+        precision=precision, inner_tol=MIXED_INNER_TOL if inner_tol is None else inner_tol,
+        mg_graphs=MG_GRAPHS, smagorinsky_cs=smag_cs)
     s = pkg.FlowSolver(cfg)
     s.init_flow(1.0, 1e-3, seed)
     s.set_eta_cylinder_z(5.0, 5.0, 0.5, 100)
@@ -157,65 +162,136 @@ def measured_traffic(workload, kernel):
     return None
 
 
-def cpu_baseline(wl, iters_per_step, budget_s):
-    """The reference's CPU path (oracle/_ref) on this host: one pass of every explicit
-    stage at the full grid, plus a BiCGSTAB solve capped at a few iterations; the
-    per-step cost is t_explicit + iters_per_step * t_iteration."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import numpy as np
-    import oracle_py as op
-
-    if not op.ref_available():
-        return None
+def box_cores():
+    """Threads for the reference's OpenMP run: the host cores this process may use, capped
+    by OMP_NUM_THREADS when that is set (the GPU box gives the process of its one GPU a
+    16-core share and sets OMP_NUM_THREADS to it)."""
     try:
         cores = len(os.sched_getaffinity(0))
     except AttributeError:
         cores = os.cpu_count() or 1
-    cores = max(1, min(cores, 16))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        cores = min(cores, int(env))
+    return max(1, cores)
+
+
+def _ref_timings(wl, threads, iters_per_step, whole_step):
+    """The reference's stages (oracle/_ref) at the workload's grid on `threads` OpenMP
+    threads: one pass of each explicit stage, the BiCGSTAB iteration cost from solves
+    capped at 2 and 5 iterations, and (whole_step) one whole loop body of runSeqOmp
+    (ref_step) with its solve capped at 2 iterations."""
+    import numpy as np
+    import oracle_py as op
     nx, ny, nz = wl["nx"], wl["ny"], wl["nz"]
     r = op.RefHarness(nx, ny, nz, 20.0, 10.0, 0.5, sml=0, re=wl["re"], dt=wl["dt"],
-                      precond=0, tol=1e-300, threads=cores)
+                      precond=0, tol=1e-300, threads=threads)
     u0 = r.field("u0")
-    rng = np.random.default_rng(1)
-    u0[:] = 1.0 + 1e-3 * (2.0 * rng.random(u0.size) - 1.0)
+    u0[:] = 1.0 + 1e-3 * (2.0 * np.random.default_rng(1).random(u0.size) - 1.0)
     r.call("eta")
     r.call("start")
-    t = time.perf_counter()
-    r.call("convdiff")
-    r.call("rhs")
-    t_rhs = time.perf_counter()
-    r.call("transform")
-    r.call("update")
-    r.call("copyvel")
-    r.call("bc_main")
-    r.call("bc_copy")
-    r.cdcl()
-    t_explicit = time.perf_counter() - t
-    # capped solves: k+1 iterations each (the solver breaks after the iteration limit)
-    timings = {}
-    for k in (1, 4):
+    ph = {}
+
+    def timed(name, *calls):
+        t = time.perf_counter()
+        for c in calls:
+            c()
+        ph[name] = ph.get(name, 0.0) + time.perf_counter() - t
+
+    timed("conv_diff", lambda: r.call("convdiff"))
+    timed("rhs", lambda: r.call("rhs"))
+    timed("projection_ib", lambda: r.call("transform"), lambda: r.call("update"))
+    timed("copy_vel", lambda: r.call("copyvel"))
+    timed("bc", lambda: r.call("bc_main"), lambda: r.call("bc_copy"))
+    timed("cd_cl", r.cdcl)
+    sol = {}
+    for k in (1, 4):  # the solver stops after k + 1 iterations
         r.set_maxit(k)
         t = time.perf_counter()
         r.solve()
-        timings[k] = time.perf_counter() - t
-        if time.perf_counter() - t_rhs > budget_s:
-            break
-    if 4 in timings:
-        t_iter = (timings[4] - timings[1]) / 3.0
+        sol[k] = time.perf_counter() - t
+    t_iter = (sol[4] - sol[1]) / 3.0
+    out = {"threads": threads,
+           "phase_s": {k: round(v, 4) for k, v in ph.items()},
+           "bicgstab_iteration_s": round(t_iter, 5)}
+    out["phase_s"]["poisson"] = round(iters_per_step * t_iter + ph["rhs"], 4)
+    out["explicit_s"] = round(sum(ph.values()), 4)
+    if whole_step:
+        r.set_maxit(1)
+        t = time.perf_counter()
+        r.step(2)  # loop 2: the pressure solve runs
+        out["whole_step_2_iterations_s"] = round(time.perf_counter() - t, 4)
+        out["step_s"] = out["whole_step_2_iterations_s"] + (iters_per_step - 2.0) * t_iter
     else:
-        t_iter = timings[1] / 2.0
-    step_s = t_explicit + iters_per_step * t_iter
+        out["step_s"] = out["explicit_s"] + iters_per_step * t_iter
+    del r
+    return out
+
+
+def _ref_whole_steps_c1(threads, nsteps, budget_s):
+    """Whole, uncapped loop bodies of runSeqOmp (ref_step) at the CPU-runnable grid,
+    128x64x4, Re 100, dt 0.002, BiCGSTAB on the Jacobi rows at tol 1e-8: the reference's
+    own FLOWpassing boundary table (it has no convective, slip or periodic faces)."""
+    import numpy as np
+    import oracle_py as op
+    r = op.RefHarness(128, 64, 4, 20.0, 10.0, 0.5, sml=0, re=100.0, dt=0.002, precond=1,
+                      tol=1e-8, threads=threads)
+    u0 = r.field("u0")
+    u0[:] = 1.0 + 1e-3 * (2.0 * np.random.default_rng(1).random(u0.size) - 1.0)
+    r.call("eta")
+    r.call("start")
+    iters, t0 = [], time.perf_counter()
+    for q in range(nsteps):
+        iters.append(r.step(q + 1)["iters"])
+        if time.perf_counter() - t0 > budget_s:
+            break
+    wall = time.perf_counter() - t0
+    n = len(iters)
+    return {"value": round(128 * 64 * 4 * n / wall / 1e6, 4), "unit": "Mcell-steps/s",
+            "steps": n, "iters_per_step": iters, "wall_s": round(wall, 3), "threads": threads,
+            "note": ("whole reference steps, measured (no model): the first steps of the "
+                     "128x64x4 case with the reference's own FLOWpassing table, so its "
+                     "iteration counts are the reference's, not the GPU run's")}
+
+
+def cpu_baseline(wl, iters_per_step, budget_s):
+    """The reference's CPU path (oracle/_ref) on this host at the workload's grid, on all
+    the cores this process may use and on one thread.  A whole step at the GPU's
+    iteration count is far beyond the time budget (minutes per step), so the per-step
+    cost is a measured whole loop body with its solve capped at 2 iterations plus the
+    remaining iterations at the measured per-iteration cost; the per-phase breakdown
+    comes from timing each stage.  Whole uncapped steps are measured at 128x64x4."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_py as op
+
+    if not op.ref_available():
+        return None
+    cores = box_cores()
+    nx, ny, nz = wl["nx"], wl["ny"], wl["nz"]
     n = nx * ny * nz
+    main = _ref_timings(wl, cores, iters_per_step, True)
+    one = _ref_timings(wl, 1, iters_per_step, False)
+    c1 = _ref_whole_steps_c1(cores, 5, budget_s)
     return {
-        "value": n / step_s / 1e6,
+        "value": n / main["step_s"] / 1e6,
         "unit": "Mcell-steps/s",
         "cores": cores,
         "kind": "reference",
         "sample": (f"reference CPU path (oracle/_ref, OpenMP {cores} threads) at "
-                   f"{nx}x{ny}x{nz}: one pass of conv-diff, RHS, transform, projection, "
-                   f"copy, BCs and Cd/Cl ({t_explicit:.2f} s) plus BiCGSTAB solves capped at "
-                   f"2 and 5 iterations ({t_iter:.3f} s per iteration); per-step cost = "
-                   f"explicit + {iters_per_step:.1f} iterations (the GPU's mean per timed step)"),
+                   f"{nx}x{ny}x{nz}: one whole runSeqOmp loop body with its BiCGSTAB solve "
+                   f"capped at 2 iterations ({main['whole_step_2_iterations_s']:.2f} s) plus "
+                   f"{iters_per_step - 2:.1f} more iterations at the measured "
+                   f"{main['bicgstab_iteration_s']:.4f} s each (from solves capped at 2 and 5 "
+                   "iterations); the GPU's mean iterations per timed step"),
+        "per_phase_s": main["phase_s"],
+        "one_thread": {"value": n / one["step_s"] / 1e6, "unit": "Mcell-steps/s",
+                       "per_phase_s": one["phase_s"],
+                       "bicgstab_iteration_s": one["bicgstab_iteration_s"],
+                       "note": "each explicit stage once plus the iterations at their "
+                               "measured cost (no whole step at one thread)"},
+        "whole_steps_128x64x4": c1,
+        "cores_note": ("all cores this process may use, capped by OMP_NUM_THREADS (16 on the "
+                       "GPU box: the CPU share of its one GPU)"),
     }
 
 
@@ -263,9 +339,14 @@ SLIP_DIR = [[1, 0, 0], [0, 0, 0], [0, 0, 0], [0, 0, 0], [0, 0, 0], [0, 0, 0]]
 # than the h * dJ/dR that central differences with h = 1e-6 resolve
 GRAD_CHECK_TOL = 1e-13
 
+# the smoothed mask's sine ramp is 1.5 cells wide in all: half-width 0.75 cells on
+# either side of the circle
+MASK_HALF_WIDTH_CELLS = 0.75
+
 
 def case_solver(a, seed, precision=0, tol=None, inflow_amp=0.0):
-    """A solver of case a (cylinder BCs, smoothed mask 1.5 cells wide at (5, 5), R = 0.5)
+    """A solver of case a (cylinder BCs, smoothed mask at (5, 5), R = 0.5, its ramp 1.5 cells
+    wide)
     and its inflow profile (1 + inflow_amp * seeded noise)."""
     import numpy as np
     import cfd_nssolver_forhpc_flowpassingcylinder_amd as pkg
@@ -283,7 +364,7 @@ def case_solver(a, seed, precision=0, tol=None, inflow_amp=0.0):
         prof[2:-2] += inflow_amp * (2.0 * rng.random(a["ny"]) - 1.0)
         s.set_inflow(prof)
     s.init_flow(1.0, 1e-3, seed)
-    s.set_eta_smoothed(5.0, 5.0, 0.5, 1.5 * 20.0 / a["nx"])
+    s.set_eta_smoothed(5.0, 5.0, 0.5, MASK_HALF_WIDTH_CELLS * 20.0 / a["nx"])
     return s, prof
 
 
@@ -296,7 +377,7 @@ def gradient_check(a, seed, nsteps, K, params, inflow_amp=0.0, h=1e-6, extra_h=(
     zero, where the update is not differentiable) lies between p - h and p + h."""
     import numpy as np
     s, prof = case_solver(a, seed, precision=2, tol=GRAD_CHECK_TOL, inflow_amp=inflow_amp)
-    eps = 1.5 * 20.0 / a["nx"]
+    eps = MASK_HALF_WIDTH_CELLS * 20.0 / a["nx"]
     J, gi, gm, info = s.adjoint_drag(nsteps, K)
     mask = [5.0, 5.0, 0.5]
     out = {"tol": GRAD_CHECK_TOL, "h": h, "nsteps": nsteps, "checkpoint_every": K, "J": J}
@@ -362,7 +443,7 @@ def adjoint_run(seed, fd=True):
         "config": (f"{a['nx']}x{a['ny']}x{a['nz']}, Re={a['re']:g}, dt={a['dt']:g}, "
                    f"{n} steps, checkpoint every {a['K']}, tol {a['tol']:g}, FP64; objective "
                    "mean Cd; parameters: 128-point inflow profile, cylinder (cx, cy, R) of a "
-                   "smoothed mask 1.5 cells wide"),
+                   "smoothed mask whose ramp is 1.5 cells wide (half-width 0.75 cells)"),
         "J": J, "J_plain_forward": J0,
         "grad_mask": {"cx": gm[0], "cy": gm[1], "R": gm[2]},
         "grad_inflow_l2": float(np.linalg.norm(gi)),
@@ -445,7 +526,7 @@ def large_run(seed, nsteps=None, fd=True):
         return wall, infos, u, pr
 
     out = {"config": (f"{a['nx']}x{a['ny']}x{a['nz']} uniform over 20x10x0.5, Re={a['re']:g}, "
-                      f"dt={a['dt']:g}, {n} steps, smoothed mask (1.5 cells) at (5, 5) R=0.5, "
+                      f"dt={a['dt']:g}, {n} steps, smoothed mask (ramp 1.5 cells wide) at (5, 5) R=0.5, "
                       f"convective outflow, slip y walls, periodic z, tol {a['tol']:g}; "
                       f"adjoint of mean Cd w.r.t. R, checkpoint every {a['K']}"),
            "hbm_total_bytes": total}
@@ -496,6 +577,130 @@ def large_run(seed, nsteps=None, fd=True):
     sa.close()
     if fd:
         out["gradient_check"] = gradient_check(a, seed, n, a["K"], ["R"], extra_h=(1e-7, 1e-8))
+    return out
+
+
+def rbsor_run(wl, args):
+    """The metric's grid with the second pressure solver: red-black SOR on SorPipeLine's
+    coefficients (omega 1.8, tol 1e-10 on max |dp| per sweep), sweeps per step capped at
+    args.sor_maxit (the reference's p_sor_iter_max is 5000).  W warm-up and K timed steps
+    (wall clock), then one profiled step for the half-sweep kernel's rate."""
+    s = make_solver(wl, "none", "rbsor", args.seed, args.sor_maxit, 0)
+    for q in range(args.warmup):
+        s.step(1)
+    s.synchronize()
+    infos = []
+    t = time.perf_counter()
+    for q in range(args.steps):
+        infos.append(s.step(1))
+        print(f"RB-SOR timed step {q + 1}: {infos[-1]['iters'] + 1} sweeps, max |dp| "
+              f"{infos[-1]['err']:.3e}", file=sys.stderr, flush=True)
+    s.synchronize()
+    wall = time.perf_counter() - t
+    s.profile(True)
+    s.step(1)
+    s.synchronize()
+    prof = s.profile_read()
+    s.profile(False)
+    s.close()
+    n_cells = wl["nx"] * wl["ny"] * wl["nz"]
+    ms, launches = prof["sor_half"]
+    avg_s = ms / 1e3 / launches
+    achieved = BYTES_PER_CELL["sor_half"] * n_cells / avg_s / 1e9
+    sweeps = [i["iters"] + 1 for i in infos]
+    return {
+        "value": round(n_cells * args.steps / wall / 1e6, 4), "unit": "Mcell-steps/s",
+        "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "config": (f"{wl['nx']}x{wl['ny']}x{wl['nz']}, Re={wl['re']:g}, dt={wl['dt']:g}, "
+                   f"red-black SOR omega 1.8, tol {wl['tol']:g} (max |dp| per sweep), at most "
+                   f"{args.sor_maxit} sweeps per step, FP64, colour-split storage"),
+        "sweep_cap": args.sor_maxit,
+        "sweeps_per_step": sweeps, "max_dp_per_step": [i["err"] for i in infos],
+        "steps_reaching_tol": sum(1 for i in infos if i["err"] < wl["tol"]),
+        "ms_per_sweep": round(wall / sum(sweeps) * 1e3, 4),
+        "roofline": {"bound": "hbm", "kernel": "k_sor_rb (one colour's half sweep)",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "avg_launch_us": round(avg_s * 1e6, 2),
+                     "bytes_per_launch": BYTES_PER_CELL["sor_half"] * n_cells,
+                     "bytes_note": ("16 B per cell of the grid per half sweep: the other "
+                                    "colour read, own p and right-hand side read, own p "
+                                    "written (32 B/cell per full sweep)")},
+        "kernels": kernel_table(prof, n_cells, BYTES_PER_CELL),
+    }
+
+
+def c3_run(wl, args):
+    """The mixed-precision configuration at its full length: 512x256x64, Re 200, dt 0.001,
+    args.c3_steps (500) steps from the seeded field, FP32 inner BiCGSTAB to 1e-5 with FP64
+    refinement to 1e-10, against the same steps in the FP64 multigrid mode (the same
+    algorithm in FP64, converged to tol) and, unless --c3-no-fp64, the FP64 path (the
+    reference's unpreconditioned BiCGSTAB, 5000-iteration cap): the Cd/Cl series and the
+    final fields."""
+    import numpy as np
+    n = args.c3_steps
+    shp = (wl["nx"] + 4, wl["ny"] + 4, wl["nz"] + 4)
+    modes = [("mixed", 1, 1e-5), ("fp64_multigrid", 2, 1e-5)]
+    if not args.c3_no_fp64:
+        modes.append(("fp64", 0, None))
+    runs = {}
+    for name, precision, itol in modes:
+        sv = make_solver(wl, "none", "bicgstab", args.seed, 5000, precision, inner_tol=itol)
+        sv.synchronize()
+        cd, cl, its, errs = [], [], [], []
+        t = time.perf_counter()
+        for q in range(n):
+            info = sv.step(1)
+            cd.append(info["cd"])
+            cl.append(info["cl"])
+            its.append(info["iters"])
+            errs.append(info["err"])
+            if (q + 1) % 50 == 0:
+                print(f"C3 {name}: step {q + 1}, {info['iters']} iterations, residual "
+                      f"{info['err']:.2e}, cd {info['cd']:.8f}", file=sys.stderr, flush=True)
+        sv.synchronize()
+        wall = time.perf_counter() - t
+        u = sv.download_velocity(0)
+        p = sv.pressure().reshape(shp)[2:-2, 2:-2, 2:-2]
+        runs[name] = dict(wall=wall, cd=np.array(cd), cl=np.array(cl), its=its, errs=errs,
+                          u=u, p=p - p.mean(), unconverged=info["unconverged"])
+        sv.close()
+    n_cells = wl["nx"] * wl["ny"] * wl["nz"]
+
+    def rel(a, b):
+        return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+    out = {"config": (f"{wl['nx']}x{wl['ny']}x{wl['nz']}, Re={wl['re']:g}, dt={wl['dt']:g}, "
+                      f"{n} steps from the seeded field; mixed: FP32 inner BiCGSTAB with a "
+                      "multigrid preconditioner to 1e-5, FP64 refinement to 1e-10, FP64 dots "
+                      "and drag")}
+    for name, r in runs.items():
+        out[name] = {"value": round(n_cells * n / r["wall"] / 1e6, 4), "unit": "Mcell-steps/s",
+                     "ms_per_step": round(r["wall"] / n * 1e3, 3),
+                     "mean_iters": sum(r["its"]) / n,
+                     "steps_above_tol": sum(1 for e in r["errs"][1:] if not e < wl["tol"]),
+                     "cd_last": float(r["cd"][-1]), "cl_last": float(r["cl"][-1])}
+    m = runs["mixed"]
+    for ref in [k for k in runs if k != "mixed"]:
+        r = runs[ref]
+        unorm = float(np.linalg.norm(r["u"][0]))
+        out["mixed"][f"errors_vs_{ref}"] = {
+            "u": rel(m["u"][0], r["u"][0]),
+            "v": float(np.linalg.norm(m["u"][1] - r["u"][1])) / unorm,
+            "w": float(np.linalg.norm(m["u"][2] - r["u"][2])) / unorm,
+            "p": rel(m["p"], r["p"]),
+            "cd_series_max_rel": float(np.max(np.abs(m["cd"] - r["cd"]) / np.abs(r["cd"]))),
+            "cl_series_max_abs_over_max_cl": float(np.max(np.abs(m["cl"] - r["cl"]))
+                                                   / np.max(np.abs(r["cl"])))}
+    out["mixed"]["speedup_vs_fp64_multigrid"] = round(
+        runs["fp64_multigrid"]["wall"] / m["wall"], 3)
+    if "fp64" in runs:
+        out["mixed"]["speedup_vs_fp64"] = round(runs["fp64"]["wall"] / m["wall"], 3)
+    out["errors_note"] = ("relative L2 of the final fields (v and w relative to |u|, p with "
+                          "its mean removed) and the largest Cd / Cl difference over all "
+                          "steps; fp64 stops at its 5000-iteration cap short of tol on most "
+                          "steps, so its distance from the converged solution is in its "
+                          "column")
     return out
 
 
@@ -595,8 +800,18 @@ def main():
     ap.add_argument("--maxit", type=int, default=5000,
                     help="BiCGSTAB iteration cap (5000, the reference default; lower only for "
                          "counter-collection passes, which need few launches)")
+    ap.add_argument("--sor-maxit", type=int, default=5000,
+                    help="red-black SOR sweeps per step at most (the reference's "
+                         "p_sor_iter_max, 5000)")
+    ap.add_argument("--c3-steps", type=int, default=500,
+                    help="steps of --part c3 (the configuration's 500)")
+    ap.add_argument("--c3-no-fp64", action="store_true",
+                    help="--part c3: leave out the FP64 BiCGSTAB run (about 1 s per step)")
+    ap.add_argument("--no-rbsor", action="store_true",
+                    help="skip the red-black SOR run at the metric's grid")
     ap.add_argument("--part", default="all",
-                    choices=["all", "mixed", "fp64mg", "adjoint", "large", "plot3d", "les"],
+                    choices=["all", "mixed", "fp64mg", "adjoint", "large", "plot3d", "les",
+                             "rbsor", "c3"],
                     help="run only the mixed-precision or only the adjoint measurement and "
                          "print its JSON object (for profiling one part)")
     ap.add_argument("--large-steps", type=int, default=0,
@@ -642,6 +857,12 @@ def main():
     if args.part == "les":
         print(json.dumps({"les": les_run(wl, args)}), flush=True)
         return
+    if args.part == "rbsor":
+        print(json.dumps({"rbsor": rbsor_run(wl, args)}), flush=True)
+        return
+    if args.part == "c3":
+        print(json.dumps({"c3": c3_run(wl, args)}), flush=True)
+        return
     if args.part == "large":
         print(json.dumps({"large": large_run(args.seed, args.large_steps or None,
                                              not args.no_fd)}), flush=True)
@@ -660,16 +881,26 @@ def main():
     iters = [i["iters"] for i in infos]
     mean_iters = sum(iters) / max(1, len(iters))
 
-    # dominant kernel by device time during the profiled steps
+    # the dominant work by device time during the profiled steps: with BiCGSTAB the
+    # iteration, its three phase kernels together (136 B/cell per iteration); otherwise
+    # the kernel class with the most device time
     cands = {k: v for k, v in prof.items() if k in BYTES_PER_CELL and v[1] > 0}
-    dom = max(cands, key=lambda k: cands[k][0])
-    ms, launches = cands[dom]
-    avg_s = ms / 1e3 / launches
-    alg_bytes = BYTES_PER_CELL[dom] * n_cells
+    if all(k in cands for k in ITERATION_PHASES):
+        iters_run = min(cands[k][1] for k in ITERATION_PHASES)
+        avg_s = sum(cands[k][0] for k in ITERATION_PHASES) / 1e3 / iters_run
+        alg_bytes = sum(BYTES_PER_CELL[k] for k in ITERATION_PHASES) * n_cells
+        dom = "bicgstab_iteration (k_march<0> + k_march<1> + k_march_C)"
+        traffic = [measured_traffic(args.workload, k) for k in ITERATION_PHASES]
+        traffic = sum(traffic) if all(t is not None for t in traffic) else None
+    else:
+        k0 = max(cands, key=lambda k: cands[k][0])
+        avg_s = cands[k0][0] / 1e3 / cands[k0][1]
+        alg_bytes = BYTES_PER_CELL[k0] * n_cells
+        dom, traffic = k0, measured_traffic(args.workload, k0)
     achieved = alg_bytes / avg_s / 1e9
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": measured_traffic(args.workload, dom),
+                "traffic": traffic,
                 "avg_launch_us": round(avg_s * 1e6, 2),
                 "bytes_per_launch": alg_bytes}
     kernels = kernel_table(prof, n_cells, BYTES_PER_CELL)
@@ -737,6 +968,13 @@ def main():
             },
         }
 
+    rbsor = None
+    if not args.no_rbsor and args.solver == "bicgstab":
+        if s is not None:
+            s.close()
+            s = None
+        rbsor = rbsor_run(wl, args)
+
     adjoint = None
     if not args.no_adjoint and args.workload == "cylinder512":
         if s is not None:
@@ -780,6 +1018,7 @@ def main():
         "kernels": kernels,
         "stream_ceiling": ceiling,
         "mixed": mixed,
+        "rbsor": rbsor,
         "adjoint": adjoint,
     }
     print(json.dumps(line), flush=True)

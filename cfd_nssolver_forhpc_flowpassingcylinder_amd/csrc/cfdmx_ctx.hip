@@ -90,6 +90,7 @@ struct cfdmx_ctx {
     double *inflow = nullptr;
     KState *kst = nullptr;
     SorState *sst = nullptr;
+    SorCoef sorc = {}; // red-black SOR coefficients of a grid with constant spacings
     int *count = nullptr;
     // pinned host mirrors
     KState *kst_h = nullptr;
@@ -523,8 +524,36 @@ static void axis_tables(const std::vector<double> &D, const std::vector<double> 
     push(12, [&](int I) { return ok(I) ? 0.125 * Ds[I - 1] * Ds[I - 1] / D[I] : 0.0; });
 }
 
+static int reset_run_state(cfdmx_ctx *ctx);
+
+// SorPipeLine_seq's coefficients (SorPipeLine.seq.hpp:40-51), formed once when every
+// spacing table is a single value (the uniform grid): then each cell's six quotients
+// and -1/(their sum) are these values, evaluated with the same operations and order
+static void sor_constant_coefficients(cfdmx_ctx *ctx)
+{
+    auto constant = [](const std::vector<double> &a) {
+        for (double x : a)
+            if (x != a[0]) return false;
+        return true;
+    };
+    SorCoef &sc = ctx->sorc;
+    sc.constant = constant(ctx->Dx) && constant(ctx->Dy) && constant(ctx->Dz) &&
+                  constant(ctx->Dxs) && constant(ctx->Dys) && constant(ctx->Dzs);
+    if (!sc.constant) return;
+    const double Dx = ctx->Dx[0], Dy = ctx->Dy[0], Dz = ctx->Dz[0];
+    const double Dxs = ctx->Dxs[0], Dys = ctx->Dys[0], Dzs = ctx->Dzs[0];
+    sc.c[0] = Dy * Dz / Dxs;
+    sc.c[1] = Dy * Dz / Dxs;
+    sc.c[2] = Dx * Dz / Dys;
+    sc.c[3] = Dx * Dz / Dys;
+    sc.c[4] = Dx * Dy / Dzs;
+    sc.c[5] = Dx * Dy / Dzs;
+    sc.c[6] = -1.0 / (sc.c[0] + sc.c[1] + sc.c[2] + sc.c[3] + sc.c[4] + sc.c[5]);
+}
+
 static int setup_device(cfdmx_ctx *ctx)
 {
+    sor_constant_coefficients(ctx);
     cfdmx_config &c = ctx->cfg;
     Geo &g = ctx->g;
     g.nxc = c.nx;
@@ -667,20 +696,8 @@ static int setup_device(cfdmx_ctx *ctx)
     }
     launch_fill(s, (int)G, 0.0, ctx->p);
     launch_fill(s, (int)G, 0.0, ctx->eta);
-    launch_fill(s, (int)(3 * G), 0.0, ctx->f);
-    double *zs[] = {ctx->x, ctx->b, ctx->px1, ctx->px2, ctx->B.r0, ctx->B.r, ctx->B.as,
-                    ctx->B.p[0], ctx->B.p[1], ctx->B.ap[0], ctx->B.ap[1]};
-    for (double *z : zs) launch_fill(s, (int)N, 0.0, z);
-    launch_fill(s, 16, 0.0, ctx->scal);
     LAUNCHCHK();
-
-    KState ks = {};
-    ks.tol = c.tol;
-    ks.restart_factor = 1e-7; // pRestartFactor_ (bicgstabRe2.hpp:52)
-    ks.maxit = c.maxit;
-    ks.done = 1;
-    HIPCHK(hipMemcpyAsync(ctx->kst, &ks, sizeof ks, hipMemcpyHostToDevice, s));
-    HIPCHK(hipStreamSynchronize(s));
+    if (int rc = reset_run_state(ctx)) return rc;
 
     // boundary table
     BCParams &bc = ctx->bc;
@@ -690,6 +707,32 @@ static int setup_device(cfdmx_ctx *ctx)
     bc.convective = c.outflow_convective;
     bc.conv_coef = c.convective_speed * c.dt / ctx->Dx[g.nx - 3];
     bc.flux_correction = c.flux_correction;
+    return CFDMX_OK;
+}
+
+// the pressure solve's state of a new run: X_result and its history, the Krylov vectors,
+// the forcing and the solver states (a fresh bicgstabRe2, bicgstabRe2.hpp:17-52), the
+// counters of the step report
+static int reset_run_state(cfdmx_ctx *ctx)
+{
+    hipStream_t s = ctx->stream;
+    const size_t G = (size_t)ctx->g.G, N = (size_t)ctx->g.N;
+    launch_fill(s, (int)(3 * G), 0.0, ctx->f);
+    double *zs[] = {ctx->x, ctx->b, ctx->px1, ctx->px2, ctx->B.r0, ctx->B.r, ctx->B.as,
+                    ctx->B.p[0], ctx->B.p[1], ctx->B.ap[0], ctx->B.ap[1]};
+    for (double *z : zs) launch_fill(s, (int)N, 0.0, z);
+    launch_fill(s, 16, 0.0, ctx->scal);
+    LAUNCHCHK();
+    KState ks = {};
+    ks.tol = ctx->cfg.tol;
+    ks.restart_factor = 1e-7; // pRestartFactor_ (bicgstabRe2.hpp:52)
+    ks.maxit = ctx->cfg.maxit;
+    ks.done = 1;
+    *ctx->kst_h = ks;
+    HIPCHK(hipMemcpyAsync(ctx->kst, ctx->kst_h, sizeof ks, hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+    ctx->resets = 0;
+    ctx->unconverged = 0;
     return CFDMX_OK;
 }
 
@@ -1013,6 +1056,8 @@ int cfdmx_start(cfdmx_ctx *ctx) { return ctx ? start_impl(ctx) : CFDMX_ERR_ARG; 
 int cfdmx_init_flow(cfdmx_ctx *ctx, double u0, double amp, unsigned long long seed)
 {
     if (!ctx) return CFDMX_ERR_ARG;
+    // a new run: the solver starts cold, as runSeqOmp's freshly built solver does
+    if (int rc = reset_run_state(ctx)) return rc;
     launch_init_flow(ctx->g, ctx->stream, u0, amp, seed, ctx->t0[0], ctx->t0[1], ctx->t0[2],
                      ctx->p);
     LAUNCHCHK();
@@ -1357,16 +1402,32 @@ static int solve_rbsor_impl(cfdmx_ctx *ctx, int *iters, double *err)
     ss.itmax = ctx->cfg.maxit;
     *ctx->sst_h = ss;
     HIPCHK(hipMemcpyAsync(ctx->sst, ctx->sst_h, sizeof ss, hipMemcpyHostToDevice, s));
-    PROF(CFDMX_K_RHS, launch_sor_rhs(g, s, ctx->t1[0], ctx->t1[1], ctx->t1[2], ctx->cfg.dt,
-                                     ctx->b));
+    // even nzc: colour-split storage (the right-hand side in b, p of both colours in
+    // the Krylov vector r0, p's natural order written back at the end); odd nzc: the
+    // natural-order half sweep
+    const bool split = (g.nzc % 2) == 0;
+    double *pc = ctx->B.r0;
+    if (split)
+        PROF(CFDMX_K_RHS, launch_sor_split(g, s, ctx->t1[0], ctx->t1[1], ctx->t1[2],
+                                           ctx->cfg.dt, ctx->p, ctx->b, pc));
+    else
+        PROF(CFDMX_K_RHS, launch_sor_rhs(g, s, ctx->t1[0], ctx->t1[1], ctx->t1[2],
+                                         ctx->cfg.dt, ctx->b));
     LAUNCHCHK();
     int batch = 4;
     for (;;) {
         for (int q = 0; q < batch; ++q) {
-            for (int colour = 0; colour < 2; ++colour)
-                PROF(CFDMX_K_SOR, launch_sor_half(g, s, colour, ctx->cfg.sor_omega, ctx->b,
-                                                  ctx->p, ctx->part, ctx->sst));
-            launch_sor_fin(g, s, ctx->part, ctx->sst);
+            for (int colour = 0; colour < 2; ++colour) {
+                if (split)
+                    PROF(CFDMX_K_SOR, launch_sor_rb(g, s, colour, ctx->cfg.sor_omega,
+                                                    ctx->sorc, ctx->b, pc, ctx->p, ctx->part,
+                                                    ctx->sst));
+                else
+                    PROF(CFDMX_K_SOR, launch_sor_half(g, s, colour, ctx->cfg.sor_omega,
+                                                      ctx->b, ctx->p, ctx->part, ctx->sst));
+            }
+            if (split) launch_sor_rb_fin(g, s, ctx->part, ctx->sst);
+            else launch_sor_fin(g, s, ctx->part, ctx->sst);
         }
         LAUNCHCHK();
         HIPCHK(hipMemcpyAsync(ctx->sst_h, ctx->sst, sizeof ss, hipMemcpyDeviceToHost, s));
@@ -1374,6 +1435,10 @@ static int solve_rbsor_impl(cfdmx_ctx *ctx, int *iters, double *err)
         prof_harvest(ctx);
         if (ctx->sst_h->done) break;
         batch = batch < 64 ? batch * 2 : 64;
+    }
+    if (split) {
+        launch_sor_merge(g, s, pc, ctx->p);
+        LAUNCHCHK();
     }
     if (iters) *iters = ctx->sst_h->iters - 1; // SorPipeLine.seq.hpp:123
     if (err) *err = ctx->sst_h->residual;

@@ -32,6 +32,13 @@ struct KState {
     int restarts;
 };
 
+// SorPipeLine's six face coefficients and -1/(their sum), formed once on the host when
+// every metric table is a constant (constant = 1), else per cell in the kernel
+struct SorCoef {
+    double c[7];
+    int constant;
+};
+
 struct SorState {
     double residual;
     double tol;
@@ -147,6 +154,14 @@ void launch_sor_rhs(const Geo &g, hipStream_t s, const double *us, const double 
 void launch_sor_half(const Geo &g, hipStream_t s, int colour, double omega, const double *cf6,
                      double *p, double *part, SorState *st);
 void launch_sor_fin(const Geo &g, hipStream_t s, double *part, SorState *st);
+// red-black SOR on colour-split storage (even nzc only)
+int sor_rb_blocks(const Geo &g);
+void launch_sor_split(const Geo &g, hipStream_t s, const double *us, const double *vs,
+                      const double *ws, double dt, const double *p, double *cf6c, double *pc);
+void launch_sor_merge(const Geo &g, hipStream_t s, const double *pc, double *p);
+void launch_sor_rb(const Geo &g, hipStream_t s, int colour, double omega, const SorCoef &sc,
+                   const double *cf6c, double *pc, const double *p, double *part, SorState *st);
+void launch_sor_rb_fin(const Geo &g, hipStream_t s, double *part, SorState *st);
 // prev (T0 of the step) and dpart (3 * nblocks doubles) add the steady-state check;
 // both null leaves it out
 void launch_project(const Geo &g, hipStream_t s, int from_x, double dt, int convective,

@@ -1528,6 +1528,169 @@ k_sor_fin(const double *part, int nb, SorState *st)
 }
 
 // ---------------------------------------------------------------------------
+// Red-black SOR on colour-split storage (even nzc).  Colour c's cells of interior row
+// (i, j) are k = 2m + q with q = (i + j + c) & 1; they are stored at (i nyc + j) nzc/2
+// + m of that colour's half (c * N/2 + ...).  Every neighbour of a cell has the other
+// colour: its x and y neighbours are the other half's entries at the same m of rows
+// (i +- 1, j) and (i, j +- 1), its z neighbours are entries m - 1 + q and m + q of its
+// own row (wrapping with periodic z).  A half sweep reads the other colour and its own
+// colour's p and right-hand side and writes its own colour: whole cache lines of only
+// the values it uses (16 bytes per cell of the grid per half sweep).  Neighbours
+// outside the interior are the ghost p, fixed during the solve, as in
+// SorPipeLine_seq.  Coefficients and update are k_sor_half's, so results are the same.
+// ---------------------------------------------------------------------------
+struct RBIdx {
+    int i, j, k, m, q, row, ic;
+};
+
+__device__ __forceinline__ RBIdx rb_decode(const Geo &g, const FastDiv &dh, int colour, int idx)
+{
+    RBIdx r;
+    const int hz = g.nzc >> 1;
+    r.row = (int)fdiv((unsigned)idx, dh);
+    r.m = idx - r.row * hz;
+    r.i = (int)fdiv((unsigned)r.row, g.dy);
+    r.j = r.row - r.i * g.nyc;
+    r.q = (r.i + r.j + colour) & 1;
+    r.k = 2 * r.m + r.q;
+    r.ic = gidx(g, r.i + kGC, r.j + kGC, r.k + kGC);
+    return r;
+}
+
+// right-hand side cf6 (k_sor_rhs's) and p of both colours, from the natural order
+__global__ void __launch_bounds__(kThreads)
+k_sor_split(Geo g, FastDiv dh, const double *__restrict__ us, const double *__restrict__ vs,
+            const double *__restrict__ ws, double rdt, const double *__restrict__ pg,
+            double *__restrict__ cf6c, double *__restrict__ pc)
+{
+    const int half = g.N >> 1;
+    const int t = blockIdx.x * kThreads + threadIdx.x;
+    if (t >= 2 * half) return;
+    const int colour = t >= half ? 1 : 0, idx = t - colour * half;
+    const RBIdx r = rb_decode(g, dh, colour, idx);
+    const Axis &X = g.ax[0], &Y = g.ax[1], &Z = g.ax[2];
+    const int I = r.i + kGC, J = r.j + kGC, K = r.k + kGC, ic = r.ic;
+    const double mChange = (us[ic] - us[ic - g.sx]) * Y.D[J] * Z.D[K] +
+                           (vs[ic] - vs[ic - g.sy]) * X.D[I] * Z.D[K] +
+                           (ws[ic] - ws[ic - 1]) * X.D[I] * Y.D[J];
+    cf6c[t] = mChange * rdt;
+    pc[t] = pg[ic];
+}
+
+// p of both colours back into the natural order
+__global__ void __launch_bounds__(kThreads)
+k_sor_merge(Geo g, FastDiv dh, const double *__restrict__ pc, double *__restrict__ pg)
+{
+    const int half = g.N >> 1;
+    const int t = blockIdx.x * kThreads + threadIdx.x;
+    if (t >= 2 * half) return;
+    const int colour = t >= half ? 1 : 0, idx = t - colour * half;
+    const RBIdx r = rb_decode(g, dh, colour, idx);
+    pg[r.ic] = pc[t];
+}
+
+// one block updates kRBCells * 256 cells of its colour, the cells of a thread 256
+// apart (each load instruction of a wave covers consecutive entries)
+constexpr int kRBCells = 4;
+
+// CONSTC: every metric table is one constant (the uniform grid), so the six face
+// coefficients and -1/sum are the same for every cell; the host forms them once with the
+// same expressions (sc), which gives the same values as forming them per cell.
+template <bool CONSTC>
+__global__ void __launch_bounds__(kThreads)
+k_sor_rb(Geo g, FastDiv dh, int colour, double omega, SorCoef sc,
+         const double *__restrict__ cf6c, double *__restrict__ pc,
+         const double *__restrict__ pg, double *__restrict__ part, const SorState *st)
+{
+    if (st->done) return;
+    const int half = g.N >> 1, hz = g.nzc >> 1;
+    const int nb = (half + kRBCells * kThreads - 1) / (kRBCells * kThreads);
+    const int L = logical_block(g);
+    if (L >= nb) return;
+    const double *__restrict__ other = pc + (colour ^ 1) * half;
+    double *__restrict__ own = pc + colour * half;
+    const Axis &X = g.ax[0], &Y = g.ax[1], &Z = g.ax[2];
+    const int cyh = g.nyc * hz;
+    double mx = 0.0;
+#pragma unroll
+    for (int c = 0; c < kRBCells; ++c) {
+        const int idx = (L * kRBCells + c) * kThreads + threadIdx.x;
+        if (idx < half) {
+            const RBIdx r = rb_decode(g, dh, colour, idx);
+            const int I = r.i + kGC, J = r.j + kGC, K = r.k + kGC, ic = r.ic;
+            const int rb = r.row * hz;
+            const double pE = r.i < g.nxc - 1 ? other[idx + cyh] : pg[ic + g.sx];
+            const double pW = r.i > 0 ? other[idx - cyh] : pg[ic - g.sx];
+            const double pN = r.j < g.nyc - 1 ? other[idx + hz] : pg[ic + g.sy];
+            const double pS = r.j > 0 ? other[idx - hz] : pg[ic - g.sy];
+            const double pT = r.k < g.nzc - 1 ? other[idx + r.q]
+                                              : (g.zper ? other[rb] : pg[ic + 1]);
+            const double pB = r.k > 0 ? other[idx - 1 + r.q]
+                                      : (g.zper ? other[rb + hz - 1] : pg[ic - 1]);
+            const double p0 = own[idx];
+            const double rhs = cf6c[idx];
+            double cf0, cf1, cf2, cf3, cf4, cf5, cf7;
+            if (CONSTC) {
+                cf0 = sc.c[0], cf1 = sc.c[1], cf2 = sc.c[2], cf3 = sc.c[3];
+                cf4 = sc.c[4], cf5 = sc.c[5], cf7 = sc.c[6];
+            } else {
+                cf0 = divr(Y.D[J] * Z.D[K], X.Ds[I], X.rDs[I]);
+                cf1 = divr(Y.D[J] * Z.D[K], X.Ds[I - 1], X.rDs[I - 1]);
+                cf2 = divr(X.D[I] * Z.D[K], Y.Ds[J], Y.rDs[J]);
+                cf3 = divr(X.D[I] * Z.D[K], Y.Ds[J - 1], Y.rDs[J - 1]);
+                cf4 = divr(X.D[I] * Y.D[J], Z.Ds[K], Z.rDs[K]);
+                cf5 = divr(X.D[I] * Y.D[J], Z.Ds[K - 1], Z.rDs[K - 1]);
+                cf7 = -1.0 / (cf0 + cf1 + cf2 + cf3 + cf4 + cf5);
+            }
+            const double pNEW =
+                (-pE * cf0 - pW * cf1 - pN * cf2 - pS * cf3 - pT * cf4 - pB * cf5 + rhs) * cf7;
+            const double d = fabs(pNEW - p0);
+            mx = d > mx ? d : mx;
+            own[idx] = p0 + omega * (pNEW - p0);
+        }
+    }
+    __shared__ double red[4];
+    const double wm = wave_max(mx);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = wm;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = red[0];
+        for (int q = 1; q < 4; ++q) m = red[q] > m ? red[q] : m;
+        part[colour * nb + L] = m;
+    }
+}
+
+// end of a colour-split sweep: max |dp| over the 2 nb partials, with eight loads in
+// flight per thread
+__global__ void __launch_bounds__(kThreads)
+k_sor_rb_fin(const double *part, int nb2, SorState *st)
+{
+    if (st->done) return;
+    __shared__ double red[4];
+    double mx = 0.0;
+    for (int m0 = threadIdx.x; m0 < nb2; m0 += 8 * kThreads) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int m = m0 + u * kThreads;
+            v[u] = m < nb2 ? part[m] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) mx = v[u] > mx ? v[u] : mx;
+    }
+    const double wm = wave_max(mx);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = wm;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = red[0];
+        for (int q = 1; q < 4; ++q) m = red[q] > m ? red[q] : m;
+        st->residual = m;
+        st->iters += 1;
+        if (!(m > st->tol && st->iters < st->itmax)) st->done = 1;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // update_UandF_omp (updateUandF.hpp:90-180) fused with Pressure_transform_X_result
 // (Transform.hpp:8-25) and the drag/lift partial sums of virtualF_Int
 // (virtualForceIntergrator.hpp:79-86).  FROM_X: pressure read from X_result.
@@ -2050,6 +2213,45 @@ void launch_sor_half(const Geo &g, hipStream_t s, int colour, double omega, cons
 {
     hipLaunchKernelGGL(k_sor_half, dim3(grid_blocks(g)), dim3(kThreads), 0, s, g, colour, omega,
                        cf6, p, part, st);
+}
+
+// colour-split red-black SOR (even nzc): cf6c and pc hold N doubles each (red half,
+// then black); part 2 * rb_blocks(g) doubles
+int sor_rb_blocks(const Geo &g) { return cdiv(g.N >> 1, kRBCells * kThreads); }
+
+void launch_sor_split(const Geo &g, hipStream_t s, const double *us, const double *vs,
+                      const double *ws, double dt, const double *p, double *cf6c, double *pc)
+{
+    const FastDiv dh = make_fastdiv((unsigned)(g.nzc >> 1));
+    hipLaunchKernelGGL(k_sor_split, dim3(cdiv(g.N, kThreads)), dim3(kThreads), 0, s, g, dh, us,
+                       vs, ws, 1.0 / dt, p, cf6c, pc);
+}
+
+void launch_sor_merge(const Geo &g, hipStream_t s, const double *pc, double *p)
+{
+    const FastDiv dh = make_fastdiv((unsigned)(g.nzc >> 1));
+    hipLaunchKernelGGL(k_sor_merge, dim3(cdiv(g.N, kThreads)), dim3(kThreads), 0, s, g, dh, pc,
+                       p);
+}
+
+void launch_sor_rb(const Geo &g, hipStream_t s, int colour, double omega, const SorCoef &sc,
+                   const double *cf6c, double *pc, const double *p, double *part, SorState *st)
+{
+    const FastDiv dh = make_fastdiv((unsigned)(g.nzc >> 1));
+    const int nb = sor_rb_blocks(g);
+    const int grid = g.xcd_remap ? ((nb + 7) / 8) * 8 : nb;
+    if (sc.constant)
+        hipLaunchKernelGGL(k_sor_rb<true>, dim3(grid), dim3(kThreads), 0, s, g, dh, colour,
+                           omega, sc, cf6c + colour * (g.N >> 1), pc, p, part, st);
+    else
+        hipLaunchKernelGGL(k_sor_rb<false>, dim3(grid), dim3(kThreads), 0, s, g, dh, colour,
+                           omega, sc, cf6c + colour * (g.N >> 1), pc, p, part, st);
+}
+
+void launch_sor_rb_fin(const Geo &g, hipStream_t s, double *part, SorState *st)
+{
+    hipLaunchKernelGGL(k_sor_rb_fin, dim3(1), dim3(kThreads), 0, s, part, 2 * sor_rb_blocks(g),
+                       st);
 }
 
 void launch_sor_fin(const Geo &g, hipStream_t s, double *part, SorState *st)
