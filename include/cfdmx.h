@@ -185,7 +185,9 @@ CFDMX_API int cfdmx_checkpoint_load(cfdmx_ctx *ctx, const char *path);
 
 /* Seeded initial field on the device: u = u0 + amp*(2U-1) on interior cells with U
  * from splitmix64(seed ^ interior index), v = w = 0, p = 0; then the initial
- * BC_staggered_main + BC_staggered_copy (RUN_CPU.cpp:66-109). */
+ * BC_staggered_main + BC_staggered_copy (RUN_CPU.cpp:66-109).  It starts a new run: the
+ * forcing, X_result and its history and the solver state are reset as for a freshly
+ * built solver (bicgstabRe2.hpp:17-52, RUN_CPU.cpp:120). */
 CFDMX_API int cfdmx_init_flow(cfdmx_ctx *ctx, double u0, double amp, unsigned long long seed);
 /* Initial T1 = T3 = T0 copy + boundary conditions (RUN_CPU.cpp:104-109) for fields
  * uploaded by the caller. */

@@ -51,6 +51,7 @@ inline cfdmx_config &config()
     static cfdmx_config c = [] {
         cfdmx_config x;
         cfdmx_config_default(&x);
+        x.steady_check = 1; // CheckSteadyStatebyMaxVal_omp every step (RUN_CPU.cpp:380-384)
 #ifdef TERBULENCE_SMAGORINSKY
         x.smagorinsky_cs = 0.18; // SmagorinskyModel.hpp:38
 #endif
@@ -302,6 +303,19 @@ template <class Dfib, class Lo, class Grid> void DFIB_CylinderZ(Dfib &dfib, Lo &
     check(cfdmx_download_eta(c, dfib.eta.data()), "download eta");
 }
 
+// the stdout line and the Time_cDcL.dat row of CD_CL (virtualForceIntergrator.hpp:113-145):
+// the file in Information/ of the working directory, its Tecplot header on the first step
+template <class Simu> void cdcl_output(Simu &simu, double cd, double cl)
+{
+    std::cout << "[cD, cL] : " << cd << ", " << cl << std::endl;
+    (void)::mkdir("Information", 0755);
+    std::ofstream file("Information/Time_cDcL.dat", std::ios::out | std::ios::app);
+    if (simu.loop == 1)
+        file << "TITLE     = \"\"\nVARIABLES = \"simulation time\",\"C<sub>D</sub>\","
+                "\"C<sub>L</sub>\"\nZONE T=\"" << simu.ZONE() << "\"";
+    file << "\n" << simu.time << " " << cd << " " << cl;
+}
+
 // CD_CL, virtualForceIntergrator.hpp:92-148 (same stdout line and Time_cDcL.dat row)
 template <class Simu, class Lo, class Dfib, class Grid>
 bool CD_CL(Simu &simu, Lo &, Dfib &dfib, Grid &gA)
@@ -312,12 +326,7 @@ bool CD_CL(Simu &simu, Lo &, Dfib &dfib, Grid &gA)
     double cd = 0.0, cl = 0.0;
     check(cfdmx_cd_cl(c, &cd, &cl), "CD_CL");
     dfib.ValSum.assign({-cd * gA.lz / 2.0, -cl * gA.lz / 2.0, 0.0});
-    std::cout << "[cD, cL] : " << cd << ", " << cl << std::endl;
-    std::ofstream file("Information/Time_cDcL.dat", std::ios::out | std::ios::app);
-    if (simu.loop == 1)
-        file << "TITLE     = \"\"\nVARIABLES = \"simulation time\",\"C<sub>D</sub>\","
-                "\"C<sub>L</sub>\"\nZONE T=\"" << simu.ZONE() << "\"";
-    file << "\n" << simu.time << " " << cd << " " << cl;
+    cdcl_output(simu, cd, cl);
     return true;
 }
 
@@ -363,12 +372,14 @@ template <class Simu, class Grid> int OutputPlot3D_Xfile(Simu &simu, Grid &gA)
 }
 
 // runSeqOmp, RUN_CPU.cpp:9-440 / run/1_1_Sequential.hpp:5-10, with every field resident
-// on the device: the reference initial state (u = 1, v = w = p = 0), its boundary
-// conditions and mask, then the loop body per step; the per-step report goes to simu
-// (iters, error) and stdout as in the reference, and the Plot3D files are written at its
-// points: the grid and the initial field before the loop (RUN_CPU.cpp:84,241), then a q
-// file whenever simu.get_writefile() says so (RUN_CPU.cpp:411-413), staged on the device
-// while the next steps run.
+// on the device: the reference initial state (u = 1, v = w = p = 0, a cold solver), its
+// boundary conditions and mask, then the loop body per step; the per-step report goes to
+// simu (iters, error), to stdout as in the reference (the solver line, the steady-state
+// check's [DIFFENT U,V,W] line when config().steady_check is on, the [cD, cL] line) and
+// to Information/Time_cDcL.dat, and the Plot3D files are written at its points: the grid
+// and the initial field before the loop (RUN_CPU.cpp:84,241), then a q file whenever
+// simu.get_writefile() says so (RUN_CPU.cpp:411-413), staged on the device while the
+// next steps run.
 template <class Grid, class Timer, class Simu>
 bool runSeqOmp(Grid &gA, Timer &, Simu &simu, int, char **)
 {
@@ -388,7 +399,10 @@ bool runSeqOmp(Grid &gA, Timer &, Simu &simu, int, char **)
         simu.iters = info.iters;
         simu.error = info.err;
         simu.printInfo();
-        std::cout << "[cD, cL] : " << info.cd << ", " << info.cl << std::endl;
+        if (config().steady_check) // CheckSteadystate.hpp:302-305
+            std::cout << "[DIFFENT U,V,W] = " << info.maxdiff[0] << ", " << info.maxdiff[1]
+                      << ", " << info.maxdiff[2] << std::endl;
+        cdcl_output(simu, info.cd, info.cl);
         if (simu.get_writefile())
             check(cfdmx_plot3d_q(c, plot3d_q_name(simu.currentFile).c_str(), simu.get_time()),
                   "OutputPLOT3D_Qfile");
