@@ -618,9 +618,10 @@ def rbsor_run(wl, args):
         "sweeps_per_step": sweeps, "max_dp_per_step": [i["err"] for i in infos],
         "steps_reaching_tol": sum(1 for i in infos if i["err"] < wl["tol"]),
         "ms_per_sweep": round(wall / sum(sweeps) * 1e3, 4),
-        "roofline": {"bound": "hbm", "kernel": "k_sor_rb (one colour's half sweep)",
+        "roofline": {"bound": "hbm", "kernel": "k_sor_rb2 (one colour's half sweep)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(args.workload, "sor_half"),
                      "avg_launch_us": round(avg_s * 1e6, 2),
                      "bytes_per_launch": BYTES_PER_CELL["sor_half"] * n_cells,
                      "bytes_note": ("16 B per cell of the grid per half sweep: the other "

@@ -1364,6 +1364,81 @@ k_mixed_resid(Geo g, int pre, const double *__restrict__ b, const double *__rest
     block_sums<3>(acc, part, g.nblocks, L);
 }
 
+// The same residual with four consecutive cells of a z-row per thread (nzc a multiple of
+// 4): x's centre, x and y neighbours, b and the three outputs move as vectors, and the
+// z neighbours are the cells beside each other, so a thread issues 8 loads and 3 stores
+// for 4 cells instead of 32 and 12.  Each row's terms and order are k_mixed_resid's; the
+// partial sums add a thread's cells in another order.
+template <typename T> struct Vec4T;
+template <> struct Vec4T<float> {
+    static __device__ __forceinline__ void store(float *p, const float (&v)[4])
+    {
+        *reinterpret_cast<float4 *>(p) = make_float4(v[0], v[1], v[2], v[3]);
+    }
+};
+template <> struct Vec4T<double> {
+    static __device__ __forceinline__ void store(double *p, const double (&v)[4])
+    {
+        reinterpret_cast<double2 *>(p)[0] = make_double2(v[0], v[1]);
+        reinterpret_cast<double2 *>(p)[1] = make_double2(v[2], v[3]);
+    }
+};
+
+__device__ __forceinline__ void load4(const double *p, double (&v)[4])
+{
+    const double2 a = reinterpret_cast<const double2 *>(p)[0];
+    const double2 b = reinterpret_cast<const double2 *>(p)[1];
+    v[0] = a.x, v[1] = a.y, v[2] = b.x, v[3] = b.y;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+k_mixed_resid4(Geo g, int pre, const double *__restrict__ b, const double *__restrict__ x,
+               T *__restrict__ r32, T *__restrict__ r0_32, T *__restrict__ e32,
+               double *__restrict__ part)
+{
+    const int L = logical_block(g);
+    if (L >= g.nblocks) return;
+    double acc[3] = {0.0, 0.0, 0.0};
+    const int n0 = L * kPerBlock + 4 * threadIdx.x;
+    if (n0 < g.N) {
+        int i, j, k0;
+        decode(g, n0, i, j, k0);
+        double xc[4], xxm[4], xxp[4], xym[4], xyp[4], bv[4];
+        const bool hxm = i != 0, hxp = i != g.nxc - 1, hym = j != 0, hyp = j != g.nyc - 1;
+        load4(x + n0, xc);
+        load4(x + (hxm ? n0 - g.cyz : n0), xxm);
+        load4(x + (hxp ? n0 + g.cyz : n0), xxp);
+        load4(x + (hym ? n0 - g.nzc : n0), xym);
+        load4(x + (hyp ? n0 + g.nzc : n0), xyp);
+        load4(b + n0, bv);
+        const double zlo = x[k0 > 0 ? n0 - 1 : (g.zper ? n0 + g.nzc - 1 : n0)];
+        const double zhi = x[k0 + 3 < g.nzc - 1 ? n0 + 4 : (g.zper ? n0 + 4 - g.nzc : n0 + 3)];
+        T rv[4], zero[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            RowCoef rc;
+            if (pre) row_coef<1>(g, i, j, k0 + q, rc);
+            else row_coef<0>(g, i, j, k0 + q, rc);
+            const double xv[7] = {xxm[q], xym[q], q > 0 ? xc[q - 1] : zlo, xc[q],
+                                  q < 3 ? xc[q + 1] : zhi, xyp[q], xxp[q]};
+            double ax = 0.0;
+#pragma unroll
+            for (int t = 0; t < 7; ++t) ax += rc.has[t] ? rc.v[t] * xv[t] : 0.0;
+            const double rr = bv[q] - ax;
+            rv[q] = (T)rr;
+            zero[q] = (T)0;
+            acc[0] += bv[q] * bv[q];
+            acc[1] += rr * rr;
+            acc[2] += (double)rv[q] * (double)rv[q];
+        }
+        Vec4T<T>::store(r32 + n0, rv);
+        Vec4T<T>::store(r0_32 + n0, rv);
+        Vec4T<T>::store(e32 + n0, zero);
+    }
+    block_sums<3>(acc, part, g.nblocks, L);
+}
+
 // x += sign * e (FP64 accumulation of the FP32 correction; sign -1 takes it back)
 template <typename T>
 __global__ void k_mixed_update(int n, double sign, double *__restrict__ x,
@@ -1647,6 +1722,101 @@ k_sor_rb(Geo g, FastDiv dh, int colour, double omega, SorCoef sc,
             const double d = fabs(pNEW - p0);
             mx = d > mx ? d : mx;
             own[idx] = p0 + omega * (pNEW - p0);
+        }
+    }
+    __shared__ double red[4];
+    const double wm = wave_max(mx);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = wm;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = red[0];
+        for (int q = 1; q < 4; ++q) m = red[q] > m ? red[q] : m;
+        part[colour * nb + L] = m;
+    }
+}
+
+// The same half sweep with two neighbouring cells of a row per thread (nzc a multiple of
+// 4, so a row holds whole pairs): their own, right-hand-side and x and y neighbour values
+// are read as one 16-byte vector each, and their z neighbours as the pair of the other
+// colour beside them plus one value, which halves the load instructions.  Each cell's
+// update is k_sor_rb's, term for term.
+template <bool CONSTC>
+__global__ void __launch_bounds__(kThreads)
+k_sor_rb2(Geo g, FastDiv dh, int colour, double omega, SorCoef sc,
+          const double *__restrict__ cf6c, double *__restrict__ pc,
+          const double *__restrict__ pg, double *__restrict__ part, const SorState *st)
+{
+    if (st->done) return;
+    const int half = g.N >> 1, hz = g.nzc >> 1;
+    const int nb = (half + kRBCells * kThreads - 1) / (kRBCells * kThreads);
+    const int L = logical_block(g);
+    if (L >= nb) return;
+    const double *__restrict__ other = pc + (colour ^ 1) * half;
+    double *__restrict__ own = pc + colour * half;
+    const Axis &X = g.ax[0], &Y = g.ax[1], &Z = g.ax[2];
+    const int cyh = g.nyc * hz;
+    double mx = 0.0;
+#pragma unroll
+    for (int c = 0; c < kRBCells / 2; ++c) {
+        const int idx = 2 * ((L * (kRBCells / 2) + c) * kThreads + threadIdx.x);
+        if (idx < half) {
+            const RBIdx r = rb_decode(g, dh, colour, idx);
+            const int rb = r.row * hz, q = r.q;
+            const int icA = r.ic, icB = r.ic + 2; // the pair's cells are k and k + 2
+            const double2 p0 = *reinterpret_cast<const double2 *>(own + idx);
+            const double2 rhs = *reinterpret_cast<const double2 *>(cf6c + idx);
+            double2 pE, pW, pN, pS;
+            if (r.i < g.nxc - 1) pE = *reinterpret_cast<const double2 *>(other + idx + cyh);
+            else pE = make_double2(pg[icA + g.sx], pg[icB + g.sx]);
+            if (r.i > 0) pW = *reinterpret_cast<const double2 *>(other + idx - cyh);
+            else pW = make_double2(pg[icA - g.sx], pg[icB - g.sx]);
+            if (r.j < g.nyc - 1) pN = *reinterpret_cast<const double2 *>(other + idx + hz);
+            else pN = make_double2(pg[icA + g.sy], pg[icB + g.sy]);
+            if (r.j > 0) pS = *reinterpret_cast<const double2 *>(other + idx - hz);
+            else pS = make_double2(pg[icA - g.sy], pg[icB - g.sy]);
+            // z: the other colour's pair at this m, and the value before (q = 0) or after
+            // (q = 1) it
+            const double2 oz = *reinterpret_cast<const double2 *>(other + idx);
+            double pBA, pTA, pBB, pTB;
+            if (q == 0) {
+                pBA = r.m > 0 ? other[idx - 1] : (g.zper ? other[rb + hz - 1] : pg[icA - 1]);
+                pTA = oz.x;
+                pBB = oz.x;
+                pTB = oz.y;
+            } else {
+                pBA = oz.x;
+                pTA = oz.y;
+                pBB = oz.y;
+                pTB = r.m + 1 < hz - 1 ? other[idx + 2] : (g.zper ? other[rb] : pg[icB + 1]);
+            }
+            const double pe[2] = {pE.x, pE.y}, pw[2] = {pW.x, pW.y}, pn[2] = {pN.x, pN.y},
+                         ps[2] = {pS.x, pS.y}, pt[2] = {pTA, pTB}, pb[2] = {pBA, pBB},
+                         po[2] = {p0.x, p0.y}, rh[2] = {rhs.x, rhs.y};
+            double out[2];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                double cf0, cf1, cf2, cf3, cf4, cf5, cf7;
+                if (CONSTC) {
+                    cf0 = sc.c[0], cf1 = sc.c[1], cf2 = sc.c[2], cf3 = sc.c[3];
+                    cf4 = sc.c[4], cf5 = sc.c[5], cf7 = sc.c[6];
+                } else {
+                    const int I = r.i + kGC, J = r.j + kGC, K = r.k + kGC + 2 * h;
+                    cf0 = divr(Y.D[J] * Z.D[K], X.Ds[I], X.rDs[I]);
+                    cf1 = divr(Y.D[J] * Z.D[K], X.Ds[I - 1], X.rDs[I - 1]);
+                    cf2 = divr(X.D[I] * Z.D[K], Y.Ds[J], Y.rDs[J]);
+                    cf3 = divr(X.D[I] * Z.D[K], Y.Ds[J - 1], Y.rDs[J - 1]);
+                    cf4 = divr(X.D[I] * Y.D[J], Z.Ds[K], Z.rDs[K]);
+                    cf5 = divr(X.D[I] * Y.D[J], Z.Ds[K - 1], Z.rDs[K - 1]);
+                    cf7 = -1.0 / (cf0 + cf1 + cf2 + cf3 + cf4 + cf5);
+                }
+                const double pNEW = (-pe[h] * cf0 - pw[h] * cf1 - pn[h] * cf2 - ps[h] * cf3 -
+                                     pt[h] * cf4 - pb[h] * cf5 + rh[h]) *
+                                    cf7;
+                const double d = fabs(pNEW - po[h]);
+                mx = d > mx ? d : mx;
+                out[h] = po[h] + omega * (pNEW - po[h]);
+            }
+            *reinterpret_cast<double2 *>(own + idx) = make_double2(out[0], out[1]);
         }
     }
     __shared__ double red[4];
@@ -2160,8 +2330,12 @@ template <typename T>
 void launch_mixed_resid(const Geo &g, hipStream_t s, int pre, const double *b, const double *x,
                         T *r, T *r0, T *e, double *part)
 {
-    hipLaunchKernelGGL(k_mixed_resid<T>, dim3(grid_blocks(g)), dim3(kThreads), 0, s, g, pre, b,
-                       x, r, r0, e, part);
+    if (g.nzc % 4 == 0 && kPerBlock == 4 * kThreads)
+        hipLaunchKernelGGL(k_mixed_resid4<T>, dim3(grid_blocks(g)), dim3(kThreads), 0, s, g, pre,
+                           b, x, r, r0, e, part);
+    else
+        hipLaunchKernelGGL(k_mixed_resid<T>, dim3(grid_blocks(g)), dim3(kThreads), 0, s, g, pre,
+                           b, x, r, r0, e, part);
 }
 template <typename T>
 void launch_mixed_update(const Geo &g, hipStream_t s, double sign, double *x, const T *e)
@@ -2240,12 +2414,21 @@ void launch_sor_rb(const Geo &g, hipStream_t s, int colour, double omega, const 
     const FastDiv dh = make_fastdiv((unsigned)(g.nzc >> 1));
     const int nb = sor_rb_blocks(g);
     const int grid = g.xcd_remap ? ((nb + 7) / 8) * 8 : nb;
-    if (sc.constant)
+    const double *rhs = cf6c + colour * (g.N >> 1);
+    if (g.nzc % 4 == 0) { // whole pairs per row: two cells per thread-step
+        if (sc.constant)
+            hipLaunchKernelGGL(k_sor_rb2<true>, dim3(grid), dim3(kThreads), 0, s, g, dh, colour,
+                               omega, sc, rhs, pc, p, part, st);
+        else
+            hipLaunchKernelGGL(k_sor_rb2<false>, dim3(grid), dim3(kThreads), 0, s, g, dh, colour,
+                               omega, sc, rhs, pc, p, part, st);
+    } else if (sc.constant) {
         hipLaunchKernelGGL(k_sor_rb<true>, dim3(grid), dim3(kThreads), 0, s, g, dh, colour,
-                           omega, sc, cf6c + colour * (g.N >> 1), pc, p, part, st);
-    else
+                           omega, sc, rhs, pc, p, part, st);
+    } else {
         hipLaunchKernelGGL(k_sor_rb<false>, dim3(grid), dim3(kThreads), 0, s, g, dh, colour,
-                           omega, sc, cf6c + colour * (g.N >> 1), pc, p, part, st);
+                           omega, sc, rhs, pc, p, part, st);
+    }
 }
 
 void launch_sor_rb_fin(const Geo &g, hipStream_t s, double *part, SorState *st)
