@@ -643,10 +643,8 @@ static int setup_device(cfdmx_ctx *ctx)
     ALLOC(ctx->B.r0, N);
     ALLOC(ctx->B.r, N);
     ALLOC(ctx->B.as, N);
-    ALLOC(ctx->B.p[0], N);
-    ALLOC(ctx->B.p[1], N);
-    ALLOC(ctx->B.ap[0], N);
-    ALLOC(ctx->B.ap[1], N);
+    ALLOC(ctx->B.q, N);
+    ALLOC(ctx->B.ap, N);
     ALLOC(ctx->part, 4 * (size_t)(g.nblocks > g.nmarch ? g.nblocks : g.nmarch) + 64);
     ALLOC(ctx->cfl_part, (size_t)g.cd_nb + 8);
     ALLOC(ctx->diff_part, 3 * (size_t)g.nblocks + 8);
@@ -719,7 +717,7 @@ static int reset_run_state(cfdmx_ctx *ctx)
     const size_t G = (size_t)ctx->g.G, N = (size_t)ctx->g.N;
     launch_fill(s, (int)(3 * G), 0.0, ctx->f);
     double *zs[] = {ctx->x, ctx->b, ctx->px1, ctx->px2, ctx->B.r0, ctx->B.r, ctx->B.as,
-                    ctx->B.p[0], ctx->B.p[1], ctx->B.ap[0], ctx->B.ap[1]};
+                    ctx->B.q, ctx->B.ap};
     for (double *z : zs) launch_fill(s, (int)N, 0.0, z);
     launch_fill(s, 16, 0.0, ctx->scal);
     LAUNCHCHK();
@@ -2045,9 +2043,32 @@ static int adj_check(cfdmx_ctx *ctx, int nsteps, int K)
     return CFDMX_OK;
 }
 
-// the forward path of the objective: BC_start, then n steps; Cd summed in step order
-static int adj_forward(cfdmx_ctx *ctx, int nsteps, int K, int keep_checkpoints, double *J,
-                       int *iters_total)
+// one time step onto tape slot `slot`: U^{n-1} and p before the step, X_result and the
+// solved flag after it
+static int adj_tape_step(cfdmx_ctx *ctx, int slot, int *iters)
+{
+    cfdmx_ctx::Adj &A = ctx->adj;
+    const size_t G = ctx->g.G, N = ctx->g.N;
+    int rc = 0;
+    for (int q = 0; q < 3; ++q)
+        if ((rc = d2d(ctx, A.tapeU + ((size_t)slot * 3 + q) * G, ctx->t0[q], G))) return rc;
+    if ((rc = d2d(ctx, A.tapeP + (size_t)slot * G, ctx->p, G))) return rc;
+    int it = 0, solved = 0;
+    double err = 0.0;
+    if ((rc = step_once(ctx, &it, &err, &solved))) return rc;
+    if ((rc = d2d(ctx, A.tapeX + (size_t)slot * N, ctx->x, N))) return rc;
+    A.solved[slot] = solved;
+    if (iters) *iters = it;
+    return CFDMX_OK;
+}
+
+// the forward path of the objective: BC_start, then n steps; Cd summed in step order.
+// Steps n >= tape_from (>= 0) go onto the tape at slot n - tape_from: the adjoint tapes
+// its last segment here, which the reverse pass then need not recompute (every kernel
+// is deterministic, so a recompute would give the same bits).  Checkpoints are kept
+// every K steps before that.
+static int adj_forward(cfdmx_ctx *ctx, int nsteps, int K, int keep_checkpoints, int tape_from,
+                       double *J, int *iters_total)
 {
     hipStream_t s = ctx->stream;
     launch_bc(ctx->g, s, ctx->bc, 0, ctx->t0[0], ctx->t0[1], ctx->t0[2], ctx->p, ctx->t1[0],
@@ -2056,13 +2077,15 @@ static int adj_forward(cfdmx_ctx *ctx, int nsteps, int K, int keep_checkpoints, 
     double sum = 0.0;
     int it_tot = 0;
     for (int n = 0; n < nsteps; ++n) {
-        if (keep_checkpoints && n % K == 0) {
+        const bool taped = tape_from >= 0 && n >= tape_from;
+        if (keep_checkpoints && n % K == 0 && !taped) {
             int rc = adj_save(ctx, n / K);
             if (rc) return rc;
         }
         int it = 0, solved = 0;
         double err = 0.0;
-        int rc = step_once(ctx, &it, &err, &solved);
+        int rc = taped ? adj_tape_step(ctx, n - tape_from, &it)
+                       : step_once(ctx, &it, &err, &solved);
         if (!rc) rc = read_scalars(ctx);
         if (rc) return rc;
         sum += ctx->scal_h[1];
@@ -2079,7 +2102,7 @@ int cfdmx_drag_objective(cfdmx_ctx *ctx, int nsteps, double *J)
     int rc = adj_check(ctx, nsteps, 1);
     if (!rc) rc = adj_alloc(ctx, 1, 1);
     if (!rc) rc = adj_save(ctx, 0);
-    if (!rc) rc = adj_forward(ctx, nsteps, 1, 0, J, nullptr);
+    if (!rc) rc = adj_forward(ctx, nsteps, 1, 0, -1, J, nullptr);
     if (!rc) rc = adj_restore(ctx, 0);
     if (!rc) HIPCHK(hipStreamSynchronize(ctx->stream));
     return rc;
@@ -2108,7 +2131,7 @@ int cfdmx_adjoint_drag(cfdmx_ctx *ctx, int nsteps, int checkpoint_every, double 
     rc = adj_save(ctx, ncp);
     if (rc) return rc;
     int fwd_iters = 0;
-    rc = adj_forward(ctx, nsteps, K, 1, J, &fwd_iters);
+    rc = adj_forward(ctx, nsteps, K, 1, (ncp - 1) * K, J, &fwd_iters);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(s));
     auto t1 = std::chrono::steady_clock::now();
@@ -2135,19 +2158,12 @@ int cfdmx_adjoint_drag(cfdmx_ctx *ctx, int nsteps, int checkpoint_every, double 
     const double nu = 1.0 / c.re;
     int adj_iters = 0;
     for (int seg = ncp - 1; seg >= 0; --seg) {
-        rc = adj_restore(ctx, seg);
-        if (rc) return rc;
         const int n0 = seg * K, n1 = std::min(n0 + K, nsteps);
-        for (int n = n0; n < n1; ++n) {
-            const int slot = n - n0;
-            for (int q = 0; q < 3; ++q)
-                if ((rc = d2d(ctx, A.tapeU + ((size_t)slot * 3 + q) * G, ctx->t0[q], G))) return rc;
-            if ((rc = d2d(ctx, A.tapeP + (size_t)slot * G, ctx->p, G))) return rc;
-            int it = 0, solved = 0;
-            double err = 0.0;
-            if ((rc = step_once(ctx, &it, &err, &solved))) return rc;
-            if ((rc = d2d(ctx, A.tapeX + (size_t)slot * N, ctx->x, N))) return rc;
-            A.solved[slot] = solved;
+        if (seg != ncp - 1) { // the last segment is on the tape from the forward pass
+            rc = adj_restore(ctx, seg);
+            if (rc) return rc;
+            for (int n = n0; n < n1; ++n)
+                if ((rc = adj_tape_step(ctx, n - n0, nullptr))) return rc;
         }
         for (int n = n1 - 1; n >= n0; --n) {
             const int slot = n - n0;

@@ -48,13 +48,11 @@ struct SorState {
     int pad;
 };
 
-// Krylov vectors of one BiCGSTAB solve (double; float for the mixed-precision inner
-// solve).  p[cur] holds q = p - omega*Ap between phase C and the next phase A.
+// Krylov vectors of one BiCGSTAB solve.  q holds p - omega*Ap between phase C and the
+// next phase A; p itself is formed in registers by A and again by C, never stored.
 template <typename T>
 struct BicgBuf {
-    T *r0, *r, *as;
-    T *p[2], *ap[2];
-    int cur;
+    T *r0, *r, *as, *q, *ap;
 };
 using BicgBuffers = BicgBuf<double>;
 

@@ -903,7 +903,7 @@ template <typename T>
 struct MarchIO {
     const T *r, *q, *ap; // inputs (A: r, q; B: r, Ap)
     T *r0;               // A: restart target
-    T *out_p, *out_ap;   // A outputs
+    T *out_ap;           // A output
     T *out_as;           // B output
     double *part;
 };
@@ -1234,9 +1234,9 @@ k_march(Geo g, MarchIO<T> io, KState *st, int m, const double *part_in)
             const T own = oc[c];
             const double sd = (double)sum;
             if (PHASE == 0) {
-                // p is next read by phase C; Ap (read by phase B) keeps the default
-                // policy, which measured faster
-                st_stream<NT>(own, io.out_p + n);
+                // p itself is not stored: phase C forms it again from r and q, which
+                // are still in place (8 bytes per cell less).  Ap (read by phase B)
+                // keeps the default policy, which measured faster
                 io.out_ap[n] = sum;
                 const T r0v = r0c[c];
                 if (mode == 1) io.r0[n] = r0v;
@@ -1271,10 +1271,13 @@ k_march(Geo g, MarchIO<T> io, KState *st, int m, const double *part_in)
 }
 
 // phase C in the march mapping (elementwise): x, r update (bicgstabRe2.hpp:123-127),
-// the partials of (r,r), (r,r0), and q = p - omega*Ap in place of p for the next phase A
+// the partials of (r,r), (r,r0), and q = p - omega*Ap in place of the q it read, for the
+// next phase A.  p is formed again here from r and q, with phase A's expression and the
+// mode and beta it left in the state (p = r + beta q, or p = r on the first iteration
+// and on a restart), so it is the same value phase A used; A therefore stores Ap only.
 template <bool NT, typename T>
 __global__ void __launch_bounds__(kThreads)
-k_march_C(Geo g, T *__restrict__ r, const T *__restrict__ ap, T *__restrict__ p,
+k_march_C(Geo g, T *__restrict__ r, const T *__restrict__ ap, T *__restrict__ q,
           const T *__restrict__ as, const T *__restrict__ r0, T *__restrict__ x,
           const double *part_in, double *__restrict__ part_out, KState *st)
 {
@@ -1287,6 +1290,8 @@ k_march_C(Geo g, T *__restrict__ r, const T *__restrict__ ap, T *__restrict__ p,
     const double omega = t[0] / t[1]; // w = dot(as, s) / dot(as, as)
     if (b == 0 && threadIdx.x == 0) st->omega = omega;
     const T alpha = (T)st->alpha, om = (T)omega;
+    const int mode = st->mode;
+    const T betaT = (T)st->beta;
     const int tile = b % g.ntile, chunk = b / g.ntile;
     int ib, d, len;
     march_span(g, chunk, ib, d, len);
@@ -1304,12 +1309,14 @@ k_march_C(Geo g, T *__restrict__ r, const T *__restrict__ ap, T *__restrict__ p,
                 // q, which the next phase A reads at once, keep plain stores (hinted,
                 // they measured slower)
                 const T apn = ld_stream<NT>(ap + n);
-                const T pn = ld_stream<NT>(p + n);
-                const T sn = ld_stream<NT>(r + n) - alpha * apn;
+                const T rold = ld_stream<NT>(r + n);
+                const T qv = ld_stream<NT>(q + n);
+                const T pn = mode == 0 ? rold + betaT * qv : rold;
+                const T sn = rold - alpha * apn;
                 st_stream<NT>(ld_stream<NT>(x + n) + (alpha * pn + om * sn), x + n);
                 const T rn = sn - om * ld_stream<NT>(as + n);
                 r[n] = rn;
-                p[n] = pn - om * apn;
+                q[n] = pn - om * apn;
                 const double rd = (double)rn;
                 acc[0] += rd * rd;
                 acc[1] += rd * (double)ld_stream<NT>(r0 + n);
@@ -2277,13 +2284,11 @@ void launch_bicg_A(const Geo &g, hipStream_t s, int pre, BicgBuf<T> &B, double *
 {
     MarchIO<T> io = {};
     io.r = B.r;
-    io.q = B.p[B.cur]; // q = p - omega*Ap, stored by phase C
+    io.q = B.q;
     io.r0 = B.r0;
-    io.out_p = B.p[1 - B.cur];
-    io.out_ap = B.ap[1 - B.cur];
+    io.out_ap = B.ap;
     io.part = part[0];
     launch_march<0>(g, s, pre, io, st, m, part[2]);
-    B.cur = 1 - B.cur;
 }
 template <typename T>
 void launch_bicg_B(const Geo &g, hipStream_t s, int pre, BicgBuf<T> &B, double *const part[3],
@@ -2291,7 +2296,7 @@ void launch_bicg_B(const Geo &g, hipStream_t s, int pre, BicgBuf<T> &B, double *
 {
     MarchIO<T> io = {};
     io.r = B.r;
-    io.ap = B.ap[B.cur];
+    io.ap = B.ap;
     io.out_as = B.as;
     io.part = part[1];
     launch_march<1>(g, s, pre, io, st, m, part[0]);
@@ -2303,11 +2308,11 @@ void launch_bicg_C(const Geo &g, hipStream_t s, BicgBuf<T> &B, T *x, double *con
     (void)m;
     if (march_streaming<T>(g))
         hipLaunchKernelGGL((k_march_C<true, T>), dim3(march_blocks(g)), dim3(kThreads), 0, s, g,
-                           B.r, B.ap[B.cur], B.p[B.cur], B.as, B.r0, x, (const double *)part[1],
+                           B.r, B.ap, B.q, B.as, B.r0, x, (const double *)part[1],
                            part[2], st);
     else
         hipLaunchKernelGGL((k_march_C<false, T>), dim3(march_blocks(g)), dim3(kThreads), 0, s,
-                           g, B.r, B.ap[B.cur], B.p[B.cur], B.as, B.r0, x,
+                           g, B.r, B.ap, B.q, B.as, B.r0, x,
                            (const double *)part[1], part[2], st);
 }
 #define CFDMX_BICG_INSTANCES(T)                                                               \

@@ -263,12 +263,16 @@ CFDMX_API int cfdmx_synchronize(cfdmx_ctx *ctx);
 /* ---- adjoint of the time-averaged drag (no reference counterpart: the reference
  * has no adjoint).  Callers: gradient-based design and its checks (the PyTorch
  * autograd wrapper in the Python package).  Requires the uniform grid, the SPE
- * operator, BiCGSTAB and FP64.  The objective is J = (1/nsteps) sum of Cd over nsteps
- * steps from the current state, whose start boundary conditions are applied first.
- * The context is returned to its state at entry. */
+ * operator, BiCGSTAB and no Smagorinsky model; any precision (the multigrid modes
+ * solve the pressure adjoint with their own solver).  The objective is
+ * J = (1/nsteps) sum of Cd over nsteps steps from the current state, whose start
+ * boundary conditions are applied first.  The context is returned to its state at
+ * entry. */
 typedef struct cfdmx_adjoint_info {
-    double forward_ms;  /* forward pass, keeping a checkpoint every K steps */
-    double reverse_ms;  /* recompute of each K-step segment plus the adjoint steps */
+    double forward_ms;  /* forward pass: a checkpoint every K steps, the last segment
+                           taped as it runs */
+    double reverse_ms;  /* recompute of every K-step segment but the last, plus the
+                           adjoint steps */
     int forward_iters;  /* pressure iterations of the forward pass */
     int adjoint_iters;  /* iterations of the pressure adjoints */
     int nsteps, checkpoints;
@@ -280,7 +284,8 @@ CFDMX_API int cfdmx_drag_objective(cfdmx_ctx *ctx, int nsteps, double *J);
  * not read; all zero without cfdmx_set_inflow) and dJ/d(cx, cy, R) of the smoothed mask
  * set by cfdmx_set_eta_smoothed (grad_mask[3]); either may be NULL.  checkpoint_every
  * = K bounds the tape to K steps (memory: about (4G + N) * 8 bytes per tape step and
- * (4G + 3N) * 8 per checkpoint, plus 63 G * 8 for the conv-diff rows). */
+ * (4G + 3N) * 8 per checkpoint, plus 63 G * 8 for the conv-diff rows).  The results
+ * are the same bits for every K; K >= nsteps tapes every step and recomputes none. */
 CFDMX_API int cfdmx_adjoint_drag(cfdmx_ctx *ctx, int nsteps, int checkpoint_every, double *J,
                        double *grad_inflow, double *grad_mask, cfdmx_adjoint_info *info);
 

@@ -114,6 +114,28 @@ def test_torch_autograd_wrapper():
     gpu.close()
 
 
+@pytest.mark.parametrize("precision", [0, PRECISION_FP64_MG])
+def test_gradients_do_not_depend_on_the_checkpoint_interval(precision):
+    """The forward pass tapes the last segment itself and the reverse pass recomputes the
+    others from their checkpoints; every kernel is deterministic, so J and both
+    gradients are the same bits for every interval: K = 1 (everything recomputed but the
+    last step), 3 (a ragged last segment), K = nsteps and K > nsteps (nothing
+    recomputed)."""
+    nsteps = 5
+    runs = []
+    for K in (1, 3, 5, 8):
+        gpu, *_ = _pair(24, 12, 4, nsteps, tol=1e-12, precision=precision)
+        J, gi, gm, info = gpu.adjoint_drag(nsteps, K)
+        print(f"K {K}: J {J!r} dJ/dR {gm[2]!r} checkpoints {info['checkpoints']}")
+        assert info["checkpoints"] == -(-nsteps // K)
+        runs.append((J, gi.copy(), np.array(gm)))
+        gpu.close()
+    for J, gi, gm in runs[1:]:
+        assert J == runs[0][0]
+        assert np.array_equal(gi, runs[0][1])
+        assert np.array_equal(gm, runs[0][2])
+
+
 @pytest.mark.parametrize("precision,bound", [(PRECISION_MIXED, 1e-6), (PRECISION_FP64_MG, 1e-8)])
 def test_multigrid_adjoint_against_fp64_adjoint(precision, bound):
     """The adjoint with the multigrid pressure solves (forward, recompute and the
