@@ -91,6 +91,9 @@ struct cfdmx_ctx {
     KState *kst = nullptr;
     SorState *sst = nullptr;
     SorCoef sorc = {}; // red-black SOR coefficients of a grid with constant spacings
+    // red-black SOR working set of a context configured for it (even nzc): p of both
+    // colours, then the right-hand side of both colours, in one allocation
+    double *sorw = nullptr;
     int *count = nullptr;
     // pinned host mirrors
     KState *kst_h = nullptr;
@@ -645,6 +648,7 @@ static int setup_device(cfdmx_ctx *ctx)
     ALLOC(ctx->B.as, N);
     ALLOC(ctx->B.q, N);
     ALLOC(ctx->B.ap, N);
+    if (ctx->cfg.solver == CFDMX_SOLVER_RBSOR && g.nzc % 2 == 0) ALLOC(ctx->sorw, 2 * N);
     ALLOC(ctx->part, 4 * (size_t)(g.nblocks > g.nmarch ? g.nblocks : g.nmarch) + 64);
     ALLOC(ctx->cfl_part, (size_t)g.cd_nb + 8);
     ALLOC(ctx->diff_part, 3 * (size_t)g.nblocks + 8);
@@ -1400,14 +1404,16 @@ static int solve_rbsor_impl(cfdmx_ctx *ctx, int *iters, double *err)
     ss.itmax = ctx->cfg.maxit;
     *ctx->sst_h = ss;
     HIPCHK(hipMemcpyAsync(ctx->sst, ctx->sst_h, sizeof ss, hipMemcpyHostToDevice, s));
-    // even nzc: colour-split storage (the right-hand side in b, p of both colours in
-    // the Krylov vector r0, p's natural order written back at the end); odd nzc: the
+    // even nzc: colour-split storage, p's natural order written back at the end; the
+    // working set is the context's own when it is configured for red-black SOR, else the
+    // Krylov vector r0 (p of both colours) and b (the right-hand side).  Odd nzc: the
     // natural-order half sweep
     const bool split = (g.nzc % 2) == 0;
-    double *pc = ctx->B.r0;
+    double *pc = ctx->sorw ? ctx->sorw : ctx->B.r0;
+    double *rhs = ctx->sorw ? ctx->sorw + g.N : ctx->b;
     if (split)
         PROF(CFDMX_K_RHS, launch_sor_split(g, s, ctx->t1[0], ctx->t1[1], ctx->t1[2],
-                                           ctx->cfg.dt, ctx->p, ctx->b, pc));
+                                           ctx->cfg.dt, ctx->p, rhs, pc));
     else
         PROF(CFDMX_K_RHS, launch_sor_rhs(g, s, ctx->t1[0], ctx->t1[1], ctx->t1[2],
                                          ctx->cfg.dt, ctx->b));
@@ -1418,7 +1424,7 @@ static int solve_rbsor_impl(cfdmx_ctx *ctx, int *iters, double *err)
             for (int colour = 0; colour < 2; ++colour) {
                 if (split)
                     PROF(CFDMX_K_SOR, launch_sor_rb(g, s, colour, ctx->cfg.sor_omega,
-                                                    ctx->sorc, ctx->b, pc, ctx->p, ctx->part,
+                                                    ctx->sorc, rhs, pc, ctx->p, ctx->part,
                                                     ctx->sst));
                 else
                     PROF(CFDMX_K_SOR, launch_sor_half(g, s, colour, ctx->cfg.sor_omega,

@@ -529,13 +529,16 @@ k_flux_partial(Geo g, const double *u, double *part, int nb)
     block_sums<3>(v, part, nb, L);
 }
 
+// every block folds the few partials itself (the same order, so the same correction)
+// and corrects kThreads cells of the outlet plane
 __global__ void __launch_bounds__(kThreads)
 k_flux_apply(Geo g, const double *part, int nb, double *u, double *u1)
 {
     double t[3];
     fold_partials<3>(part, nb, nb, t);
     const double corr = (t[0] - t[1]) / t[2];
-    for (int m = threadIdx.x; m < g.cyz; m += kThreads) {
+    const int m = blockIdx.x * kThreads + threadIdx.x;
+    if (m < g.cyz) {
         const int J = m / g.nzc + kGC, K = m % g.nzc + kGC;
         const int e3 = gidx(g, g.nx - 3, J, K);
         const double ub = u[e3] + corr;
@@ -808,7 +811,7 @@ __global__ void __launch_bounds__(kThreads)
 k_rhs(Geo g, const double *__restrict__ us, const double *__restrict__ vs,
       const double *__restrict__ ws, double ddt, double *__restrict__ b)
 {
-    const int L = logical_block(g);
+    const int L = dealt_block();
     if (L >= g.nblocks) return;
     const Axis &X = g.ax[0], &Y = g.ax[1], &Z = g.ax[2];
     for (int c = 0; c < kCells; ++c) {
@@ -843,6 +846,68 @@ k_rhs(Geo g, const double *__restrict__ us, const double *__restrict__ vs,
     }
 }
 
+// The same right-hand side with four consecutive cells of a z-row per thread (nzc a
+// multiple of 4, so a thread's cells share i and j, and the ghosted index of the first
+// is even): u* and v* at the cells and at their -x / -y neighbours and b move as 16-byte
+// vectors, and w* at the cells plus the one below.  Each cell's expression is k_rhs's.
+__device__ __forceinline__ void load4(const double *p, double (&v)[4])
+{
+    const double2 a = reinterpret_cast<const double2 *>(p)[0];
+    const double2 b = reinterpret_cast<const double2 *>(p)[1];
+    v[0] = a.x, v[1] = a.y, v[2] = b.x, v[3] = b.y;
+}
+
+template <int PRE>
+__global__ void __launch_bounds__(kThreads)
+k_rhs4(Geo g, const double *__restrict__ us, const double *__restrict__ vs,
+       const double *__restrict__ ws, double ddt, double *__restrict__ b)
+{
+    const int L = dealt_block();
+    if (L >= g.nblocks) return;
+    const int n0 = L * kPerBlock + 4 * threadIdx.x;
+    if (n0 >= g.N) return;
+    const Axis &X = g.ax[0], &Y = g.ax[1], &Z = g.ax[2];
+    int i, j, k0;
+    decode(g, n0, i, j, k0);
+    const int I = i + kGC, J = j + kGC;
+    const int ic = gidx(g, I, J, k0 + kGC);
+    double uc[4], uw[4], vc[4], vs4[4], wc[4];
+    load4(us + ic, uc);
+    load4(us + ic - g.sx, uw);
+    load4(vs + ic, vc);
+    load4(vs + ic - g.sy, vs4);
+    load4(ws + ic, wc);
+    const double wb = ws[ic - 1];
+    double out[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int k = k0 + q, K = k + kGC;
+        const double div = divr(uc[q] - uw[q], X.D[I], X.rD[I]) +
+                           divr(vc[q] - vs4[q], Y.D[J], Y.rD[J]) +
+                           divr(wc[q] - (q > 0 ? wc[q - 1] : wb), Z.D[K], Z.rD[K]);
+        if (PRE == 1) {
+            const double xm = g.pa[0].m[i], xp = g.pa[0].p[i];
+            const double ym = g.pa[1].m[j], yp = g.pa[1].p[j];
+            const double zm = g.pa[2].m[k], zp = g.pa[2].p[k];
+            double aa = 0.0, bb = 0.0, cc = 0.0;
+            if (i == 0) aa = -xm;
+            if (i == g.nxc - 1) aa = -xp;
+            if (j == 0) bb = -ym;
+            if (j == g.nyc - 1) bb = -yp;
+            if (!g.zper) {
+                if (k == 0) cc = -zm;
+                if (k == g.nzc - 1) cc = -zp;
+            }
+            const double jp = 1.0 / (xm + ym + zm + xp + yp + zp + aa + bb + cc);
+            out[q] = -jp * ddt * div;
+        } else {
+            out[q] = -ddt * div;
+        }
+    }
+    reinterpret_cast<double2 *>(b + n0)[0] = make_double2(out[0], out[1]);
+    reinterpret_cast<double2 *>(b + n0)[1] = make_double2(out[2], out[3]);
+}
+
 // ---------------------------------------------------------------------------
 // BiCGSTAB, bicgstabRe2::solve (bicgstabRe2.hpp:82-180), as three fused phases per
 // iteration with the scalars kept on the device:
@@ -863,7 +928,7 @@ k_bicg_init(Geo g, int pre, const double *__restrict__ b, const double *__restri
             double *__restrict__ r0, double *__restrict__ r, double *__restrict__ px1,
             double *__restrict__ part, KState *st)
 {
-    const int L = logical_block(g);
+    const int L = dealt_block();
     if (L >= g.nblocks) return;
     double acc[2] = {0.0, 0.0};
     for (int c = 0; c < kCells; ++c) {
@@ -1337,7 +1402,7 @@ k_mixed_resid(Geo g, int pre, const double *__restrict__ b, const double *__rest
               T *__restrict__ r32, T *__restrict__ r0_32, T *__restrict__ e32,
               double *__restrict__ part)
 {
-    const int L = logical_block(g);
+    const int L = dealt_block();
     if (L >= g.nblocks) return;
     double acc[3] = {0.0, 0.0, 0.0};
     auto cell = [&](int n) {
@@ -1391,20 +1456,13 @@ template <> struct Vec4T<double> {
     }
 };
 
-__device__ __forceinline__ void load4(const double *p, double (&v)[4])
-{
-    const double2 a = reinterpret_cast<const double2 *>(p)[0];
-    const double2 b = reinterpret_cast<const double2 *>(p)[1];
-    v[0] = a.x, v[1] = a.y, v[2] = b.x, v[3] = b.y;
-}
-
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
 k_mixed_resid4(Geo g, int pre, const double *__restrict__ b, const double *__restrict__ x,
                T *__restrict__ r32, T *__restrict__ r0_32, T *__restrict__ e32,
                double *__restrict__ part)
 {
-    const int L = logical_block(g);
+    const int L = dealt_block();
     if (L >= g.nblocks) return;
     double acc[3] = {0.0, 0.0, 0.0};
     const int n0 = L * kPerBlock + 4 * threadIdx.x;
@@ -1687,7 +1745,7 @@ k_sor_rb(Geo g, FastDiv dh, int colour, double omega, SorCoef sc,
     if (st->done) return;
     const int half = g.N >> 1, hz = g.nzc >> 1;
     const int nb = (half + kRBCells * kThreads - 1) / (kRBCells * kThreads);
-    const int L = logical_block(g);
+    const int L = dealt_block();
     if (L >= nb) return;
     const double *__restrict__ other = pc + (colour ^ 1) * half;
     double *__restrict__ own = pc + colour * half;
@@ -1756,7 +1814,7 @@ k_sor_rb2(Geo g, FastDiv dh, int colour, double omega, SorCoef sc,
     if (st->done) return;
     const int half = g.N >> 1, hz = g.nzc >> 1;
     const int nb = (half + kRBCells * kThreads - 1) / (kRBCells * kThreads);
-    const int L = logical_block(g);
+    const int L = dealt_block();
     if (L >= nb) return;
     const double *__restrict__ other = pc + (colour ^ 1) * half;
     double *__restrict__ own = pc + colour * half;
@@ -1872,6 +1930,40 @@ k_sor_rb_fin(const double *part, int nb2, SorState *st)
 // (Transform.hpp:8-25) and the drag/lift partial sums of virtualF_Int
 // (virtualForceIntergrator.hpp:79-86).  FROM_X: pressure read from X_result.
 // ---------------------------------------------------------------------------
+// One cell's three faces (updateUandF.hpp:90-180 with Transform.hpp's pressure): the
+// corrected velocity T2 = U* - dt dp/dn, the masked value (1 - eta_face) T2 (the solid
+// velocity is 0) and the forcing (new - T2) / dt.  pc, pe, pn, pt: pressure at the cell and
+// its +x, +y, +z neighbours; ec..et: the mask likewise; us, vs, ws: U* at the cell.
+struct ProjIn {
+    double pc, pe, pn, pt, ec, ee, en, et, us, vs, ws;
+};
+struct ProjFaces {
+    double un, vn, wn, fx, fy, fz;
+};
+__device__ __forceinline__ ProjFaces project_faces(const Geo &g, int I, int J, int K, double dt,
+                                                   double rdt, const ProjIn &a)
+{
+    const Axis &X = g.ax[0], &Y = g.ax[1], &Z = g.ax[2];
+    const double uS = 0.0, vS = 0.0, wS = 0.0;
+    ProjFaces o;
+    // x face
+    const double T2u = a.us - divr(dt * (a.pe - a.pc), X.Ds[I], X.rDs[I]);
+    const double eu = a.ec + (a.ee - a.ec) * X.IT[I];
+    o.un = a.ec * uS + (1.0 - eu) * T2u;
+    o.fx = divr(o.un - T2u, dt, rdt);
+    // y face
+    const double T2v = a.vs - divr(dt * (a.pn - a.pc), Y.Ds[J], Y.rDs[J]);
+    const double ev = a.ec + (a.en - a.ec) * Y.IT[J];
+    o.vn = a.ec * vS + (1.0 - ev) * T2v;
+    o.fy = divr(o.vn - T2v, dt, rdt);
+    // z face
+    const double T2w = a.ws - divr(dt * (a.pt - a.pc), Z.Ds[K], Z.rDs[K]);
+    const double ew = a.ec + (a.et - a.ec) * Z.IT[K];
+    o.wn = a.ec * wS + (1.0 - ew) * T2w;
+    o.fz = divr(o.wn - T2w, dt, rdt);
+    return o;
+}
+
 // DIFF: also the steady-state check of CheckSteadyStatebyMaxVal_omp
 // (CheckSteadystate.hpp:247-307), max(T0 - T3) per component starting at 0, over the
 // faces written here, from the previous field u0, v0, w0: block maxima into dpart
@@ -1885,10 +1977,9 @@ k_project(Geo g, double dt, double rdt, int convective, const double *__restrict
           const double *__restrict__ u0, const double *__restrict__ v0,
           const double *__restrict__ w0, double *__restrict__ dpart)
 {
-    const int L = logical_block(g);
+    const int L = dealt_block();
     if (L >= g.nblocks) return;
     const Axis &X = g.ax[0], &Y = g.ax[1], &Z = g.ax[2];
-    const double uS = 0.0, vS = 0.0, wS = 0.0;
     double acc[2] = {0.0, 0.0};
     double dm[3] = {0.0, 0.0, 0.0};
     // Every load is unconditional: where a face is skipped (the last interior x or y
@@ -1912,46 +2003,149 @@ k_project(Geo g, double dt, double rdt, int convective, const double *__restrict
         const double pe = FROM_X ? xres[ox ? n + g.cyz : n] : p[icE];
         const double pn = FROM_X ? xres[oy ? n + g.nzc : n] : p[icN];
         const double pt = FROM_X ? xres[oz ? (wrap ? n - (g.nzc - 1) : n + 1) : n] : p[icP];
-        const double ec = eta[ic], ee = eta[icE], en = eta[icN], et = eta[icP];
-        const double usc = us[ic], vsc = vs[ic], wsc = ws[ic];
+        const ProjFaces o = project_faces(g, I, J, K, dt, rdt,
+                                          {pc, pe, pn, pt, eta[ic], eta[icE], eta[icN],
+                                           eta[icP], us[ic], vs[ic], ws[ic]});
         if (FROM_X) p[ic] = pc;
-        // x face
-        const double T2u = usc - divr(dt * (pe - pc), X.Ds[I], X.rDs[I]);
-        const double eu = ec + (ee - ec) * X.IT[I];
-        const double un = ec * uS + (1.0 - eu) * T2u;
-        const double fxv = divr(un - T2u, dt, rdt);
         if (ox) {
-            u[ic] = un;
-            f[ic] = fxv;
+            u[ic] = o.un;
+            f[ic] = o.fx;
         } else if (convective) {
-            u[ic] = usc;
+            u[ic] = us[ic];
         }
-        // y face
-        const double T2v = vsc - divr(dt * (pn - pc), Y.Ds[J], Y.rDs[J]);
-        const double ev = ec + (en - ec) * Y.IT[J];
-        const double vn = ec * vS + (1.0 - ev) * T2v;
-        const double fyv = divr(vn - T2v, dt, rdt);
         if (oy) {
-            v[ic] = vn;
-            f[ic + g.G] = fyv;
+            v[ic] = o.vn;
+            f[ic + g.G] = o.fy;
         }
-        // z face
-        const double T2w = wsc - divr(dt * (pt - pc), Z.Ds[K], Z.rDs[K]);
-        const double ew = ec + (et - ec) * Z.IT[K];
-        const double wn = ec * wS + (1.0 - ew) * T2w;
         if (oz) {
-            w[ic] = wn;
-            f[ic + 2 * g.G] = divr(wn - T2w, dt, rdt);
+            w[ic] = o.wn;
+            f[ic + 2 * g.G] = o.fz;
         }
-        const double fx = ox ? fxv : 0.0, fy = oy ? fyv : 0.0;
+        const double fx = ox ? o.fx : 0.0, fy = oy ? o.fy : 0.0;
         const double vol = X.D[I] * Y.D[J] * Z.D[K];
         acc[0] += fx * vol;
         acc[1] += fy * vol;
         if (DIFF) {
-            const double du = u0[ic] - un, dv = v0[ic] - vn, dw = w0[ic] - wn;
+            const double du = u0[ic] - o.un, dv = v0[ic] - o.vn, dw = w0[ic] - o.wn;
             if (ox && du > dm[0]) dm[0] = du;
             if (oy && dv > dm[1]) dm[1] = dv;
             if (oz && dw > dm[2]) dm[2] = dw;
+        }
+    }
+    block_sums<2>(acc, part, g.nblocks, L);
+    if (DIFF) block_maxes<3>(dm, dpart, g.nblocks, L);
+}
+
+// The same projection with four consecutive cells of a z-row per thread (nzc a multiple
+// of 4: a thread's cells share i and j, and the first one's interior and ghosted indices
+// are even).  The pressure, mask and velocity values of the cells and of their +x / +y
+// neighbours, and every store, move as 16-byte vectors; the +z neighbours are the cells
+// beside each other plus one value.  Each cell's values come from project_faces, as in
+// k_project.  The drag partials keep k_project's order: the per-cell terms go through LDS
+// and each thread adds the cells k_project gives it (c * kThreads + thread), so the
+// partial sums, and Cd and Cl, are the same bits.
+template <int FROM_X, int DIFF>
+__global__ void __launch_bounds__(kThreads)
+k_project4(Geo g, double dt, double rdt, int convective, const double *__restrict__ xres,
+           double *__restrict__ p, const double *__restrict__ eta,
+           const double *__restrict__ us, const double *__restrict__ vs,
+           const double *__restrict__ ws, double *__restrict__ u, double *__restrict__ v,
+           double *__restrict__ w, double *__restrict__ f, double *__restrict__ part,
+           const double *__restrict__ u0, const double *__restrict__ v0,
+           const double *__restrict__ w0, double *__restrict__ dpart)
+{
+    const int L = dealt_block();
+    if (L >= g.nblocks) return;
+    __shared__ double sfx[kPerBlock], sfy[kPerBlock];
+    const Axis &X = g.ax[0], &Y = g.ax[1], &Z = g.ax[2];
+    double dm[3] = {0.0, 0.0, 0.0};
+    const int n0 = L * kPerBlock + 4 * threadIdx.x;
+    if (n0 < g.N) {
+        int i, j, k0;
+        decode(g, n0, i, j, k0);
+        const int I = i + kGC, J = j + kGC, K0 = k0 + kGC;
+        const int ic = gidx(g, I, J, K0);
+        const bool ox = I < g.nx - 3, oy = J < g.ny - 3;
+        // the last cell's +z face: inside the row, wrapped (periodic z) or skipped
+        const bool last = K0 + 3 == g.nz - 3;
+        const bool oz3 = !last || g.zper;
+        const int dz3 = last ? (g.zper ? 3 - (g.nzc - 1) : 3) : 4;
+        const int icE = ox ? ic + g.sx : ic, icN = oy ? ic + g.sy : ic;
+        double pc[4], pe[4], pn[4], pt3;
+        if (FROM_X) {
+            load4(xres + n0, pc);
+            load4(xres + (ox ? n0 + g.cyz : n0), pe);
+            load4(xres + (oy ? n0 + g.nzc : n0), pn);
+            pt3 = xres[n0 + dz3];
+        } else {
+            load4(p + ic, pc);
+            load4(p + icE, pe);
+            load4(p + icN, pn);
+            pt3 = p[ic + dz3];
+        }
+        double ec[4], ee[4], en[4], uc[4], vc[4], wc[4];
+        load4(eta + ic, ec);
+        load4(eta + icE, ee);
+        load4(eta + icN, en);
+        const double et3 = eta[ic + dz3];
+        load4(us + ic, uc);
+        load4(vs + ic, vc);
+        load4(ws + ic, wc);
+        double un[4], vn[4], wn[4], fx[4], fy[4], fz[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const ProjFaces o = project_faces(
+                g, I, J, K0 + q, dt, rdt,
+                {pc[q], pe[q], pn[q], q < 3 ? pc[q + 1] : pt3, ec[q], ee[q], en[q],
+                 q < 3 ? ec[q + 1] : et3, uc[q], vc[q], wc[q]});
+            un[q] = o.un, vn[q] = o.vn, wn[q] = o.wn;
+            fx[q] = o.fx, fy[q] = o.fy, fz[q] = o.fz;
+            const double vol = X.D[I] * Y.D[J] * Z.D[K0 + q];
+            sfx[4 * threadIdx.x + q] = (ox ? o.fx : 0.0) * vol;
+            sfy[4 * threadIdx.x + q] = (oy ? o.fy : 0.0) * vol;
+        }
+        if (FROM_X) Vec4T<double>::store(p + ic, pc);
+        if (ox) {
+            Vec4T<double>::store(u + ic, un);
+            Vec4T<double>::store(f + ic, fx);
+        } else if (convective) {
+            Vec4T<double>::store(u + ic, uc);
+        }
+        if (oy) {
+            Vec4T<double>::store(v + ic, vn);
+            Vec4T<double>::store(f + g.G + ic, fy);
+        }
+        if (oz3) {
+            Vec4T<double>::store(w + ic, wn);
+            Vec4T<double>::store(f + 2 * g.G + ic, fz);
+        } else {
+            reinterpret_cast<double2 *>(w + ic)[0] = make_double2(wn[0], wn[1]);
+            w[ic + 2] = wn[2];
+            reinterpret_cast<double2 *>(f + 2 * g.G + ic)[0] = make_double2(fz[0], fz[1]);
+            f[2 * g.G + ic + 2] = fz[2];
+        }
+        if (DIFF) {
+            double a[4], b[4], c[4];
+            load4(u0 + ic, a);
+            load4(v0 + ic, b);
+            load4(w0 + ic, c);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const double du = a[q] - un[q], dv = b[q] - vn[q], dw = c[q] - wn[q];
+                if (ox && du > dm[0]) dm[0] = du;
+                if (oy && dv > dm[1]) dm[1] = dv;
+                if ((q < 3 || oz3) && dw > dm[2]) dm[2] = dw;
+            }
+        }
+    }
+    __syncthreads();
+    double acc[2] = {0.0, 0.0};
+#pragma unroll
+    for (int c = 0; c < kCells; ++c) {
+        const int t = c * kThreads + threadIdx.x;
+        if (L * kPerBlock + t < g.N) {
+            acc[0] += sfx[t];
+            acc[1] += sfy[t];
         }
     }
     block_sums<2>(acc, part, g.nblocks, L);
@@ -2175,8 +2369,8 @@ void launch_bc(const Geo &g, hipStream_t s, const BCParams &bc, int advance, dou
     if (bc.flux_correction) {
         const int nb = cdiv(g.cyz, kPerBlock);
         hipLaunchKernelGGL(k_flux_partial, dim3(nb), dim3(kThreads), 0, s, g, u, flux_part, nb);
-        hipLaunchKernelGGL(k_flux_apply, dim3(1), dim3(kThreads), 0, s, g, flux_part, nb, u,
-                           u1);
+        hipLaunchKernelGGL(k_flux_apply, dim3(cdiv(g.cyz, kThreads)), dim3(kThreads), 0, s, g,
+                           flux_part, nb, u, u1);
     }
     hipLaunchKernelGGL(k_bc_y, dim3(cdiv(g.nxc * g.nzc, 256)), dim3(256), 0, s, g, bc, u, v, w,
                        p, u1, v1, w1);
@@ -2197,7 +2391,15 @@ void launch_rhs(const Geo &g, hipStream_t s, int pre, const double *us, const do
                 const double *ws, double dt, double *b)
 {
     const double ddt = 1.0 / dt;
-    if (pre)
+    static_assert(kPerBlock == 4 * kThreads, "k_rhs4 covers a block's cells four per thread");
+    if (g.nzc % 4 == 0) {
+        if (pre)
+            hipLaunchKernelGGL(k_rhs4<1>, dim3(grid_blocks(g)), dim3(kThreads), 0, s, g, us, vs,
+                               ws, ddt, b);
+        else
+            hipLaunchKernelGGL(k_rhs4<0>, dim3(grid_blocks(g)), dim3(kThreads), 0, s, g, us, vs,
+                               ws, ddt, b);
+    } else if (pre)
         hipLaunchKernelGGL(k_rhs<1>, dim3(grid_blocks(g)), dim3(kThreads), 0, s, g, us, vs, ws,
                            ddt, b);
     else
@@ -2456,21 +2658,22 @@ void launch_project(const Geo &g, hipStream_t s, int from_x, double dt, int conv
     const dim3 grid(grid_blocks(g)), block(kThreads);
     const double *u0 = prev ? prev[0] : nullptr, *v0 = prev ? prev[1] : nullptr,
                  *w0 = prev ? prev[2] : nullptr;
-    if (dpart) {
-        if (from_x)
-            hipLaunchKernelGGL((k_project<1, 1>), grid, block, 0, s, g, dt, rdt, convective, xres,
-                               p, eta, us, vs, ws, u, v, w, f, part, u0, v0, w0, dpart);
-        else
-            hipLaunchKernelGGL((k_project<0, 1>), grid, block, 0, s, g, dt, rdt, convective, xres,
-                               p, eta, us, vs, ws, u, v, w, f, part, u0, v0, w0, dpart);
+#define CFDMX_PROJECT(KERNEL, FX, DF)                                                         \
+    hipLaunchKernelGGL((KERNEL<FX, DF>), grid, block, 0, s, g, dt, rdt, convective, xres, p,  \
+                       eta, us, vs, ws, u, v, w, f, part, u0, v0, w0, dpart)
+    const int df = dpart ? 1 : 0;
+    if (g.nzc % 4 == 0) { // four cells per thread (k_project4 covers a block four per thread)
+        if (from_x && df) CFDMX_PROJECT(k_project4, 1, 1);
+        else if (from_x) CFDMX_PROJECT(k_project4, 1, 0);
+        else if (df) CFDMX_PROJECT(k_project4, 0, 1);
+        else CFDMX_PROJECT(k_project4, 0, 0);
     } else {
-        if (from_x)
-            hipLaunchKernelGGL((k_project<1, 0>), grid, block, 0, s, g, dt, rdt, convective, xres,
-                               p, eta, us, vs, ws, u, v, w, f, part, u0, v0, w0, dpart);
-        else
-            hipLaunchKernelGGL((k_project<0, 0>), grid, block, 0, s, g, dt, rdt, convective, xres,
-                               p, eta, us, vs, ws, u, v, w, f, part, u0, v0, w0, dpart);
+        if (from_x && df) CFDMX_PROJECT(k_project, 1, 1);
+        else if (from_x) CFDMX_PROJECT(k_project, 1, 0);
+        else if (df) CFDMX_PROJECT(k_project, 0, 1);
+        else CFDMX_PROJECT(k_project, 0, 0);
     }
+#undef CFDMX_PROJECT
 }
 
 void launch_max3_fin(const Geo &g, hipStream_t s, const double *dpart, double *out)
