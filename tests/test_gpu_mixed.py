@@ -226,3 +226,25 @@ def test_refinement_that_cannot_converge_is_reported(precision):
     r = b - o.apply_A(g, gpu.xresult(), 0, 1)
     assert np.isclose(np.linalg.norm(r) / np.linalg.norm(b), info["err"], rtol=1e-6)
     gpu.close()
+
+
+def test_multigrid_fields_match_the_committed_digests():
+    """Both multigrid modes on six grids (every V-cycle kernel variant) give the fields
+    whose digests tests/golden/multigrid_digests.json holds, bit for bit."""
+    import json
+    import os
+    import sys
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    sys.path.insert(0, here)
+    import make_multigrid_digests as mk
+    ref = json.load(open(os.path.join(here, "multigrid_digests.json")))
+    assert ref["steps"] == mk.STEPS and ref["seed"] == mk.SEED
+    bad = []
+    for c in mk.CASES:
+        for precision in (PRECISION_MIXED, PRECISION_FP64_MG):
+            k = mk.key(*c, precision)
+            got = mk.run_case(*c, precision)
+            print(k, got)
+            if got != ref["digests"][k]:
+                bad.append(k)
+    assert not bad, bad
