@@ -647,6 +647,7 @@ static int setup_device(cfdmx_ctx *ctx)
     ALLOC(ctx->B.r, N);
     ALLOC(ctx->B.as, N);
     ALLOC(ctx->B.q, N);
+    ALLOC(ctx->B.p, N);
     ALLOC(ctx->B.ap, N);
     if (ctx->cfg.solver == CFDMX_SOLVER_RBSOR && g.nzc % 2 == 0) ALLOC(ctx->sorw, 2 * N);
     ALLOC(ctx->part, 4 * (size_t)(g.nblocks > g.nmarch ? g.nblocks : g.nmarch) + 64);
@@ -721,7 +722,7 @@ static int reset_run_state(cfdmx_ctx *ctx)
     const size_t G = (size_t)ctx->g.G, N = (size_t)ctx->g.N;
     launch_fill(s, (int)(3 * G), 0.0, ctx->f);
     double *zs[] = {ctx->x, ctx->b, ctx->px1, ctx->px2, ctx->B.r0, ctx->B.r, ctx->B.as,
-                    ctx->B.q, ctx->B.ap};
+                    ctx->B.q, ctx->B.p, ctx->B.ap};
     for (double *z : zs) launch_fill(s, (int)N, 0.0, z);
     launch_fill(s, 16, 0.0, ctx->scal);
     LAUNCHCHK();

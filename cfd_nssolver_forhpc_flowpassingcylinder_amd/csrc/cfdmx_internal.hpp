@@ -49,10 +49,11 @@ struct SorState {
 };
 
 // Krylov vectors of one BiCGSTAB solve.  q holds p - omega*Ap between phase C and the
-// next phase A; p itself is formed in registers by A and again by C, never stored.
+// next phase A.  p is written by phase A for phase C only on grids whose vectors fit the
+// caches; above that C forms it again from r and q (k_march_C).
 template <typename T>
 struct BicgBuf {
-    T *r0, *r, *as, *q, *ap;
+    T *r0, *r, *as, *q, *p, *ap;
 };
 using BicgBuffers = BicgBuf<double>;
 

@@ -951,6 +951,70 @@ k_bicg_init(Geo g, int pre, const double *__restrict__ b, const double *__restri
     block_sums<2>(acc, part, g.nblocks, L);
 }
 
+// The same initialisation with four consecutive cells of a z-row per thread (nzc a
+// multiple of 4): x at the cells and at their x and y neighbours, b and the three outputs
+// move as 16-byte vectors, and the z neighbours are the cells beside each other plus one
+// value at either end.  Each row's terms and order are k_bicg_init's.  The partial sums
+// keep k_bicg_init's order, which the solver's stopping test depends on: each cell's
+// (b, b) and (r, r) terms go through LDS and every thread adds the cells that k_bicg_init
+// gives it (c * kThreads + thread, c = 0..3) in that order.
+__global__ void __launch_bounds__(kThreads)
+k_bicg_init4(Geo g, int pre, const double *__restrict__ b, const double *__restrict__ x,
+             double *__restrict__ r0, double *__restrict__ r, double *__restrict__ px1,
+             double *__restrict__ part, KState *st)
+{
+    const int L = dealt_block();
+    if (L >= g.nblocks) return;
+    __shared__ double sbb[kPerBlock], srr[kPerBlock];
+    const int n0 = L * kPerBlock + 4 * threadIdx.x;
+    if (n0 < g.N) {
+        int i, j, k0;
+        decode(g, n0, i, j, k0);
+        double xc[4], xxm[4], xxp[4], xym[4], xyp[4], bv[4];
+        const bool hxm = i != 0, hxp = i != g.nxc - 1, hym = j != 0, hyp = j != g.nyc - 1;
+        load4(x + n0, xc);
+        load4(x + (hxm ? n0 - g.cyz : n0), xxm);
+        load4(x + (hxp ? n0 + g.cyz : n0), xxp);
+        load4(x + (hym ? n0 - g.nzc : n0), xym);
+        load4(x + (hyp ? n0 + g.nzc : n0), xyp);
+        load4(b + n0, bv);
+        const double zlo = x[k0 > 0 ? n0 - 1 : (g.zper ? n0 + g.nzc - 1 : n0)];
+        const double zhi = x[k0 + 3 < g.nzc - 1 ? n0 + 4 : (g.zper ? n0 + 4 - g.nzc : n0 + 3)];
+        double rv[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            RowCoef rc;
+            if (pre) row_coef<1>(g, i, j, k0 + q, rc);
+            else row_coef<0>(g, i, j, k0 + q, rc);
+            const double xv[7] = {xxm[q], xym[q], q > 0 ? xc[q - 1] : zlo, xc[q],
+                                  q < 3 ? xc[q + 1] : zhi, xyp[q], xxp[q]};
+            double ax = 0.0;
+#pragma unroll
+            for (int t = 0; t < 7; ++t) ax += rc.has[t] ? rc.v[t] * xv[t] : 0.0;
+            rv[q] = bv[q] - ax;
+            sbb[4 * threadIdx.x + q] = bv[q] * bv[q];
+            srr[4 * threadIdx.x + q] = rv[q] * rv[q];
+        }
+        reinterpret_cast<double2 *>(r0 + n0)[0] = make_double2(rv[0], rv[1]);
+        reinterpret_cast<double2 *>(r0 + n0)[1] = make_double2(rv[2], rv[3]);
+        reinterpret_cast<double2 *>(r + n0)[0] = make_double2(rv[0], rv[1]);
+        reinterpret_cast<double2 *>(r + n0)[1] = make_double2(rv[2], rv[3]);
+        reinterpret_cast<double2 *>(px1 + n0)[0] = make_double2(xc[0], xc[1]);
+        reinterpret_cast<double2 *>(px1 + n0)[1] = make_double2(xc[2], xc[3]);
+    }
+    __syncthreads();
+    double acc[2] = {0.0, 0.0};
+#pragma unroll
+    for (int c = 0; c < kCells; ++c) {
+        const int t = c * kThreads + threadIdx.x;
+        if (L * kPerBlock + t < g.N) {
+            acc[0] += sbb[t];
+            acc[1] += srr[t];
+        }
+    }
+    block_sums<2>(acc, part, g.nblocks, L);
+}
+
 // Plane-marching stencil phases.  A block owns the plane-index tile
 // q in [q0, q0+1024) (q = j*nzc + k) for the planes i in [i0, i0+8); even chunks march
 // down (i = i0+7 .. i0), odd chunks up, so that two neighbouring chunks read their
@@ -968,7 +1032,7 @@ template <typename T>
 struct MarchIO {
     const T *r, *q, *ap; // inputs (A: r, q; B: r, Ap)
     T *r0;               // A: restart target
-    T *out_ap;           // A output
+    T *out_p, *out_ap;   // A outputs (p only where it is stored: see k_march_C)
     T *out_as;           // B output
     double *part;
 };
@@ -1299,9 +1363,11 @@ k_march(Geo g, MarchIO<T> io, KState *st, int m, const double *part_in)
             const T own = oc[c];
             const double sd = (double)sum;
             if (PHASE == 0) {
-                // p itself is not stored: phase C forms it again from r and q, which
+                // p is stored only on grids whose vectors stay in the caches (NT off);
+                // with the streaming hints phase C forms it again from r and q, which
                 // are still in place (8 bytes per cell less).  Ap (read by phase B)
                 // keeps the default policy, which measured faster
+                if (!NT) io.out_p[n] = own;
                 io.out_ap[n] = sum;
                 const T r0v = r0c[c];
                 if (mode == 1) io.r0[n] = r0v;
@@ -1336,15 +1402,19 @@ k_march(Geo g, MarchIO<T> io, KState *st, int m, const double *part_in)
 }
 
 // phase C in the march mapping (elementwise): x, r update (bicgstabRe2.hpp:123-127),
-// the partials of (r,r), (r,r0), and q = p - omega*Ap in place of the q it read, for the
-// next phase A.  p is formed again here from r and q, with phase A's expression and the
-// mode and beta it left in the state (p = r + beta q, or p = r on the first iteration
-// and on a restart), so it is the same value phase A used; A therefore stores Ap only.
+// the partials of (r,r), (r,r0), and q = p - omega*Ap for the next phase A.  With the
+// streaming hints (NT: the vectors exceed the memory-side cache) p is formed again here
+// from r and q, with phase A's expression and the mode and beta it left in the state
+// (p = r + beta q, or p = r on the first iteration and on a restart), so it is the same
+// value phase A used, and A stores Ap only: 8 bytes per cell less (at 512x256x64 the
+// iteration took 185 -> 175 us).  On smaller grids phase A stores p (pin) and C reads it:
+// there the recomputing form measured slower (the C launch 18 -> 25 us at 256x128x32),
+// the stored p being still in the caches.
 template <bool NT, typename T>
 __global__ void __launch_bounds__(kThreads)
 k_march_C(Geo g, T *__restrict__ r, const T *__restrict__ ap, T *__restrict__ q,
-          const T *__restrict__ as, const T *__restrict__ r0, T *__restrict__ x,
-          const double *part_in, double *__restrict__ part_out, KState *st)
+          const T *__restrict__ pin, const T *__restrict__ as, const T *__restrict__ r0,
+          T *__restrict__ x, const double *part_in, double *__restrict__ part_out, KState *st)
 {
     if (st->done) return;
     int b = blockIdx.x;
@@ -1375,8 +1445,13 @@ k_march_C(Geo g, T *__restrict__ r, const T *__restrict__ ap, T *__restrict__ q,
                 // they measured slower)
                 const T apn = ld_stream<NT>(ap + n);
                 const T rold = ld_stream<NT>(r + n);
-                const T qv = ld_stream<NT>(q + n);
-                const T pn = mode == 0 ? rold + betaT * qv : rold;
+                T pn;
+                if (NT) {
+                    const T qv = ld_stream<NT>(q + n);
+                    pn = mode == 0 ? rold + betaT * qv : rold;
+                } else {
+                    pn = pin[n];
+                }
                 const T sn = rold - alpha * apn;
                 st_stream<NT>(ld_stream<NT>(x + n) + (alpha * pn + om * sn), x + n);
                 const T rn = sn - om * ld_stream<NT>(as + n);
@@ -2418,8 +2493,12 @@ void launch_apply(const Geo &g, hipStream_t s, int pre, const double *x, double 
 void launch_bicg_init(const Geo &g, hipStream_t s, int pre, const double *b, const double *x,
                       double *r0, double *r, double *px1, double *part, KState *st)
 {
-    hipLaunchKernelGGL(k_bicg_init, dim3(grid_blocks(g)), dim3(kThreads), 0, s, g, pre, b, x,
-                       r0, r, px1, part, st);
+    if (g.nzc % 4 == 0)
+        hipLaunchKernelGGL(k_bicg_init4, dim3(grid_blocks(g)), dim3(kThreads), 0, s, g, pre, b,
+                           x, r0, r, px1, part, st);
+    else
+        hipLaunchKernelGGL(k_bicg_init, dim3(grid_blocks(g)), dim3(kThreads), 0, s, g, pre, b,
+                           x, r0, r, px1, part, st);
 }
 
 static inline int march_blocks(const Geo &g)
@@ -2488,6 +2567,7 @@ void launch_bicg_A(const Geo &g, hipStream_t s, int pre, BicgBuf<T> &B, double *
     io.r = B.r;
     io.q = B.q;
     io.r0 = B.r0;
+    io.out_p = B.p;
     io.out_ap = B.ap;
     io.part = part[0];
     launch_march<0>(g, s, pre, io, st, m, part[2]);
@@ -2510,11 +2590,11 @@ void launch_bicg_C(const Geo &g, hipStream_t s, BicgBuf<T> &B, T *x, double *con
     (void)m;
     if (march_streaming<T>(g))
         hipLaunchKernelGGL((k_march_C<true, T>), dim3(march_blocks(g)), dim3(kThreads), 0, s, g,
-                           B.r, B.ap, B.q, B.as, B.r0, x, (const double *)part[1],
+                           B.r, B.ap, B.q, B.p, B.as, B.r0, x, (const double *)part[1],
                            part[2], st);
     else
         hipLaunchKernelGGL((k_march_C<false, T>), dim3(march_blocks(g)), dim3(kThreads), 0, s,
-                           g, B.r, B.ap, B.q, B.as, B.r0, x,
+                           g, B.r, B.ap, B.q, B.p, B.as, B.r0, x,
                            (const double *)part[1], part[2], st);
 }
 #define CFDMX_BICG_INSTANCES(T)                                                               \
