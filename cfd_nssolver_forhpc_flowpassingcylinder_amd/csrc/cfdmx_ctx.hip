@@ -573,6 +573,16 @@ static int setup_device(cfdmx_ctx *ctx)
     g.zper = c.z_periodic ? 1 : 0;
     g.nblocks = (g.N + kPerBlock - 1) / kPerBlock;
     g.xcd_remap = c.xcd_remap ? 1 : 0;
+    {
+        auto pow2 = [](double x) {
+            int e = 0;
+            return x > 0.0 && std::frexp(x, &e) == 0.5;
+        };
+        bool z2 = true;
+        for (int k = 0; k < (int)ctx->Dz.size(); ++k)
+            z2 = z2 && pow2(ctx->Dz[k]) && pow2(ctx->Dzs[k]);
+        g.zpow2 = z2 ? 1 : 0;
+    }
     g.dz = make_fastdiv((unsigned)c.nz);
     g.dy = make_fastdiv((unsigned)c.ny);
     g.ntile = (g.cyz + kTile - 1) / kTile;

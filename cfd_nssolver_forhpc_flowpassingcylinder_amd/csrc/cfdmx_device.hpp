@@ -96,6 +96,7 @@ struct Geo {
     int zper;          // periodic z
     int nblocks;       // ceil(N / 1024)
     int xcd_remap;
+    int zpow2;         // every z spacing (Dz, Dzs) is a power of two
     FastDiv dz, dy;    // by nzc, by nyc
     int ntile, nchunk, nmarch; // plane tiles, plane chunks, march blocks
     // conv-diff tiles: cd_tk cells of k by cd_tj rows of j (cd_tk * cd_tj = 256), tiles

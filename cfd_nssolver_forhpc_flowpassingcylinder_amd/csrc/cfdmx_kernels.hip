@@ -19,13 +19,23 @@ struct QuickOut {
     double dm1, d0, d1; // A: g0 - g-1, g1 - g0, g2 - g1   B: h-1 - h-2, h0 - h-1, h1 - h0
 };
 
+// a / b for the metric quotients: the correctly rounded division, or, when every spacing
+// of the axis is a power of two (P2), the product with rb = 1/b, which is then exact and
+// so the same value (both are the correctly rounded a * 2^-k)
+template <bool P2>
+__device__ __forceinline__ double dv(double a, double b, double rb)
+{
+    return P2 ? a * rb : divr(a, b, rb);
+}
+
+template <bool P2 = false>
 __device__ __forceinline__ void quick_grads_A(const Axis &a, int I, const double f[5],
                                               double &gm1, double &g0, double &g1, double &g2)
 {
-    gm1 = divr(f[1] - f[0], a.D[I - 1], a.rD[I - 1]);
-    g0 = divr(f[2] - f[1], a.D[I], a.rD[I]);
-    g1 = divr(f[3] - f[2], a.D[I + 1], a.rD[I + 1]);
-    g2 = divr(f[4] - f[3], a.D[I + 2], a.rD[I + 2]);
+    gm1 = dv<P2>(f[1] - f[0], a.D[I - 1], a.rD[I - 1]);
+    g0 = dv<P2>(f[2] - f[1], a.D[I], a.rD[I]);
+    g1 = dv<P2>(f[3] - f[2], a.D[I + 1], a.rD[I + 1]);
+    g2 = dv<P2>(f[4] - f[3], a.D[I + 2], a.rD[I + 2]);
 }
 
 // face values from the four A quotients g(-1..2) = (f[q+1] - f[q]) / D[I+q-1]
@@ -41,12 +51,13 @@ __device__ __forceinline__ void quickA_g(const Axis &a, int I, double Up, double
     N = 0.5 * (f[1] + f[2]) - cn * dn;
 }
 
+template <bool P2 = false>
 __device__ __forceinline__ void quickA(const Axis &a, int I, double Up, double Un,
                                        const double f[5], double &P, double &N, double &g0,
                                        double &g1)
 {
     double gm1, g2;
-    quick_grads_A(a, I, f, gm1, g0, g1, g2);
+    quick_grads_A<P2>(a, I, f, gm1, g0, g1, g2);
     quickA_g(a, I, Up, Un, f, gm1, g0, g1, g2, P, N);
 }
 
@@ -63,14 +74,15 @@ __device__ __forceinline__ void quickB_h(const Axis &a, int I, double Up, double
     N = 0.5 * (f[1] + f[2]) - cn * dn;
 }
 
+template <bool P2 = false>
 __device__ __forceinline__ void quickB(const Axis &a, int I, double Up, double Un,
                                        const double f[5], double &P, double &N, double &hm1,
                                        double &h0)
 {
-    const double hm2 = divr(f[1] - f[0], a.Ds[I - 2], a.rDs[I - 2]);
-    hm1 = divr(f[2] - f[1], a.Ds[I - 1], a.rDs[I - 1]);
-    h0 = divr(f[3] - f[2], a.Ds[I], a.rDs[I]);
-    const double h1 = divr(f[4] - f[3], a.Ds[I + 1], a.rDs[I + 1]);
+    const double hm2 = dv<P2>(f[1] - f[0], a.Ds[I - 2], a.rDs[I - 2]);
+    hm1 = dv<P2>(f[2] - f[1], a.Ds[I - 1], a.rDs[I - 1]);
+    h0 = dv<P2>(f[3] - f[2], a.Ds[I], a.rDs[I]);
+    const double h1 = dv<P2>(f[4] - f[3], a.Ds[I + 1], a.rDs[I + 1]);
     quickB_h(a, I, Up, Un, f, hm2, hm1, h0, h1, P, N);
 }
 
@@ -135,7 +147,9 @@ __host__ __device__ inline size_t cd_lds_bytes(const Geo &g)
            sizeof(double);
 }
 
-template <int NE, bool LES>
+// ZP2: every z spacing (Dz and Dzs) is a power of two, as on the uniform grids whose nz
+// divides lz by a power of two: the z quotients are then exact products (dv)
+template <int NE, bool LES, bool ZP2>
 __global__ void __launch_bounds__(kThreads)
 k_convdiff(Geo g, const double *__restrict__ u, const double *__restrict__ v,
            const double *__restrict__ w, double *__restrict__ us, double *__restrict__ vs,
@@ -258,15 +272,15 @@ k_convdiff(Geo g, const double *__restrict__ u, const double *__restrict__ v,
                 const double fy[5] = {uc[c - 2 * RW], uc[c - RW], uc[c], uc[c + RW], uc[c + 2 * RW]};
                 quickB(Y, J, Vp, Vn, fy, uPy, uNy, ay0, ay1);
                 const double fz[5] = {uc[c - 2], uc[c - 1], uc[c], uc[c + 1], uc[c + 2]};
-                quickB(Z, K, Wp, Wn, fz, uPz, uNz, az0, az1);
+                quickB<ZP2>(Z, K, Wp, Wn, fz, uPz, uNz, az0, az1);
                 const double convection =
                     mdt * (divr(uPx * uPx - uNx * uNx, X.Ds[I], X.rDs[I]) +
                            divr(uPy * Vp - uNy * Vn, Y.D[J], Y.rD[J]) +
-                           divr(uPz * Wp - uNz * Wn, Z.D[K], Z.rD[K]));
+                           dv<ZP2>(uPz * Wp - uNz * Wn, Z.D[K], Z.rD[K]));
                 // ((u[xp]-u)/Dx[i+1] - (u-u[xm])/Dx[i]) = ax1 - ax0, etc.
                 const double difussion = nudt_c * (divr(ax1 - ax0, X.Ds[I], X.rDs[I]) +
                                                  divr(ay1 - ay0, Y.D[J], Y.rD[J]) +
-                                                 divr(az1 - az0, Z.D[K], Z.rD[K]));
+                                                 dv<ZP2>(az1 - az0, Z.D[K], Z.rD[K]));
                 const double r = uc[c] + convection + difussion;
                 us[ic] = r;
                 cfl += divr(r, X.Ds[I], X.rDs[I]);
@@ -288,14 +302,14 @@ k_convdiff(Geo g, const double *__restrict__ u, const double *__restrict__ v,
                 const double fy[5] = {vc[c - 2 * RW], vc[c - RW], vc[c], vc[c + RW], vc[c + 2 * RW]};
                 quickA(Y, J, Vp, Vn, fy, vPy, vNy, ay0, ay1);
                 const double fz[5] = {vc[c - 2], vc[c - 1], vc[c], vc[c + 1], vc[c + 2]};
-                quickB(Z, K, Wp, Wn, fz, vPz, vNz, az0, az1);
+                quickB<ZP2>(Z, K, Wp, Wn, fz, vPz, vNz, az0, az1);
                 const double convection =
                     mdt * (divr(Up * vPx - Un * vNx, X.D[I], X.rD[I]) +
                            divr(vPy * vPy - vNy * vNy, Y.Ds[J], Y.rDs[J]) +
-                           divr(Wp * vPz - Wn * vNz, Z.D[K], Z.rD[K]));
+                           dv<ZP2>(Wp * vPz - Wn * vNz, Z.D[K], Z.rD[K]));
                 const double difussion = nudt_c * (divr(ax1 - ax0, X.D[I], X.rD[I]) +
                                                  divr(ay1 - ay0, Y.Ds[J], Y.rDs[J]) +
-                                                 divr(az1 - az0, Z.D[K], Z.rD[K]));
+                                                 dv<ZP2>(az1 - az0, Z.D[K], Z.rD[K]));
                 const double r = vc[c] + convection + difussion;
                 vs[ic] = r;
                 cfl += divr(r, Y.Ds[J], Y.rDs[J]);
@@ -317,17 +331,17 @@ k_convdiff(Geo g, const double *__restrict__ u, const double *__restrict__ v,
                 const double fy[5] = {wc[c - 2 * RW], wc[c - RW], wc[c], wc[c + RW], wc[c + 2 * RW]};
                 quickB(Y, J, Vp, Vn, fy, wPy, wNy, ay0, ay1);
                 const double fz[5] = {wc[c - 2], wc[c - 1], wc[c], wc[c + 1], wc[c + 2]};
-                quickA(Z, K, Wp, Wn, fz, wPz, wNz, az0, az1);
+                quickA<ZP2>(Z, K, Wp, Wn, fz, wPz, wNz, az0, az1);
                 const double convection =
                     mdt * (divr(Up * wPx - Un * wNx, X.D[I], X.rD[I]) +
                            divr(Vp * wPy - Vn * wNy, Y.D[J], Y.rD[J]) +
-                           divr(wPz * wPz - wNz * wNz, Z.Ds[K], Z.rDs[K]));
+                           dv<ZP2>(wPz * wPz - wNz * wNz, Z.Ds[K], Z.rDs[K]));
                 const double difussion = nudt_c * (divr(ax1 - ax0, X.D[I], X.rD[I]) +
                                                  divr(ay1 - ay0, Y.D[J], Y.rD[J]) +
-                                                 divr(az1 - az0, Z.Ds[K], Z.rDs[K]));
+                                                 dv<ZP2>(az1 - az0, Z.Ds[K], Z.rDs[K]));
                 const double r = wc[c] + convection + difussion;
                 ws[ic] = r;
-                cfl += divr(r, Z.Ds[K], Z.rDs[K]);
+                cfl += dv<ZP2>(r, Z.Ds[K], Z.rDs[K]);
             }
             cmax = cfl > cmax ? cfl : cmax;
             if (g.zper) {
@@ -2418,21 +2432,28 @@ void launch_convdiff(const Geo &g, hipStream_t s, const double *u, const double 
     const int nb = g.xcd_remap ? ((g.cd_nb + 7) / 8) * 8 : g.cd_nb;
     const int P = (g.cd_tj + 4) * (g.cd_tk + 4);
     const size_t lds = cd_lds_bytes(g);
+#define CFDMX_CD(NE, LES, ZP2)                                                                \
+    hipLaunchKernelGGL((k_convdiff<NE, LES, ZP2>), dim3(nb), dim3(kThreads), lds, s, g, u, v, w, \
+                       us, vs, ws, -dt, nu * dt, dt, cfl_part, nu, viseff)
+    const bool les = viseff != nullptr, z2 = g.zpow2 != 0;
     if (P <= 2 * kThreads) {
-        if (viseff)
-            hipLaunchKernelGGL((k_convdiff<2, true>), dim3(nb), dim3(kThreads), lds, s, g, u, v,
-                               w, us, vs, ws, -dt, nu * dt, dt, cfl_part, nu, viseff);
-        else
-            hipLaunchKernelGGL((k_convdiff<2, false>), dim3(nb), dim3(kThreads), lds, s, g, u, v,
-                               w, us, vs, ws, -dt, nu * dt, dt, cfl_part, nu, viseff);
+        if (les) {
+            if (z2) CFDMX_CD(2, true, true);
+            else CFDMX_CD(2, true, false);
+        } else {
+            if (z2) CFDMX_CD(2, false, true);
+            else CFDMX_CD(2, false, false);
+        }
     } else {
-        if (viseff)
-            hipLaunchKernelGGL((k_convdiff<3, true>), dim3(nb), dim3(kThreads), lds, s, g, u, v,
-                               w, us, vs, ws, -dt, nu * dt, dt, cfl_part, nu, viseff);
-        else
-            hipLaunchKernelGGL((k_convdiff<3, false>), dim3(nb), dim3(kThreads), lds, s, g, u, v,
-                               w, us, vs, ws, -dt, nu * dt, dt, cfl_part, nu, viseff);
+        if (les) {
+            if (z2) CFDMX_CD(3, true, true);
+            else CFDMX_CD(3, true, false);
+        } else {
+            if (z2) CFDMX_CD(3, false, true);
+            else CFDMX_CD(3, false, false);
+        }
     }
+#undef CFDMX_CD
 }
 
 void launch_bc(const Geo &g, hipStream_t s, const BCParams &bc, int advance, double *u,
