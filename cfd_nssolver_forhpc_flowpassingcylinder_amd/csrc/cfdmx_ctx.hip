@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -28,6 +29,10 @@ struct cfdmx_ctx {
     double *Xn = nullptr, *Yn = nullptr;
     double *t0[3] = {}, *t1[3] = {}, *t3[3] = {};
     double *p = nullptr, *eta = nullptr, *f = nullptr;
+    // every z ghost of f (k = 0, 1, nz-2, nz-1: entries nothing computes) holds +0.0, as the
+    // reset leaves them; an upload or a checkpoint load tests the values it brings.  The
+    // time step's projection then stores them too, so that f's rows go out whole
+    int f_ghosts_zero = 0;
     double *x = nullptr, *b = nullptr, *px1 = nullptr, *px2 = nullptr;
     BicgBuffers B = {};
     // multigrid-preconditioned solve (mixed precision: float; FP64 multigrid: double):
@@ -723,6 +728,20 @@ static int setup_device(cfdmx_ctx *ctx)
     return CFDMX_OK;
 }
 
+// whether every z ghost of the three components of a force field (3*G doubles) is +0.0
+static bool force_zghosts_zero(const Geo &g, const double *f)
+{
+    const int ks[4] = {0, 1, g.nz - 2, g.nz - 1};
+    for (int q = 0; q < 3; ++q)
+        for (size_t row = 0; row < (size_t)g.nx * g.ny; ++row)
+            for (int k : ks) {
+                uint64_t bits;
+                std::memcpy(&bits, f + (size_t)q * g.G + row * g.nz + k, sizeof bits);
+                if (bits) return false;
+            }
+    return true;
+}
+
 // the pressure solve's state of a new run: X_result and its history, the Krylov vectors,
 // the forcing and the solver states (a fresh bicgstabRe2, bicgstabRe2.hpp:17-52), the
 // counters of the step report
@@ -731,6 +750,7 @@ static int reset_run_state(cfdmx_ctx *ctx)
     hipStream_t s = ctx->stream;
     const size_t G = (size_t)ctx->g.G, N = (size_t)ctx->g.N;
     launch_fill(s, (int)(3 * G), 0.0, ctx->f);
+    ctx->f_ghosts_zero = 1;
     double *zs[] = {ctx->x, ctx->b, ctx->px1, ctx->px2, ctx->B.r0, ctx->B.r, ctx->B.as,
                     ctx->B.q, ctx->B.p, ctx->B.ap};
     for (double *z : zs) launch_fill(s, (int)N, 0.0, z);
@@ -1028,7 +1048,10 @@ int cfdmx_download_force(cfdmx_ctx *ctx, double *f)
 
 int cfdmx_upload_force(cfdmx_ctx *ctx, const double *f)
 {
-    return ctx ? copy_h2d(ctx, ctx->f, f, 3 * (size_t)ctx->g.G) : CFDMX_ERR_ARG;
+    if (!ctx || !f) return CFDMX_ERR_ARG;
+    const int rc = copy_h2d(ctx, ctx->f, f, 3 * (size_t)ctx->g.G);
+    if (!rc) ctx->f_ghosts_zero = force_zghosts_zero(ctx->g, f) ? 1 : 0;
+    return rc;
 }
 
 int cfdmx_set_inflow(cfdmx_ctx *ctx, const double *profile)
@@ -1583,7 +1606,8 @@ static int step_once(cfdmx_ctx *ctx, int *iters, double *err, int *solved)
          launch_project(g, s, from_x, c.dt, c.outflow_convective, ctx->x, ctx->p, ctx->eta,
                         ctx->t1[0], ctx->t1[1], ctx->t1[2], ctx->t3[0], ctx->t3[1], ctx->t3[2],
                         ctx->f, ctx->part, c.steady_check ? prev : nullptr,
-                        c.steady_check ? ctx->diff_part : nullptr));
+                        c.steady_check ? ctx->diff_part : nullptr,
+                        ctx->f_ghosts_zero ? 2 : 1)); // launch_bc below rewrites the z ghosts
     launch_cdcl_fin(g, s, ctx->part, c.lz, ctx->scal + 1);
     if (c.steady_check) launch_max3_fin(g, s, ctx->diff_part, ctx->scal + 3);
     for (int q = 0; q < 3; ++q) std::swap(ctx->t0[q], ctx->t3[q]);
@@ -1877,6 +1901,7 @@ int cfdmx_checkpoint_load(cfdmx_ctx *ctx, const char *path)
             std::fclose(fp);
             return rc;
         }
+        if (a.first == ctx->f) ctx->f_ghosts_zero = force_zghosts_zero(ctx->g, buf.data()) ? 1 : 0;
     }
     std::fclose(fp);
     *ctx->kst_h = h.kst;

@@ -162,12 +162,15 @@ void launch_sor_rb(const Geo &g, hipStream_t s, int colour, double omega, const 
                    const double *cf6c, double *pc, const double *p, double *part, SorState *st);
 void launch_sor_rb_fin(const Geo &g, hipStream_t s, double *part, SorState *st);
 // prev (T0 of the step) and dpart (3 * nblocks doubles) add the steady-state check;
-// both null leaves it out
+// both null leaves it out.  ghosts (grids with nzc a multiple of 4): 0 leaves every z
+// ghost untouched; 1 also stores the z ghosts of u, v, w and p in the rows written, for a
+// caller whose boundary conditions rewrite them next; 2 as 1 and f's z ghosts as zeros,
+// for a caller that knows they are zero
 void launch_project(const Geo &g, hipStream_t s, int from_x, double dt, int convective,
                     const double *xres, double *p, const double *eta, const double *us,
                     const double *vs, const double *ws, double *u, double *v, double *w,
                     double *f, double *part, const double *const *prev = nullptr,
-                    double *dpart = nullptr);
+                    double *dpart = nullptr, int ghosts = 0);
 void launch_max3_fin(const Geo &g, hipStream_t s, const double *dpart, double *out);
 void launch_cdcl_fin(const Geo &g, hipStream_t s, const double *part, double lz, double *out);
 void launch_force_partial(const Geo &g, hipStream_t s, const double *f, double *part);

@@ -2110,7 +2110,8 @@ k_project(Geo g, double dt, double rdt, int convective, const double *__restrict
             w[ic] = o.wn;
             f[ic + 2 * g.G] = o.fz;
         }
-        const double fx = ox ? o.fx : 0.0, fy = oy ? o.fy : 0.0;
+        // CD_CL sums every interior entry of f: a face not written here keeps its value
+        const double fx = ox ? o.fx : f[ic], fy = oy ? o.fy : f[ic + g.G];
         const double vol = X.D[I] * Y.D[J] * Z.D[K];
         acc[0] += fx * vol;
         acc[1] += fy * vol;
@@ -2133,7 +2134,12 @@ k_project(Geo g, double dt, double rdt, int convective, const double *__restrict
 // k_project.  The drag partials keep k_project's order: the per-cell terms go through LDS
 // and each thread adds the cells k_project gives it (c * kThreads + thread), so the
 // partial sums, and Cd and Cl, are the same bits.
-template <int FROM_X, int DIFF>
+// GW > 0 also stores the z ghosts (k = 0, 1 and nz-2, nz-1) of the rows it writes, so
+// that each row goes out whole: HBM takes rows with unwritten gaps between them ~1.4x
+// slower.  The ghosts of u, v, w and p get values the boundary conditions of the same
+// step replace (k_bc_z rewrites them, and nothing reads them before); GW == 2 also
+// stores f's ghosts, with the zeros the caller knows they hold.
+template <int FROM_X, int DIFF, int GW>
 __global__ void __launch_bounds__(kThreads)
 k_project4(Geo g, double dt, double rdt, int convective, const double *__restrict__ xres,
            double *__restrict__ p, const double *__restrict__ eta,
@@ -2180,6 +2186,11 @@ k_project4(Geo g, double dt, double rdt, int convective, const double *__restric
         load4(us + ic, uc);
         load4(vs + ic, vc);
         load4(ws + ic, wc);
+        // CD_CL sums every interior entry of f: on the last x (y) plane, whose x (y) faces
+        // are not written here, the sums take the values f holds
+        double fxo[4] = {0.0, 0.0, 0.0, 0.0}, fyo[4] = {0.0, 0.0, 0.0, 0.0};
+        if (!ox) load4(f + ic, fxo);
+        if (!oy) load4(f + g.G + ic, fyo);
         double un[4], vn[4], wn[4], fx[4], fy[4], fz[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -2190,8 +2201,8 @@ k_project4(Geo g, double dt, double rdt, int convective, const double *__restric
             un[q] = o.un, vn[q] = o.vn, wn[q] = o.wn;
             fx[q] = o.fx, fy[q] = o.fy, fz[q] = o.fz;
             const double vol = X.D[I] * Y.D[J] * Z.D[K0 + q];
-            sfx[4 * threadIdx.x + q] = (ox ? o.fx : 0.0) * vol;
-            sfy[4 * threadIdx.x + q] = (oy ? o.fy : 0.0) * vol;
+            sfx[4 * threadIdx.x + q] = (ox ? o.fx : fxo[q]) * vol;
+            sfy[4 * threadIdx.x + q] = (oy ? o.fy : fyo[q]) * vol;
         }
         if (FROM_X) Vec4T<double>::store(p + ic, pc);
         if (ox) {
@@ -2212,6 +2223,26 @@ k_project4(Geo g, double dt, double rdt, int convective, const double *__restric
             w[ic + 2] = wn[2];
             reinterpret_cast<double2 *>(f + 2 * g.G + ic)[0] = make_double2(fz[0], fz[1]);
             f[2 * g.G + ic + 2] = fz[2];
+        }
+        if (GW) {
+            // the two ghosts before the row's first cell and after its last
+            for (int side = 0; side < 2; ++side) {
+                if (side == 0 ? K0 != kGC : !last) continue;
+                const int o = side == 0 ? ic - 2 : ic + 4, e = side == 0 ? 0 : 3;
+                auto put = [o](double *a, double val) {
+                    *reinterpret_cast<double2 *>(a + o) = make_double2(val, val);
+                };
+                if (FROM_X) put(p, pc[e]);
+                if (ox) put(u, un[e]);
+                else if (convective) put(u, uc[e]);
+                if (oy) put(v, vn[e]);
+                put(w, wn[e]);
+                if (GW == 2) {
+                    if (ox) put(f, 0.0);
+                    if (oy) put(f + g.G, 0.0);
+                    put(f + 2 * g.G, 0.0);
+                }
+            }
         }
         if (DIFF) {
             double a[4], b[4], c[4];
@@ -2753,27 +2784,35 @@ void launch_sor_fin(const Geo &g, hipStream_t s, double *part, SorState *st)
 void launch_project(const Geo &g, hipStream_t s, int from_x, double dt, int convective,
                     const double *xres, double *p, const double *eta, const double *us,
                     const double *vs, const double *ws, double *u, double *v, double *w,
-                    double *f, double *part, const double *const *prev, double *dpart)
+                    double *f, double *part, const double *const *prev, double *dpart,
+                    int ghosts)
 {
     const double rdt = 1.0 / dt;
     const dim3 grid(grid_blocks(g)), block(kThreads);
     const double *u0 = prev ? prev[0] : nullptr, *v0 = prev ? prev[1] : nullptr,
                  *w0 = prev ? prev[2] : nullptr;
-#define CFDMX_PROJECT(KERNEL, FX, DF)                                                         \
-    hipLaunchKernelGGL((KERNEL<FX, DF>), grid, block, 0, s, g, dt, rdt, convective, xres, p,  \
-                       eta, us, vs, ws, u, v, w, f, part, u0, v0, w0, dpart)
+#define CFDMX_PROJECT(KERNEL, ...)                                                            \
+    hipLaunchKernelGGL((KERNEL<__VA_ARGS__>), grid, block, 0, s, g, dt, rdt, convective,      \
+                       xres, p, eta, us, vs, ws, u, v, w, f, part, u0, v0, w0, dpart)
+#define CFDMX_PROJECT4(FX, DF)                                                                \
+    do {                                                                                      \
+        if (ghosts == 2) CFDMX_PROJECT(k_project4, FX, DF, 2);                                \
+        else if (ghosts == 1) CFDMX_PROJECT(k_project4, FX, DF, 1);                           \
+        else CFDMX_PROJECT(k_project4, FX, DF, 0);                                            \
+    } while (0)
     const int df = dpart ? 1 : 0;
     if (g.nzc % 4 == 0) { // four cells per thread (k_project4 covers a block four per thread)
-        if (from_x && df) CFDMX_PROJECT(k_project4, 1, 1);
-        else if (from_x) CFDMX_PROJECT(k_project4, 1, 0);
-        else if (df) CFDMX_PROJECT(k_project4, 0, 1);
-        else CFDMX_PROJECT(k_project4, 0, 0);
+        if (from_x && df) CFDMX_PROJECT4(1, 1);
+        else if (from_x) CFDMX_PROJECT4(1, 0);
+        else if (df) CFDMX_PROJECT4(0, 1);
+        else CFDMX_PROJECT4(0, 0);
     } else {
         if (from_x && df) CFDMX_PROJECT(k_project, 1, 1);
         else if (from_x) CFDMX_PROJECT(k_project, 1, 0);
         else if (df) CFDMX_PROJECT(k_project, 0, 1);
         else CFDMX_PROJECT(k_project, 0, 0);
     }
+#undef CFDMX_PROJECT4
 #undef CFDMX_PROJECT
 }
 

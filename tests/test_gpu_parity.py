@@ -202,6 +202,30 @@ def test_time_steps_bitwise(case):
     gpu.close()
 
 
+@pytest.mark.parametrize("n,bc,ghosts", [((40, 20, 4), "cylinder", "random"),
+                                         ((32, 16, 8), "flowpassing", "random"),
+                                         ((32, 16, 8), "cylinder", "zero")])
+def test_uploaded_force_ghosts_survive_steps(n, bc, ghosts):
+    """The step's projection stores whole rows of f (z ghosts included) only while every
+    z ghost of f is +0.0; a force uploaded with other ghost values keeps them through the
+    steps, as the reference's update_UandF never writes them."""
+    gpu, sim, o, g = make_pair(*n, bc=bc, precond=1, tol=1e-8)
+    rng = np.random.default_rng(7)
+    f = rng.standard_normal(3 * g.G)
+    if ghosts == "zero":
+        f.reshape(3, -1, g.shape()[2])[:, :, [0, 1, -2, -1]] = 0.0
+    gpu.set_force(f)
+    sim.field("f")[:] = f
+    for s in range(3):
+        a = gpu.step(1)
+        b = sim.step()
+        assert _same(a["cd"], b["cd"]) and _same(a["iters"], b["iters"]), (s, a, b)
+        assert fields_equal(gpu, sim) == [], s
+    zg = gpu.force().reshape(3, -1, g.shape()[2])[:, :, [0, 1, -2, -1]]
+    assert np.array_equal(zg, f.reshape(3, -1, g.shape()[2])[:, :, [0, 1, -2, -1]])
+    gpu.close()
+
+
 def test_xcd_remap_does_not_change_results():
     a, sa, _, _ = make_pair(40, 20, 6, bc="cylinder", precond=1, tol=1e-8, xcd_remap=1)
     b, sb, _, _ = make_pair(40, 20, 6, bc="cylinder", precond=1, tol=1e-8, xcd_remap=0)
