@@ -112,6 +112,9 @@ struct MGLevelT {
 // the hierarchy stops at the first level of at most this many cells, solved exactly
 constexpr int kMGCoarseCells = 512;
 constexpr int kMGCoarseAxis = 512; // largest axis length of that level
+// a level above the coarsest of at most this many cells (stored by colour) runs its whole
+// V-cycle, the coarse solve included, in one workgroup with its u in LDS (k_mg_tail)
+constexpr int kMGTailCells = 4096;
 
 struct PBState {
     double rho, alpha, omega, beta, norm0, res, tol;

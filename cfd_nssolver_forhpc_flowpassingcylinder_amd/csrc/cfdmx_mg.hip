@@ -160,16 +160,23 @@ __device__ __forceinline__ T mg_split_update(const MGLevelT<T> &L, const T *u, i
     return s;
 }
 
+// the half sweep of colour c at z-pair m (the kernels below and the tail kernel)
 template <typename T>
-__global__ void __launch_bounds__(256) k_mg_smooth(MGLevelT<T> L, int colour)
+__device__ __forceinline__ void mg_smooth_at(const MGLevelT<T> &L, int m, int colour)
 {
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= (L.n >> 1)) return;
     int i, j, k0, row;
     mg_pair(L, m, i, j, k0, row);
     int own, k;
     const T s = mg_split_update(L, L.u, m, row, i, j, k0 >> 1, colour, own, k);
     L.u[own] = (L.f[own] + s) / mg_diag(L, i, j, k);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_mg_smooth(MGLevelT<T> L, int colour)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= (L.n >> 1)) return;
+    mg_smooth_at(L, m, colour);
 }
 
 // The same half sweep with two neighbouring cells of a colour per thread (FP64 levels
@@ -264,10 +271,8 @@ __global__ void __launch_bounds__(256) k_mg_smooth_out(MGLevelT<T> L)
 // first sweep from u = 0: red u = f/d, then black u = (f + sum c (f/d)(red nb)) / d.
 // On level 0 f comes in natural order (fin) and is stored by colour here.
 template <typename T>
-__global__ void __launch_bounds__(256) k_mg_smooth_first(MGLevelT<T> L)
+__device__ __forceinline__ void mg_smooth_first_at(const MGLevelT<T> &L, int m)
 {
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= (L.n >> 1)) return;
     int i, j, k0, row;
     mg_pair(L, m, i, j, k0, row);
     const int hn = L.n >> 1;
@@ -291,12 +296,19 @@ __global__ void __launch_bounds__(256) k_mg_smooth_first(MGLevelT<T> L)
     L.u[hn + m] = (fb + s) / mg_diag(L, i, j, kb);
 }
 
-// prolongation fused with the red half sweep (see the file comment)
 template <typename T>
-__global__ void __launch_bounds__(256) k_mg_prolong_smooth(MGLevelT<T> F, MGLevelT<T> C)
+__global__ void __launch_bounds__(256) k_mg_smooth_first(MGLevelT<T> L)
 {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= (F.n >> 1)) return;
+    if (m >= (L.n >> 1)) return;
+    mg_smooth_first_at(L, m);
+}
+
+// prolongation fused with the red half sweep (see the file comment)
+template <typename T>
+__device__ __forceinline__ void mg_prolong_smooth_at(const MGLevelT<T> &F, const MGLevelT<T> &C,
+                                                     int m)
+{
     int i, j, k0, row;
     mg_pair(F, m, i, j, k0, row);
     const int k = k0 + ((i + j) & 1);
@@ -306,6 +318,14 @@ __global__ void __launch_bounds__(256) k_mg_prolong_smooth(MGLevelT<T> F, MGLeve
         return u[mg_at(F, a, b, c)] + e[mg_at(C, a >> shx, b >> shy, c >> shz)];
     });
     F.u[m] = (F.f[m] + s) / mg_diag(F, i, j, k);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_mg_prolong_smooth(MGLevelT<T> F, MGLevelT<T> C)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= (F.n >> 1)) return;
+    mg_prolong_smooth_at(F, C, m);
 }
 
 // ---- z-only coarsening (fx = fy = 1, fz = 2) of a level stored by colour: the
@@ -411,10 +431,8 @@ __global__ void __launch_bounds__(256) k_mg_prolong(MGLevelT<T> F, MGLevelT<T> C
 
 // coarse right-hand side: mean over the children of the fine residual f - A u
 template <typename T>
-__global__ void __launch_bounds__(256) k_mg_restrict(MGLevelT<T> F, MGLevelT<T> C)
+__device__ __forceinline__ void mg_restrict_at(const MGLevelT<T> &F, const MGLevelT<T> &C, int m)
 {
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= C.n) return;
     int I, J, K;
     mg_coords(C, m, I, J, K);
     const T *u = F.u;
@@ -438,6 +456,14 @@ __global__ void __launch_bounds__(256) k_mg_restrict(MGLevelT<T> F, MGLevelT<T> 
     C.f[mg_at(C, I, J, K)] = acc / (T)(C.fx * C.fy * C.fz);
 }
 
+template <typename T>
+__global__ void __launch_bounds__(256) k_mg_restrict(MGLevelT<T> F, MGLevelT<T> C)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= C.n) return;
+    mg_restrict_at(F, C, m);
+}
+
 // coarsest level: the exact solve by fast diagonalisation.  The level's operator is the
 // Kronecker sum of three 1-D chains with orthonormal eigenvectors Qx, Qy, Qz, so
 // u = Q diag(1 / lambda) Q^T f with Q = Qx (x) Qy (x) Qz and lambda = lx + ly + lz; the
@@ -446,10 +472,8 @@ __global__ void __launch_bounds__(256) k_mg_restrict(MGLevelT<T> F, MGLevelT<T> 
 // a barrier apart.
 constexpr int kMGCoarseThreads = 1024;
 template <typename T>
-__global__ void __launch_bounds__(kMGCoarseThreads) k_mg_coarse(MGLevelT<T> L)
+__device__ __forceinline__ void mg_coarse_solve(const MGLevelT<T> &L, T *a, T *b)
 {
-    extern __shared__ double coarse_lds[];
-    T *a = reinterpret_cast<T *>(coarse_lds), *b = a + L.n;
     const int nx = L.nx, ny = L.ny, nz = L.nz;
     const T *Qx = L.fd, *Qy = Qx + nx * nx, *Qz = Qy + ny * ny;
     const T *lx = Qz + nz * nz, *ly = lx + nx, *lz = ly + ny;
@@ -506,6 +530,70 @@ __global__ void __launch_bounds__(kMGCoarseThreads) k_mg_coarse(MGLevelT<T> L)
         for (int m = 0; m < nx; ++m) acc += Qx[m * nx + i] * b[(m * ny + j) * nz + k];
         L.u[n] = acc;
     }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kMGCoarseThreads) k_mg_coarse(MGLevelT<T> L)
+{
+    extern __shared__ double coarse_lds[];
+    T *a = reinterpret_cast<T *>(coarse_lds);
+    mg_coarse_solve(L, a, a + L.n);
+}
+
+// The two coarsest levels in one workgroup: the level F above the coarsest (stored by
+// colour) from its first sweep to its last, with the restriction to the coarsest level
+// C, C's exact solve and the prolongation between.  These levels hold a few thousand
+// cells, where a launch per half sweep costs more than the sweep.  Here F's u (and its
+// f, where both fit in 64 KiB of LDS with C's two work arrays) stay in LDS from the
+// first step to the last, the steps a barrier apart, and only F's final u goes back to
+// global memory for the prolongation to the finer level.  Every cell is updated by the
+// same device function as in the per-step kernels (the restriction and prolongation by
+// their general forms, which give the same values as the z-only ones), so the result
+// is the same bits.
+constexpr size_t kMGTailLds = 65536;
+template <typename T>
+__host__ __device__ inline bool mg_tail_f_in_lds(int fn, int cn)
+{
+    return (2 * (size_t)fn + 2 * (size_t)cn) * sizeof(T) <= kMGTailLds;
+}
+
+template <typename T, bool F_LDS>
+__global__ void __launch_bounds__(kMGCoarseThreads) k_mg_tail(MGLevelT<T> F, MGLevelT<T> C)
+{
+    extern __shared__ double coarse_lds[];
+    const int tid = threadIdx.x, nt = blockDim.x, hn = F.n >> 1;
+    T *su = reinterpret_cast<T *>(coarse_lds);
+    T *sf = su + F.n;
+    T *ca = F_LDS ? sf + F.n : sf, *cb = ca + C.n;
+    MGLevelT<T> FL = F, CL = C;
+    FL.u = su;
+    if (F_LDS) {
+        FL.f = sf;
+        for (int n = tid; n < F.n; n += nt) sf[n] = F.f[n];
+        __syncthreads();
+    }
+    // C's right-hand side is built in the first work array and its result left there
+    CL.f = ca;
+    CL.u = ca;
+    for (int m = tid; m < hn; m += nt) mg_smooth_first_at(FL, m);
+    __syncthreads();
+    for (int m = tid; m < hn; m += nt) mg_smooth_at(FL, m, 0);
+    __syncthreads();
+    for (int m = tid; m < hn; m += nt) mg_smooth_at(FL, m, 1);
+    __syncthreads();
+    for (int m = tid; m < C.n; m += nt) mg_restrict_at(FL, CL, m);
+    __syncthreads();
+    mg_coarse_solve(CL, ca, cb);
+    __syncthreads();
+    for (int m = tid; m < hn; m += nt) mg_prolong_smooth_at(FL, CL, m);
+    __syncthreads();
+    for (int m = tid; m < hn; m += nt) mg_smooth_at(FL, m, 1);
+    __syncthreads();
+    for (int m = tid; m < hn; m += nt) mg_smooth_at(FL, m, 0);
+    __syncthreads();
+    for (int m = tid; m < hn; m += nt) mg_smooth_at(FL, m, 1);
+    __syncthreads();
+    for (int n = tid; n < F.n; n += nt) F.u[n] = su[n];
 }
 
 // ---- preconditioned BiCGSTAB pieces (linear cell order, canonical block sums) ----
@@ -738,6 +826,18 @@ void launch_mg_vcycle(hipStream_t s, const MGLevelT<T> *L, int nlev, int l)
         return;
     }
     const MGLevelT<T> &C = L[l + 1];
+    const size_t tail_lds =
+        ((mg_tail_f_in_lds<T>(F.n, C.n) ? 2 : 1) * (size_t)F.n + 2 * (size_t)C.n) * sizeof(T);
+    if (l > 0 && l == nlev - 2 && (F.nz & 1) == 0 && F.n <= kMGTailCells &&
+        tail_lds <= kMGTailLds) {
+        if (mg_tail_f_in_lds<T>(F.n, C.n))
+            hipLaunchKernelGGL((k_mg_tail<T, true>), dim3(1), dim3(kMGCoarseThreads), tail_lds, s,
+                               F, C);
+        else
+            hipLaunchKernelGGL((k_mg_tail<T, false>), dim3(1), dim3(kMGCoarseThreads), tail_lds, s,
+                               F, C);
+        return;
+    }
     if ((F.nz & 1) == 0) {
         const int nb = cdiv(F.n >> 1, 256);
         // a plain half sweep: in FP64 two cells of a colour per thread where rows hold
