@@ -48,7 +48,7 @@ BYTES_PER_CELL = {
 # per iteration (32 + 24 + 72)
 ITERATION_PHASES = ("bicg_A", "bicg_B", "bicg_C")
 
-MIXED_INNER_TOL = 0.0  # the library's per-mode default: 3e-3 mixed, 1e-5 FP64 multigrid
+MIXED_INNER_TOL = 0.0  # the library's per-mode default: 1e-2 mixed, 1e-5 FP64 multigrid
 
 # multigrid V-cycles from captured HIP graphs (cfdmx_config.mg_graphs); --no-mg-graphs
 # launches them kernel by kernel (the same bits and, at these sizes, the same time)
@@ -644,6 +644,35 @@ def rbsor_run(wl, args):
     }
 
 
+def sustained_run(wl, args, warm=100, steps=20):
+    """Both multigrid modes past the start-up: `warm` untimed steps from the seeded field,
+    then `steps` timed ones, each mode at its default inner tolerance.  The mixed line's
+    own window (the FP64 run's warm-up and timed steps, for the field errors) lies in the
+    first steps, where the pressure still changes fast and every solve needs more inner
+    iterations."""
+    out = {"warmup_steps": warm, "steps": steps}
+    for name, precision in (("mixed", 1), ("fp64_multigrid", 2)):
+        sv = make_solver(wl, "none", "bicgstab", args.seed, 5000, precision)
+        for _ in range(warm):
+            sv.step(1)
+        sv.synchronize()
+        its = []
+        t = time.perf_counter()
+        for _ in range(steps):
+            its.append(sv.step(1)["iters"])
+        sv.synchronize()
+        wall = time.perf_counter() - t
+        sv.close()
+        out[name] = {"value": round(wl["nx"] * wl["ny"] * wl["nz"] * steps / wall / 1e6, 4),
+                     "unit": "Mcell-steps/s", "ms_per_step": round(wall / steps * 1e3, 3),
+                     "mean_inner_iters": sum(its) / steps}
+        print(f"sustained {name}: {out[name]['ms_per_step']} ms per step", file=sys.stderr,
+              flush=True)
+    out["speedup_vs_fp64_multigrid"] = round(out["mixed"]["value"]
+                                             / out["fp64_multigrid"]["value"], 3)
+    return out
+
+
 def c3_run(wl, args):
     """The mixed-precision configuration at its full length: 512x256x64, Re 200, dt 0.001,
     args.c3_steps (500) steps from the seeded field, FP32 inner BiCGSTAB to 1e-5 with FP64
@@ -654,7 +683,9 @@ def c3_run(wl, args):
     import numpy as np
     n = args.c3_steps
     shp = (wl["nx"] + 4, wl["ny"] + 4, wl["nz"] + 4)
-    modes = [("mixed", 1, 1e-5), ("fp64_multigrid", 2, 1e-5)]
+    # the configuration's inner tolerance 1e-5 in both multigrid modes, and the mixed mode
+    # at its own default (1e-2) beside it
+    modes = [("mixed", 1, 1e-5), ("fp64_multigrid", 2, 1e-5), ("mixed_default_inner_tol", 1, 0.0)]
     if not args.c3_no_fp64:
         modes.append(("fp64", 0, None))
     runs = {}
@@ -687,29 +718,35 @@ def c3_run(wl, args):
     out = {"config": (f"{wl['nx']}x{wl['ny']}x{wl['nz']}, Re={wl['re']:g}, dt={wl['dt']:g}, "
                       f"{n} steps from the seeded field; mixed: FP32 inner BiCGSTAB with a "
                       "multigrid preconditioner to 1e-5, FP64 refinement to 1e-10, FP64 dots "
-                      "and drag")}
+                      "and drag; mixed_default_inner_tol: the same with inner solves to the "
+                      "mode's default 1e-2; every multigrid solve starts from the "
+                      "extrapolation of the last two")}
     for name, r in runs.items():
         out[name] = {"value": round(n_cells * n / r["wall"] / 1e6, 4), "unit": "Mcell-steps/s",
                      "ms_per_step": round(r["wall"] / n * 1e3, 3),
                      "mean_iters": sum(r["its"]) / n,
                      "steps_above_tol": sum(1 for e in r["errs"][1:] if not e < wl["tol"]),
                      "cd_last": float(r["cd"][-1]), "cl_last": float(r["cl"][-1])}
-    m = runs["mixed"]
-    for ref in [k for k in runs if k != "mixed"]:
-        r = runs[ref]
-        unorm = float(np.linalg.norm(r["u"][0]))
-        out["mixed"][f"errors_vs_{ref}"] = {
-            "u": rel(m["u"][0], r["u"][0]),
-            "v": float(np.linalg.norm(m["u"][1] - r["u"][1])) / unorm,
-            "w": float(np.linalg.norm(m["u"][2] - r["u"][2])) / unorm,
-            "p": rel(m["p"], r["p"]),
-            "cd_series_max_rel": float(np.max(np.abs(m["cd"] - r["cd"]) / np.abs(r["cd"]))),
-            "cl_series_max_abs_over_max_cl": float(np.max(np.abs(m["cl"] - r["cl"]))
-                                                   / np.max(np.abs(r["cl"])))}
-    out["mixed"]["speedup_vs_fp64_multigrid"] = round(
-        runs["fp64_multigrid"]["wall"] / m["wall"], 3)
-    if "fp64" in runs:
-        out["mixed"]["speedup_vs_fp64"] = round(runs["fp64"]["wall"] / m["wall"], 3)
+    for mk in ("mixed", "mixed_default_inner_tol"):
+        m = runs[mk]
+        for ref in ("fp64_multigrid", "fp64"):
+            if ref not in runs:
+                continue
+            r = runs[ref]
+            unorm = float(np.linalg.norm(r["u"][0]))
+            out[mk][f"errors_vs_{ref}"] = {
+                "u": rel(m["u"][0], r["u"][0]),
+                "v": float(np.linalg.norm(m["u"][1] - r["u"][1])) / unorm,
+                "w": float(np.linalg.norm(m["u"][2] - r["u"][2])) / unorm,
+                "p": rel(m["p"], r["p"]),
+                "cd_series_max_rel": float(np.max(np.abs(m["cd"] - r["cd"])
+                                                  / np.abs(r["cd"]))),
+                "cl_series_max_abs_over_max_cl": float(np.max(np.abs(m["cl"] - r["cl"]))
+                                                       / np.max(np.abs(r["cl"])))}
+        out[mk]["speedup_vs_fp64_multigrid"] = round(
+            runs["fp64_multigrid"]["wall"] / m["wall"], 3)
+        if "fp64" in runs:
+            out[mk]["speedup_vs_fp64"] = round(runs["fp64"]["wall"] / m["wall"], 3)
     out["errors_note"] = ("relative L2 of the final fields (v and w relative to |u|, p with "
                           "its mean removed) and the largest Cd / Cl difference over all "
                           "steps; fp64 stops at its 5000-iteration cap short of tol on most "
@@ -967,7 +1004,7 @@ def main():
                              "at its own default inner tolerance"),
             "error_gate": {"bound": 1e-6, "against": "fp64_multigrid (converged to tol)",
                            "passed": None},
-            "inner_tol": MIXED_INNER_TOL or {"mixed": 3e-3, "fp64_multigrid": 1e-5},
+            "inner_tol": MIXED_INNER_TOL or {"mixed": 1e-2, "fp64_multigrid": 1e-5},
             "tol": wl["tol"],
             "iters_per_step": [i["iters"] for i in minfos],
             "residual_per_step": [i["err"] for i in minfos],
@@ -980,6 +1017,7 @@ def main():
                             "cap short of tol); fp64_multigrid: the mixed mode's algorithm in "
                             "FP64, converged to tol"),
             "kernels": kernel_table(mprof, n_cells, BYTES_PER_CELL_FP32),
+            "sustained": sustained_run(wl, args),
             "fp64_multigrid": {
                 "value": round(gvalue, 4), "unit": "Mcell-steps/s",
                 "ms_per_step": round(gwall / args.steps * 1e3, 3),

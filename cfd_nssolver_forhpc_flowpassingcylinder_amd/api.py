@@ -65,6 +65,7 @@ class Config(C.Structure):
         ("sor_omega", C.c_double), ("skip_first_pressure", C.c_int), ("precision", C.c_int),
         ("inner_tol", C.c_double), ("xcd_remap", C.c_int), ("mg_graphs", C.c_int),
         ("smagorinsky_cs", C.c_double), ("steady_check", C.c_int),
+        ("mg_extrapolate", C.c_int),
     ]
 
 

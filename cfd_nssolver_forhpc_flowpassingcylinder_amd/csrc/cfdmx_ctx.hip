@@ -794,6 +794,8 @@ static int validate(cfdmx_ctx *ctx)
         FAIL(CFDMX_ERR_ARG, "smagorinsky_cs must be in [0, 1) (got %g)", c.smagorinsky_cs);
     if (c.steady_check != 0 && c.steady_check != 1)
         FAIL(CFDMX_ERR_ARG, "steady_check must be 0 or 1 (got %d)", c.steady_check);
+    if (c.mg_extrapolate != 0 && c.mg_extrapolate != 1)
+        FAIL(CFDMX_ERR_ARG, "mg_extrapolate must be 0 or 1 (got %d)", c.mg_extrapolate);
     if (c.precision != CFDMX_PRECISION_FP64 && c.precision != CFDMX_PRECISION_MIXED &&
         c.precision != CFDMX_PRECISION_FP64_MG)
         FAIL(CFDMX_ERR_ARG, "unknown precision %d", c.precision);
@@ -858,6 +860,7 @@ int cfdmx_config_default(cfdmx_config *cfg)
     cfg->mg_graphs = 1;
     cfg->smagorinsky_cs = 0.0;
     cfg->steady_check = 0;
+    cfg->mg_extrapolate = 1;
     return CFDMX_OK;
 }
 
@@ -1297,11 +1300,12 @@ static int solve_bicgstab_impl(cfdmx_ctx *ctx, int *iters, double *res)
 // the FP64 residual (a stalled inner Krylov process) is taken back and ends the solve
 // with the previous x.
 static constexpr int kMaxOuter = 40;
-// inner_tol = 0 takes the mode's default, the fastest measured at 512x256x64 and
-// 1024x512x128 (tol 1e-10): in float, loose inner solves and more FP64 passes need the
-// fewest inner iterations in all (5.6 ms per step against 6.2 ms at 1e-5); in double,
-// 1e-5 is fastest (8.4 ms against 9.6 ms at 3e-3)
-static constexpr double kInnerTolMixed = 3e-3, kInnerTolFp64 = 1e-5;
+// inner_tol = 0 takes the mode's default, the fastest measured at 512x256x64 (tol 1e-10,
+// extrapolated starts).  In float one inner solve cannot take the FP64 residual far below
+// ~1e-2 of where it started, so a loose inner solve and another FP64 pass cost the least:
+// 2.80 ms per step of the developed flow at 1e-2, 3.39 ms at 3e-3, 3.75 ms at 1e-5.  In
+// double one pass to 1e-5 reaches tol (3.73 ms, 2.1 inner iterations per step).
+static constexpr double kInnerTolMixed = 1e-2, kInnerTolFp64 = 1e-5;
 
 } // extern "C" (the multigrid solve below is templated on its arithmetic type)
 
@@ -1415,8 +1419,19 @@ static int solve_mg_target(cfdmx_ctx *ctx, const double *b, double *x, int *iter
                : solve_refine<double>(ctx, b, x, iters, res);
 }
 
+// The time step's pressure solve in a multigrid mode.  With mg_extrapolate the start is
+// 2 x_n - x_(n-1) from the two solutions before this one; px1 (in these modes only this
+// history, and checkpointed) holds x_(n-1).  The solves before this step's follow from
+// the step counter (the first step has none with skip_first_pressure), so a run resumed
+// from a checkpoint or recomputed from an adjoint checkpoint starts the same way.
 static int solve_mixed_impl(cfdmx_ctx *ctx, int *iters, double *res)
 {
+    if (ctx->cfg.mg_extrapolate) {
+        const int solved_before = (ctx->loop - 1) - (ctx->cfg.skip_first_pressure ? 1 : 0);
+        if (solved_before >= 2) launch_extrapolate(ctx->g, ctx->stream, ctx->x, ctx->px1);
+        else launch_copy(ctx->stream, ctx->g.N, ctx->x, ctx->px1);
+        LAUNCHCHK();
+    }
     return solve_mg_target(ctx, ctx->b, ctx->x, iters, res);
 }
 
@@ -2199,6 +2214,7 @@ int cfdmx_adjoint_drag(cfdmx_ctx *ctx, int nsteps, int checkpoint_every, double 
     const double wf = (-2.0 / c.lz) / nsteps;
     const double nu = 1.0 / c.re;
     int adj_iters = 0;
+    int mg_solves = 0; // multigrid pressure solves of the reverse pass so far
     for (int seg = ncp - 1; seg >= 0; --seg) {
         const int n0 = seg * K, n1 = std::min(n0 + K, nsteps);
         if (seg != ncp - 1) { // the last segment is on the tape from the forward pass
@@ -2237,8 +2253,16 @@ int cfdmx_adjoint_drag(cfdmx_ctx *ctx, int nsteps, int checkpoint_every, double 
                     // its adjoint has zero mean up to rounding), so that the singular
                     // system is consistent and the FP64 refinement can reach tol
                     launch_adj_remove_mean(g, s, A.lx, A.apart);
+                    // with mg_extrapolate, as in the forward solves: the start is
+                    // 2 mu_k - mu_(k-1) from the two solutions before (mpx1 holds the
+                    // older; the FP64 branch alone uses it otherwise)
+                    if (c.mg_extrapolate) {
+                        if (mg_solves >= 2) launch_extrapolate(g, s, A.mu, A.mpx1);
+                        else launch_copy(s, (int)N, A.mu, A.mpx1);
+                    }
                     LAUNCHCHK();
                     if ((rc = solve_mg_target(ctx, A.lx, A.mu, &it, &res))) return rc;
+                    ++mg_solves;
                 }
                 adj_iters += it;
                 launch_adj_remove_mean(g, s, A.mu, A.apart);

@@ -150,6 +150,7 @@ void launch_mixed_fold(const Geo &g, hipStream_t s, const double *part, double *
 void launch_bicg_fin0(const Geo &g, hipStream_t s, double *part, KState *st);
 void launch_nanfix(hipStream_t s, int n, double *x, const double *px1, int *count);
 void launch_copy(hipStream_t s, int n, const double *a, double *b);
+void launch_extrapolate(const Geo &g, hipStream_t s, double *x, double *xp);
 void launch_fill(hipStream_t s, int n, double val, double *a);
 void launch_sor_rhs(const Geo &g, hipStream_t s, const double *us, const double *vs,
                     const double *ws, double dt, double *cf6);

@@ -1602,6 +1602,17 @@ __global__ void k_mixed_update(int n, double sign, double *__restrict__ x,
         x[q] += sign * (double)e[q];
 }
 
+// the start of a multigrid pressure solve from the last two solutions: x <- 2 x - xp,
+// xp <- x (the previous solution, kept for the next step)
+__global__ void k_extrapolate(int n, double *__restrict__ x, double *__restrict__ xp)
+{
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        const double xo = x[q];
+        x[q] = 2.0 * xo - xp[q];
+        xp[q] = xo;
+    }
+}
+
 __global__ void __launch_bounds__(kThreads)
 k_mixed_fold(const double *part, int nb, double *out)
 {
@@ -2690,6 +2701,11 @@ template void launch_mixed_update<float>(const Geo &, hipStream_t, double, doubl
                                          const float *);
 template void launch_mixed_update<double>(const Geo &, hipStream_t, double, double *,
                                           const double *);
+void launch_extrapolate(const Geo &g, hipStream_t s, double *x, double *xp)
+{
+    hipLaunchKernelGGL(k_extrapolate, dim3(2048), dim3(kThreads), 0, s, g.N, x, xp);
+}
+
 void launch_mixed_fold(const Geo &g, hipStream_t s, const double *part, double *out)
 {
     hipLaunchKernelGGL(k_mixed_fold, dim3(1), dim3(kThreads), 0, s, part, g.nblocks, out);

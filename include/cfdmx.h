@@ -75,7 +75,7 @@ typedef struct cfdmx_config {
     int precision;
     double inner_tol; /* multigrid modes: relative tolerance of each inner solve;
                          0 (the default) takes the mode's fastest measured value,
-                         3e-3 for MIXED and 1e-5 for FP64_MG.  The FP64 refinement stops
+                         1e-2 for MIXED and 1e-5 for FP64_MG.  The FP64 refinement stops
                          at tol, or unconverged (cfdmx_step_info.unconverged) after 40
                          passes, at maxit inner iterations in all, on a pass that raises
                          the residual (taken back) or on an inner divergence */
@@ -92,6 +92,13 @@ typedef struct cfdmx_config {
      * cfdmx_step_info.maxdiff, folded into the projection (it then reads T0 as well:
      * 24 more bytes per cell); 0 (the default) = off */
     int steady_check;
+    /* multigrid modes (no reference counterpart): 1 (the default) starts each pressure
+     * solve (and each pressure solve of the adjoint's reverse pass, from its last two) from
+     * 2 x_n - x_(n-1), the linear extrapolation of the last two solutions
+     * (x_(n-1) kept in the solver's history vector, which checkpoints store), instead of x_n; the
+     * solve still ends at tol.  0 = start from x_n.  The FP64 path always starts from x_n,
+     * as bicgstabRe2 does */
+    int mg_extrapolate;
 } cfdmx_config;
 
 typedef struct cfdmx_ctx cfdmx_ctx;
@@ -178,7 +185,7 @@ CFDMX_API int cfdmx_plot3d_x(cfdmx_ctx *ctx, const char *path);
  * solver, operator, boundary-condition table, outflow and flux-correction settings and
  * Smagorinsky constant: CFDMX_ERR_ARG otherwise), after which steps continue bit for
  * bit as in the saved run as long as the tolerances (tol, maxit, sor_omega, inner_tol)
- * are also the saved run's, which the load does not check.  Both synchronise the
+ * and mg_extrapolate are also the saved run's, which the load does not check.  Both synchronise the
  * context's stream. */
 CFDMX_API int cfdmx_checkpoint_save(cfdmx_ctx *ctx, const char *path);
 CFDMX_API int cfdmx_checkpoint_load(cfdmx_ctx *ctx, const char *path);

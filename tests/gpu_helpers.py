@@ -20,7 +20,7 @@ def make_gpu(nxc, nyc, nzc, *, lengths=(20.0, 10.0, 0.5), kind=0, re=100.0, dt=0
              solver=0, precond=0, tol=1e-6, maxit=5000, omega=1.8, bc="flowpassing",
              seed=20261202, amp=1e-3, cyl=(5.0, 5.0, 0.5), xcd_remap=1, smoothed_eps=None,
              skip_first=1, inflow=None, precision=0, inner_tol=1e-5, mg_graphs=1,
-             smag_cs=0.0, steady_check=0):
+             smag_cs=0.0, steady_check=0, mg_extrapolate=1):
     """The device context of a case, with its seeded initial field and mask."""
     num, dirr, zper, conv, flux = BC_PRESETS[bc]
     cfg = pkg.default_config(nx=nxc, ny=nyc, nz=nzc, lx=lengths[0], ly=lengths[1],
@@ -30,7 +30,8 @@ def make_gpu(nxc, nyc, nzc, *, lengths=(20.0, 10.0, 0.5), kind=0, re=100.0, dt=0
                              convective_speed=1.0, flux_correction=flux,
                              skip_first_pressure=skip_first, xcd_remap=xcd_remap,
                              precision=precision, inner_tol=inner_tol, mg_graphs=mg_graphs,
-                             smagorinsky_cs=smag_cs, steady_check=steady_check)
+                             smagorinsky_cs=smag_cs, steady_check=steady_check,
+                             mg_extrapolate=mg_extrapolate)
     gpu = pkg.FlowSolver(cfg)
     if inflow is not None:
         gpu.set_inflow(inflow)
@@ -47,7 +48,7 @@ def make_pair(nxc, nyc, nzc, *, lengths=(20.0, 10.0, 0.5), kind=0, re=100.0, dt=
               solver=0, precond=0, tol=1e-6, maxit=5000, omega=1.8, bc="flowpassing",
               seed=20261202, amp=1e-3, cyl=(5.0, 5.0, 0.5), xcd_remap=1, smoothed_eps=None,
               skip_first=1, inflow=None, precision=0, inner_tol=1e-5, mg_graphs=1,
-              smag_cs=0.0, steady_check=0):
+              smag_cs=0.0, steady_check=0, mg_extrapolate=1):
     """The device context of a case and the oracle's run of the same case."""
     num, dirr, zper, conv, flux = BC_PRESETS[bc]
     gpu = make_gpu(nxc, nyc, nzc, lengths=lengths, kind=kind, re=re, dt=dt, solver=solver,
@@ -55,7 +56,7 @@ def make_pair(nxc, nyc, nzc, *, lengths=(20.0, 10.0, 0.5), kind=0, re=100.0, dt=
                    amp=amp, cyl=cyl, xcd_remap=xcd_remap, smoothed_eps=smoothed_eps,
                    skip_first=skip_first, inflow=inflow, precision=precision,
                    inner_tol=inner_tol, mg_graphs=mg_graphs, smag_cs=smag_cs,
-                   steady_check=steady_check)
+                   steady_check=steady_check, mg_extrapolate=mg_extrapolate)
     o = op.Oracle()
     g = o.grid(nxc, nyc, nzc, *lengths, kind=kind)
     obc = op.make_bc(num, dirr, inflow=inflow, z_periodic=zper, outflow_convective=conv,
