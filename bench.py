@@ -18,7 +18,11 @@ The JSON line also carries:
                    headers) timed on this host on a bounded sample of the same grid;
   stream_ceiling -- what plain streaming kernels (copy, and a two-read-one-write triad
                    of phase B's shape) reach at the same array size with the caches
-                   flushed: the attainable rate the phases are read against.
+                   flushed: the attainable rate the phases are read against;
+  mixed         -- the mixed-precision mode and the same algorithm in FP64 (FP64
+                   multigrid) over the FP64 run's steps, with field errors against both;
+                   mixed.sustained: both multigrid modes timed over 20 steps after 100;
+  rbsor, adjoint -- the red-black SOR leg and the drag adjoint (their own workloads).
 """
 import argparse
 import json
