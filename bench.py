@@ -52,7 +52,7 @@ BYTES_PER_CELL = {
 # per iteration (32 + 24 + 72)
 ITERATION_PHASES = ("bicg_A", "bicg_B", "bicg_C")
 
-MIXED_INNER_TOL = 0.0  # the library's per-mode default: 1e-2 mixed, 1e-5 FP64 multigrid
+MIXED_INNER_TOL = 0.0  # the library's per-mode default: 1e-5 in both multigrid modes
 
 # multigrid V-cycles from captured HIP graphs (cfdmx_config.mg_graphs); --no-mg-graphs
 # launches them kernel by kernel (the same bits and, at these sizes, the same time)
@@ -687,9 +687,8 @@ def c3_run(wl, args):
     import numpy as np
     n = args.c3_steps
     shp = (wl["nx"] + 4, wl["ny"] + 4, wl["nz"] + 4)
-    # the configuration's inner tolerance 1e-5 in both multigrid modes, and the mixed mode
-    # at its own default (1e-2) beside it
-    modes = [("mixed", 1, 1e-5), ("fp64_multigrid", 2, 1e-5), ("mixed_default_inner_tol", 1, 0.0)]
+    # the configuration's inner tolerance 1e-5 (also both modes' default)
+    modes = [("mixed", 1, 1e-5), ("fp64_multigrid", 2, 1e-5)]
     if not args.c3_no_fp64:
         modes.append(("fp64", 0, None))
     runs = {}
@@ -722,16 +721,15 @@ def c3_run(wl, args):
     out = {"config": (f"{wl['nx']}x{wl['ny']}x{wl['nz']}, Re={wl['re']:g}, dt={wl['dt']:g}, "
                       f"{n} steps from the seeded field; mixed: FP32 inner BiCGSTAB with a "
                       "multigrid preconditioner to 1e-5, FP64 refinement to 1e-10, FP64 dots "
-                      "and drag; mixed_default_inner_tol: the same with inner solves to the "
-                      "mode's default 1e-2; every multigrid solve starts from the "
-                      "extrapolation of the last two")}
+                      "and drag, the correction accumulated in FP64; every multigrid solve "
+                      "starts from the extrapolation of the last two")}
     for name, r in runs.items():
         out[name] = {"value": round(n_cells * n / r["wall"] / 1e6, 4), "unit": "Mcell-steps/s",
                      "ms_per_step": round(r["wall"] / n * 1e3, 3),
                      "mean_iters": sum(r["its"]) / n,
                      "steps_above_tol": sum(1 for e in r["errs"][1:] if not e < wl["tol"]),
                      "cd_last": float(r["cd"][-1]), "cl_last": float(r["cl"][-1])}
-    for mk in ("mixed", "mixed_default_inner_tol"):
+    for mk in ("mixed",):
         m = runs[mk]
         for ref in ("fp64_multigrid", "fp64"):
             if ref not in runs:
@@ -1008,7 +1006,7 @@ def main():
                              "at its own default inner tolerance"),
             "error_gate": {"bound": 1e-6, "against": "fp64_multigrid (converged to tol)",
                            "passed": None},
-            "inner_tol": MIXED_INNER_TOL or {"mixed": 1e-2, "fp64_multigrid": 1e-5},
+            "inner_tol": MIXED_INNER_TOL or {"mixed": 1e-5, "fp64_multigrid": 1e-5},
             "tol": wl["tol"],
             "iters_per_step": [i["iters"] for i in minfos],
             "residual_per_step": [i["err"] for i in minfos],

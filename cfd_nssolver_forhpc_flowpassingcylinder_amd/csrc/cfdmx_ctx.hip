@@ -412,7 +412,7 @@ template <typename T> static int setup_multigrid(cfdmx_ctx *ctx)
         ctx->pb[q] = z;
     }
     {
-        T *z = nullptr;
+        double *z = nullptr; // the correction is FP64 in both modes (k_pb_update_xr)
         ALLOC(z, N);
         ctx->ecorr = z;
     }
@@ -1301,11 +1301,11 @@ static int solve_bicgstab_impl(cfdmx_ctx *ctx, int *iters, double *res)
 // with the previous x.
 static constexpr int kMaxOuter = 40;
 // inner_tol = 0 takes the mode's default, the fastest measured at 512x256x64 (tol 1e-10,
-// extrapolated starts).  In float one inner solve cannot take the FP64 residual far below
-// ~1e-2 of where it started, so a loose inner solve and another FP64 pass cost the least:
-// 2.80 ms per step of the developed flow at 1e-2, 3.39 ms at 3e-3, 3.75 ms at 1e-5.  In
-// double one pass to 1e-5 reaches tol (3.73 ms, 2.1 inner iterations per step).
-static constexpr double kInnerTolMixed = 1e-2, kInnerTolFp64 = 1e-5;
+// extrapolated starts): 1e-5 in both, where one pass usually reaches tol.  With the
+// correction accumulated in FP64 the FP32 inner solve takes the pass as far as the FP64
+// one does, in the same iterations (2.14 per step of the developed flow; 2.78 ms at 1e-5
+// against 2.89 ms at 1e-2 in the mixed mode, 3.78 ms in the FP64 multigrid mode).
+static constexpr double kInnerTolMixed = 1e-5, kInnerTolFp64 = 1e-5;
 
 } // extern "C" (the multigrid solve below is templated on its arithmetic type)
 
@@ -1320,7 +1320,7 @@ static int solve_pb_inner(cfdmx_ctx *ctx, double rr, double tol, int maxit, int 
     T *const *pv = reinterpret_cast<T *const *>(ctx->pb);
     T *r = pv[0], *r0 = pv[1], *p = pv[2], *v = pv[3], *sv = pv[4], *t = pv[5], *ph = pv[6],
       *sh = pv[7];
-    T *e = static_cast<T *>(ctx->ecorr);
+    double *e = static_cast<double *>(ctx->ecorr);
     int rc = CFDMX_OK;
     PBState ps = {};
     ps.rho = rr; // (r, r0) with r0 = r
@@ -1360,11 +1360,11 @@ static int solve_refine(cfdmx_ctx *ctx, const double *b, double *x, int *iters, 
     const Geo &g = ctx->g;
     const cfdmx_config &c = ctx->cfg;
     T *const *pv = reinterpret_cast<T *const *>(ctx->pb);
-    T *e = static_cast<T *>(ctx->ecorr);
+    double *e = static_cast<double *>(ctx->ecorr);
     int total = 0, outer = 0;
     double res64 = 0.0, res_prev = 0.0;
     for (;;) {
-        if (outer > 0) PROF(CFDMX_K_BICG_INIT, launch_mixed_update<T>(g, s, 1.0, x, e));
+        if (outer > 0) PROF(CFDMX_K_BICG_INIT, launch_mixed_update(g, s, 1.0, x, e));
         PROF(CFDMX_K_BICG_INIT, launch_mixed_resid<T>(g, s, 0, b, x, pv[0], pv[1], e, ctx->part));
         launch_mixed_fold(g, s, ctx->part, ctx->mix);
         LAUNCHCHK();
@@ -1377,7 +1377,7 @@ static int solve_refine(cfdmx_ctx *ctx, const double *b, double *x, int *iters, 
         if (outer > 0 && !(res64 <= res_prev)) {
             // take the pass back: x - e is within a rounding of the previous x, so its
             // residual is formed again and reported, not the previous pass's
-            PROF(CFDMX_K_BICG_INIT, launch_mixed_update<T>(g, s, -1.0, x, e));
+            PROF(CFDMX_K_BICG_INIT, launch_mixed_update(g, s, -1.0, x, e));
             PROF(CFDMX_K_BICG_INIT,
                  launch_mixed_resid<T>(g, s, 0, b, x, pv[0], pv[1], e, ctx->part));
             launch_mixed_fold(g, s, ctx->part, ctx->mix);

@@ -1488,7 +1488,7 @@ k_march_C(Geo g, T *__restrict__ r, const T *__restrict__ ap, T *__restrict__ q,
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
 k_mixed_resid(Geo g, int pre, const double *__restrict__ b, const double *__restrict__ x,
-              T *__restrict__ r32, T *__restrict__ r0_32, T *__restrict__ e32,
+              T *__restrict__ r32, T *__restrict__ r0_32, double *__restrict__ e,
               double *__restrict__ part)
 {
     const int L = dealt_block();
@@ -1506,7 +1506,7 @@ k_mixed_resid(Geo g, int pre, const double *__restrict__ b, const double *__rest
         const T rf = (T)rr;
         r32[n] = rf;
         r0_32[n] = rf;
-        e32[n] = (T)0;
+        e[n] = 0.0;
         acc[0] += bn * bn;
         acc[1] += rr * rr;
         acc[2] += (double)rf * (double)rf;
@@ -1548,7 +1548,7 @@ template <> struct Vec4T<double> {
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
 k_mixed_resid4(Geo g, int pre, const double *__restrict__ b, const double *__restrict__ x,
-               T *__restrict__ r32, T *__restrict__ r0_32, T *__restrict__ e32,
+               T *__restrict__ r32, T *__restrict__ r0_32, double *__restrict__ e,
                double *__restrict__ part)
 {
     const int L = dealt_block();
@@ -1568,7 +1568,8 @@ k_mixed_resid4(Geo g, int pre, const double *__restrict__ b, const double *__res
         load4(b + n0, bv);
         const double zlo = x[k0 > 0 ? n0 - 1 : (g.zper ? n0 + g.nzc - 1 : n0)];
         const double zhi = x[k0 + 3 < g.nzc - 1 ? n0 + 4 : (g.zper ? n0 + 4 - g.nzc : n0 + 3)];
-        T rv[4], zero[4];
+        T rv[4];
+        double zero[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             RowCoef rc;
@@ -1581,25 +1582,25 @@ k_mixed_resid4(Geo g, int pre, const double *__restrict__ b, const double *__res
             for (int t = 0; t < 7; ++t) ax += rc.has[t] ? rc.v[t] * xv[t] : 0.0;
             const double rr = bv[q] - ax;
             rv[q] = (T)rr;
-            zero[q] = (T)0;
+            zero[q] = 0.0;
             acc[0] += bv[q] * bv[q];
             acc[1] += rr * rr;
             acc[2] += (double)rv[q] * (double)rv[q];
         }
         Vec4T<T>::store(r32 + n0, rv);
         Vec4T<T>::store(r0_32 + n0, rv);
-        Vec4T<T>::store(e32 + n0, zero);
+        Vec4T<double>::store(e + n0, zero);
     }
     block_sums<3>(acc, part, g.nblocks, L);
 }
 
-// x += sign * e (FP64 accumulation of the FP32 correction; sign -1 takes it back)
-template <typename T>
+// x += sign * e (the correction, accumulated in FP64 by the inner solve; sign -1 takes
+// it back)
 __global__ void k_mixed_update(int n, double sign, double *__restrict__ x,
-                               const T *__restrict__ e)
+                               const double *__restrict__ e)
 {
     for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x)
-        x[q] += sign * (double)e[q];
+        x[q] += sign * e[q];
 }
 
 // the start of a multigrid pressure solve from the last two solutions: x <- 2 x - xp,
@@ -2678,7 +2679,7 @@ void launch_bicg_check(const Geo &g, hipStream_t s, double *const part[3], KStat
 
 template <typename T>
 void launch_mixed_resid(const Geo &g, hipStream_t s, int pre, const double *b, const double *x,
-                        T *r, T *r0, T *e, double *part)
+                        T *r, T *r0, double *e, double *part)
 {
     if (g.nzc % 4 == 0 && kPerBlock == 4 * kThreads)
         hipLaunchKernelGGL(k_mixed_resid4<T>, dim3(grid_blocks(g)), dim3(kThreads), 0, s, g, pre,
@@ -2687,20 +2688,15 @@ void launch_mixed_resid(const Geo &g, hipStream_t s, int pre, const double *b, c
         hipLaunchKernelGGL(k_mixed_resid<T>, dim3(grid_blocks(g)), dim3(kThreads), 0, s, g, pre,
                            b, x, r, r0, e, part);
 }
-template <typename T>
-void launch_mixed_update(const Geo &g, hipStream_t s, double sign, double *x, const T *e)
+void launch_mixed_update(const Geo &g, hipStream_t s, double sign, double *x, const double *e)
 {
-    hipLaunchKernelGGL(k_mixed_update<T>, dim3(2048), dim3(kThreads), 0, s, g.N, sign, x, e);
+    hipLaunchKernelGGL(k_mixed_update, dim3(2048), dim3(kThreads), 0, s, g.N, sign, x, e);
 }
 template void launch_mixed_resid<float>(const Geo &, hipStream_t, int, const double *,
-                                        const double *, float *, float *, float *, double *);
+                                        const double *, float *, float *, double *, double *);
 template void launch_mixed_resid<double>(const Geo &, hipStream_t, int, const double *,
                                          const double *, double *, double *, double *,
                                          double *);
-template void launch_mixed_update<float>(const Geo &, hipStream_t, double, double *,
-                                         const float *);
-template void launch_mixed_update<double>(const Geo &, hipStream_t, double, double *,
-                                          const double *);
 void launch_extrapolate(const Geo &g, hipStream_t s, double *x, double *xp)
 {
     hipLaunchKernelGGL(k_extrapolate, dim3(2048), dim3(kThreads), 0, s, g.N, x, xp);

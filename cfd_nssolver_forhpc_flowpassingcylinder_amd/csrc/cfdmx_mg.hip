@@ -773,21 +773,26 @@ __global__ void k_pb_update_s(Geo g, const T *r, const T *v, T *s, const PBState
         s[n] = r[n] - a * v[n];
 }
 
-// e += alpha p^ + omega s^; r = s - omega t; partials (r, r), (r, r0)
+// e += alpha p^ + omega s^; r = s - omega t; partials (r, r), (r, r0).  The correction e
+// and its increments are FP64 in both modes: in FP32 its own rounding (6e-8 of e),
+// multiplied by the operator's condition number (~1e5 here), would leave a true residual
+// near 1e-2 of the pass's start, whatever the inner tolerance; in FP64 the pass's floor is
+// the FP32 recursion's drift from the true residual, ~1e-6
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
-k_pb_update_xr(Geo g, const T *ph, const T *sh, const T *s, const T *t, const T *r0, T *e, T *r,
-               double *part_out, const PBState *st)
+k_pb_update_xr(Geo g, const T *ph, const T *sh, const T *s, const T *t, const T *r0, double *e,
+               T *r, double *part_out, const PBState *st)
 {
     if (st->done) return;
     const int B = logical_block(g);
     if (B >= g.nblocks) return;
-    const T a = (T)st->alpha, om = (T)st->omega;
+    const T om = (T)st->omega;
+    const double ad = st->alpha, omd = st->omega;
     double acc[2] = {0.0, 0.0};
     for (int c = 0; c < kCells; ++c) {
         const int n = B * kPerBlock + c * kThreads + threadIdx.x;
         if (n < g.N) {
-            e[n] += a * ph[n] + om * sh[n];
+            e[n] += ad * (double)ph[n] + omd * (double)sh[n];
             const T rn = s[n] - om * t[n];
             r[n] = rn;
             acc[0] += (double)rn * (double)rn;
@@ -946,7 +951,7 @@ void launch_pb_update_s(const Geo &g, hipStream_t s, const double *part, const T
 
 template <typename T>
 void launch_pb_update_xr(const Geo &g, hipStream_t s, const double *part_in, const T *ph,
-                         const T *sh, const T *sv, const T *t, const T *r0, T *e, T *r,
+                         const T *sh, const T *sv, const T *t, const T *r0, double *e, T *r,
                          double *part_out, PBState *st)
 {
     hipLaunchKernelGGL(k_pb_omega, dim3(1), dim3(kThreads), 0, s, g, part_in, st);
@@ -968,8 +973,8 @@ void launch_pb_end(const Geo &g, hipStream_t s, const double *part, PBState *st)
     template void launch_pb_update_s<T>(const Geo &, hipStream_t, const double *, const T *,  \
                                         const T *, T *, PBState *);                           \
     template void launch_pb_update_xr<T>(const Geo &, hipStream_t, const double *, const T *, \
-                                         const T *, const T *, const T *, const T *, T *,     \
-                                         T *, double *, PBState *);
+                                         const T *, const T *, const T *, const T *,          \
+                                         double *, T *, double *, PBState *);
 CFDMX_MG_INSTANCES(float)
 CFDMX_MG_INSTANCES(double)
 #undef CFDMX_MG_INSTANCES

@@ -75,7 +75,7 @@ typedef struct cfdmx_config {
     int precision;
     double inner_tol; /* multigrid modes: relative tolerance of each inner solve;
                          0 (the default) takes the mode's fastest measured value,
-                         1e-2 for MIXED and 1e-5 for FP64_MG.  The FP64 refinement stops
+                         1e-5 in both modes.  The FP64 refinement stops
                          at tol, or unconverged (cfdmx_step_info.unconverged) after 40
                          passes, at maxit inner iterations in all, on a pass that raises
                          the residual (taken back) or on an inner divergence */

@@ -8,7 +8,9 @@ digests pin them against later changes of their kernels: each kernel variant kee
 cell update's terms and order, and a run must give the same bits.  The digests were made
 by this library on an MI355X (ROCm 7.2) before the two-cells-per-thread FP64 half sweep
 was added, and the run with it gave the same digests, as did the run with the two
-coarsest levels in one workgroup.  The inner tolerances are the ones they were made with
+coarsest levels in one workgroup.  The mixed-mode digests were made again when the
+inner solve's correction became FP64 (its arithmetic changed); the FP64 multigrid ones
+stayed the same.  The inner tolerances are the ones they were made with
 (INNER_TOL).  Three steps end before the extrapolated start takes effect (it needs two
 earlier solves).
 

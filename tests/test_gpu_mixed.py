@@ -59,7 +59,7 @@ def test_mixed_solve_reaches_fp64_residual(n, precision):
 
 # consistent pressure systems only (flux-corrected outlet, periodic z): at tol 1e-10 an
 # inconsistent system ends at the iteration limit in either precision
-# inner_tol 0: the mode's default (1e-2 mixed, 1e-5 FP64 multigrid)
+# inner_tol 0: the mode's default (1e-5 in both modes)
 @pytest.mark.parametrize("inner_tol", [1e-5, 3e-3, 0.0])
 @pytest.mark.parametrize("precision", PRECISIONS)
 @pytest.mark.parametrize("kind,n,precond,steps", [(0, (48, 24, 8), 0, 6),
