@@ -81,6 +81,11 @@ struct cfdmx_ctx {
         std::vector<double> cp_time;
         double *lu[3] = {}, *ls[3] = {}, *le[3] = {}, *us[3] = {};
         double *lx = nullptr, *mu = nullptr, *mpx1 = nullptr, *mpx2 = nullptr;
+        // the pressure solution (X_result) of every forward step before the taped
+        // segment, so that the reverse pass recomputes those steps without their solves;
+        // null where it did not fit
+        double *allX = nullptr;
+        size_t allX_n = 0;
         KState *kadj = nullptr, *kadj_h = nullptr;
         double *leta = nullptr, *Jr = nullptr, *lin = nullptr, *tin = nullptr;
         double *apart = nullptr, *out = nullptr, *out_h = nullptr;
@@ -886,6 +891,7 @@ int cfdmx_destroy(cfdmx_ctx *ctx)
     if (ctx->kst_h) (void)hipHostFree(ctx->kst_h);
     if (ctx->pbs_h) (void)hipHostFree(ctx->pbs_h);
     for (void *q : ctx->adj.mem) (void)hipFree(q);
+    if (ctx->adj.allX) (void)hipFree(ctx->adj.allX);
     if (ctx->adj.kadj_h) (void)hipHostFree(ctx->adj.kadj_h);
     if (ctx->adj.out_h) (void)hipHostFree(ctx->adj.out_h);
     if (ctx->mix_h) (void)hipHostFree(ctx->mix_h);
@@ -1588,8 +1594,12 @@ int cfdmx_apply_operator(cfdmx_ctx *ctx, const double *x, double *r)
     return copy_d2h(ctx, r, ctx->B.r, ctx->g.N);
 }
 
-// one iteration of the runSeqOmp loop body (RUN_CPU.cpp:257-436)
-static int step_once(cfdmx_ctx *ctx, int *iters, double *err, int *solved)
+// one iteration of the runSeqOmp loop body (RUN_CPU.cpp:257-436).  x_given (the
+// adjoint's recompute): the pressure solution this step is known to reach, copied into
+// X_result in place of the right-hand side and the solve; every other stage is the same,
+// so the step gives the same bits as the one that solved.
+static int step_once(cfdmx_ctx *ctx, int *iters, double *err, int *solved,
+                     const double *x_given = nullptr)
 {
     hipStream_t s = ctx->stream;
     const Geo &g = ctx->g;
@@ -1604,15 +1614,21 @@ static int step_once(cfdmx_ctx *ctx, int *iters, double *err, int *solved)
     *solved = 0;
     if (ctx->loop > 1 || !c.skip_first_pressure) {
         *solved = 1;
-        if (c.solver == CFDMX_SOLVER_BICGSTAB) {
-            rc = cfdmx_pressure_rhs(ctx);
-            if (!rc) rc = c.precision != CFDMX_PRECISION_FP64 ? solve_mixed_impl(ctx, iters, err)
-                                                               : solve_bicgstab_impl(ctx, iters, err);
+        if (x_given) {
+            HIPCHK(hipMemcpyAsync(ctx->x, x_given, (size_t)g.N * sizeof(double),
+                                  hipMemcpyDeviceToDevice, s));
         } else {
-            rc = solve_rbsor_impl(ctx, iters, err);
+            if (c.solver == CFDMX_SOLVER_BICGSTAB) {
+                rc = cfdmx_pressure_rhs(ctx);
+                if (!rc)
+                    rc = c.precision != CFDMX_PRECISION_FP64 ? solve_mixed_impl(ctx, iters, err)
+                                                             : solve_bicgstab_impl(ctx, iters, err);
+            } else {
+                rc = solve_rbsor_impl(ctx, iters, err);
+            }
+            if (rc) return rc;
+            if (!(*err < c.tol)) ++ctx->unconverged;
         }
-        if (rc) return rc;
-        if (!(*err < c.tol)) ++ctx->unconverged;
     }
     const int from_x = (c.solver == CFDMX_SOLVER_BICGSTAB && *solved) ? 1 : 0;
     // the steady-state check reads T0 (u^n) beside the new field the projection writes
@@ -2101,8 +2117,8 @@ static int adj_check(cfdmx_ctx *ctx, int nsteps, int K)
 }
 
 // one time step onto tape slot `slot`: U^{n-1} and p before the step, X_result and the
-// solved flag after it
-static int adj_tape_step(cfdmx_ctx *ctx, int slot, int *iters)
+// solved flag after it; x_given as in step_once
+static int adj_tape_step(cfdmx_ctx *ctx, int slot, int *iters, const double *x_given = nullptr)
 {
     cfdmx_ctx::Adj &A = ctx->adj;
     const size_t G = ctx->g.G, N = ctx->g.N;
@@ -2112,7 +2128,7 @@ static int adj_tape_step(cfdmx_ctx *ctx, int slot, int *iters)
     if ((rc = d2d(ctx, A.tapeP + (size_t)slot * G, ctx->p, G))) return rc;
     int it = 0, solved = 0;
     double err = 0.0;
-    if ((rc = step_once(ctx, &it, &err, &solved))) return rc;
+    if ((rc = step_once(ctx, &it, &err, &solved, x_given))) return rc;
     if ((rc = d2d(ctx, A.tapeX + (size_t)slot * N, ctx->x, N))) return rc;
     A.solved[slot] = solved;
     if (iters) *iters = it;
@@ -2123,7 +2139,8 @@ static int adj_tape_step(cfdmx_ctx *ctx, int slot, int *iters)
 // Steps n >= tape_from (>= 0) go onto the tape at slot n - tape_from: the adjoint tapes
 // its last segment here, which the reverse pass then need not recompute (every kernel
 // is deterministic, so a recompute would give the same bits).  Checkpoints are kept
-// every K steps before that.
+// every K steps before that, and with A.allX the pressure solution of every step before
+// the tape.
 static int adj_forward(cfdmx_ctx *ctx, int nsteps, int K, int keep_checkpoints, int tape_from,
                        double *J, int *iters_total)
 {
@@ -2143,6 +2160,8 @@ static int adj_forward(cfdmx_ctx *ctx, int nsteps, int K, int keep_checkpoints, 
         double err = 0.0;
         int rc = taped ? adj_tape_step(ctx, n - tape_from, &it)
                        : step_once(ctx, &it, &err, &solved);
+        if (!rc && keep_checkpoints && !taped && ctx->adj.allX)
+            rc = d2d(ctx, ctx->adj.allX + (size_t)n * ctx->g.N, ctx->x, ctx->g.N);
         if (!rc) rc = read_scalars(ctx);
         if (rc) return rc;
         sum += ctx->scal_h[1];
@@ -2178,6 +2197,24 @@ int cfdmx_adjoint_drag(cfdmx_ctx *ctx, int nsteps, int checkpoint_every, double 
     rc = adj_alloc(ctx, K, ncp + 1);
     if (rc) return rc;
     cfdmx_ctx::Adj &A = ctx->adj;
+    // the pressure solutions of the steps the reverse pass recomputes (all but the taped
+    // last segment): N doubles per step, kept where the device has room, else the
+    // recompute solves again (the same bits either way)
+    {
+        const size_t need = (size_t)(ncp - 1) * K * ctx->g.N;
+        if (need > A.allX_n) {
+            if (A.allX) (void)hipFree(A.allX);
+            A.allX = nullptr;
+            A.allX_n = 0;
+            void *q = nullptr;
+            if (need > 0 && hipMalloc(&q, need * sizeof(double)) == hipSuccess) {
+                A.allX = static_cast<double *>(q);
+                A.allX_n = need;
+            } else {
+                (void)hipGetLastError(); // an allocation failure is not an error here
+            }
+        }
+    }
     hipStream_t s = ctx->stream;
     const Geo &g = ctx->g;
     const cfdmx_config &c = ctx->cfg;
@@ -2221,7 +2258,9 @@ int cfdmx_adjoint_drag(cfdmx_ctx *ctx, int nsteps, int checkpoint_every, double 
             rc = adj_restore(ctx, seg);
             if (rc) return rc;
             for (int n = n0; n < n1; ++n)
-                if ((rc = adj_tape_step(ctx, n - n0, nullptr))) return rc;
+                if ((rc = adj_tape_step(ctx, n - n0, nullptr,
+                                        A.allX ? A.allX + (size_t)n * N : nullptr)))
+                    return rc;
         }
         for (int n = n1 - 1; n >= n0; --n) {
             const int slot = n - n0;
