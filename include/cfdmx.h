@@ -291,8 +291,11 @@ CFDMX_API int cfdmx_drag_objective(cfdmx_ctx *ctx, int nsteps, double *J);
  * not read; all zero without cfdmx_set_inflow) and dJ/d(cx, cy, R) of the smoothed mask
  * set by cfdmx_set_eta_smoothed (grad_mask[3]); either may be NULL.  checkpoint_every
  * = K bounds the tape to K steps (memory: about (4G + N) * 8 bytes per tape step and
- * (4G + 3N) * 8 per checkpoint, plus 63 G * 8 for the conv-diff rows).  The results
- * are the same bits for every K; K >= nsteps tapes every step and recomputes none. */
+ * (4G + 3N) * 8 per checkpoint, plus 63 G * 8 for the conv-diff rows).  Where the device
+ * has room, the forward pass also keeps the pressure solution of every step before the
+ * last segment (N * 8 bytes per step), and the recompute of the segments then copies it
+ * instead of solving.  The results are the same bits for every K and either way;
+ * K >= nsteps tapes every step and recomputes none. */
 CFDMX_API int cfdmx_adjoint_drag(cfdmx_ctx *ctx, int nsteps, int checkpoint_every, double *J,
                        double *grad_inflow, double *grad_mask, cfdmx_adjoint_info *info);
 
