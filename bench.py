@@ -463,7 +463,9 @@ def adjoint_run(seed, fd=True):
         "adjoint_step_cost": round(info["reverse_ms"] / (plain * 1e3), 3),
         "adjoint_step_cost_note": ("wall(recompute + adjoint step) / wall(forward step); "
                                    "the reverse pass re-runs every checkpointed segment "
-                                   "but the last, which the forward pass tapes"),
+                                   "but the last, which the forward pass tapes, with the "
+                                   "pressure solutions the forward pass kept in place of "
+                                   "the solves"),
         "gradient_cost": round((info["forward_ms"] + info["reverse_ms"]) / (plain * 1e3), 3),
         "gradient_cost_note": ("wall(checkpointed forward + reverse pass) / wall(plain "
                                "forward): J and both gradients in plain forward runs"),
