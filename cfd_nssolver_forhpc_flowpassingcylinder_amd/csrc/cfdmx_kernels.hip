@@ -163,7 +163,6 @@ k_convdiff(Geo g, const double *__restrict__ u, const double *__restrict__ v,
     const int kt = b % g.cd_nkt, jt = (b / g.cd_nkt) % g.cd_njt, ch = b / (g.cd_nkt * g.cd_njt);
     const int k0 = kt * TK, j0 = jt * TJ, i0 = ch * kChunk;
     const int I0 = i0 + kGC, len = min(kChunk, g.nxc - i0);
-    const int sx = g.sx, sy = g.sy;
     double *ring = lds;                 // [slot][field][P]
     double *mt = lds + 9 * P;           // metric tables
     const Axis X = cd_axis_lds(g.ax[0], I0 - 2, kChunk + 4, g.nx, mt);
@@ -940,7 +939,7 @@ k_rhs4(Geo g, const double *__restrict__ us, const double *__restrict__ vs,
 __global__ void __launch_bounds__(kThreads)
 k_bicg_init(Geo g, int pre, const double *__restrict__ b, const double *__restrict__ x,
             double *__restrict__ r0, double *__restrict__ r, double *__restrict__ px1,
-            double *__restrict__ part, KState *st)
+            double *__restrict__ part, KState *)
 {
     const int L = dealt_block();
     if (L >= g.nblocks) return;
@@ -975,7 +974,7 @@ k_bicg_init(Geo g, int pre, const double *__restrict__ b, const double *__restri
 __global__ void __launch_bounds__(kThreads)
 k_bicg_init4(Geo g, int pre, const double *__restrict__ b, const double *__restrict__ x,
              double *__restrict__ r0, double *__restrict__ r, double *__restrict__ px1,
-             double *__restrict__ part, KState *st)
+             double *__restrict__ part, KState *)
 {
     const int L = dealt_block();
     if (L >= g.nblocks) return;
