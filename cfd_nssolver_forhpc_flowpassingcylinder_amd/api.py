@@ -31,8 +31,8 @@ EXPORTS = [
     "cfdmx_set_eta_cylinder_z", "cfdmx_set_eta_smoothed", "cfdmx_convdiff", "cfdmx_bc_main",
     "cfdmx_bc_copy", "cfdmx_pressure_rhs", "cfdmx_solve_bicgstab", "cfdmx_solve_rbsor",
     "cfdmx_transform", "cfdmx_project_ib", "cfdmx_swap_t0_t3", "cfdmx_cd_cl", "cfdmx_cfl",
-    "cfdmx_apply_operator", "cfdmx_step", "cfdmx_synchronize", "cfdmx_profile",
-    "cfdmx_profile_read", "cfdmx_set_tolerance", "cfdmx_set_physics", "cfdmx_upload_force",
+    "cfdmx_apply_operator", "cfdmx_apply_operator_transpose", "cfdmx_step",
+    "cfdmx_synchronize", "cfdmx_profile", "cfdmx_profile_read", "cfdmx_set_tolerance", "cfdmx_set_physics", "cfdmx_upload_force",
     "cfdmx_drag_objective", "cfdmx_adjoint_drag", "cfdmx_plot3d_q", "cfdmx_plot3d_flush",
     "cfdmx_plot3d_x", "cfdmx_checkpoint_save", "cfdmx_checkpoint_load", "cfdmx_smagorinsky",
     "cfdmx_download_viseff", "cfdmx_cd_cl_noalloc",
@@ -128,6 +128,7 @@ def load_library(path=LIB_PATH):
     L.cfdmx_cd_cl_noalloc.argtypes = [vp, vp, vp]
     L.cfdmx_cfl.argtypes = [vp, DP]
     L.cfdmx_apply_operator.argtypes = [vp, DP, DP]
+    L.cfdmx_apply_operator_transpose.argtypes = [vp, DP, DP]
     L.cfdmx_step.argtypes = [vp, C.c_int, C.POINTER(StepInfo)]
     L.cfdmx_profile.argtypes = [vp, C.c_int]
     L.cfdmx_set_tolerance.argtypes = [vp, C.c_double, C.c_int]
@@ -354,6 +355,14 @@ class FlowSolver:
         x = np.ascontiguousarray(x, dtype=np.float64)
         r = np.empty(self.N)
         self._chk(self.L.cfdmx_apply_operator(self.h, _p(x), _p(r)), "apply_operator")
+        return r
+
+    def apply_operator_transpose(self, y):
+        """A^T y: the transpose of apply_operator's matrix (no reference counterpart)."""
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        r = np.empty(self.N)
+        self._chk(self.L.cfdmx_apply_operator_transpose(self.h, _p(y), _p(r)),
+                  "apply_operator_transpose")
         return r
 
     def step(self, n=1):

@@ -1594,6 +1594,16 @@ int cfdmx_apply_operator(cfdmx_ctx *ctx, const double *x, double *r)
     return copy_d2h(ctx, r, ctx->B.r, ctx->g.N);
 }
 
+int cfdmx_apply_operator_transpose(cfdmx_ctx *ctx, const double *y, double *r)
+{
+    if (!ctx) return CFDMX_ERR_ARG;
+    int rc = copy_h2d(ctx, ctx->B.as, y, ctx->g.N);
+    if (rc) return rc;
+    launch_apply_t(ctx->g, ctx->stream, ctx->cfg.precond, ctx->B.as, ctx->B.r);
+    LAUNCHCHK();
+    return copy_d2h(ctx, r, ctx->B.r, ctx->g.N);
+}
+
 // one iteration of the runSeqOmp loop body (RUN_CPU.cpp:257-436).  x_given (the
 // adjoint's recompute): the pressure solution this step is known to reach, copied into
 // X_result in place of the right-hand side and the solve; every other stage is the same,

@@ -70,6 +70,7 @@ void launch_bc_copy(const Geo &g, hipStream_t s, const double *u0, const double 
 void launch_rhs(const Geo &g, hipStream_t s, int pre, const double *us, const double *vs,
                 const double *ws, double dt, double *b);
 void launch_apply(const Geo &g, hipStream_t s, int pre, const double *x, double *r);
+void launch_apply_t(const Geo &g, hipStream_t s, int pre, const double *y, double *r);
 void launch_bicg_init(const Geo &g, hipStream_t s, int pre, const double *b, const double *x,
                       double *r0, double *r, double *px1, double *part, KState *st);
 // one BiCGSTAB iteration m = three kernels; each folds the previous phase's partials

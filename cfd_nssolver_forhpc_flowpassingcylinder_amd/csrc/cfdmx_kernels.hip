@@ -818,6 +818,48 @@ k_apply(Geo g, const double *__restrict__ x, double *__restrict__ r)
     }
 }
 
+// The transpose, r = A^T y (no reference counterpart): what the adjoint of the pressure
+// solve needs where A is not symmetric (the SPE operator on the SML grid, with its
+// stretched widths and y-slot swap; the row-scaled CSR operator).  Every coupling is
+// mutual in position: row n has a neighbour in slot q exactly when that row holds n in
+// slot 6 - q, also across a periodic z wrap.  So column n takes, in a row's slot order,
+// slot 6 - q of the row in its slot q, from +0.0, skipping the absent slots.  Each
+// neighbour row's coefficients are formed by row_coef, as in k_apply.
+template <int PRE>
+__global__ void __launch_bounds__(kThreads)
+k_apply_t(Geo g, const double *__restrict__ y, double *__restrict__ r)
+{
+    const int L = logical_block(g);
+    if (L >= g.nblocks) return;
+    for (int c = 0; c < kCells; ++c) {
+        const int n = L * kPerBlock + c * kThreads + threadIdx.x;
+        if (n >= g.N) break;
+        int i, j, k;
+        decode(g, n, i, j, k);
+        RowCoef me;
+        row_coef<PRE>(g, i, j, k, me);
+        const int km = k == 0 ? g.nzc - 1 : k - 1, kp = k == g.nzc - 1 ? 0 : k + 1;
+        const int ni[7] = {i - 1, i, i, i, i, i, i + 1};
+        const int nj[7] = {j, j - 1, j, j, j, j + 1, j};
+        const int nk[7] = {k, k, km, k, kp, k, k};
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < 7; ++q) {
+            if (!me.has[q]) continue;
+            double v;
+            if (q == 3) {
+                v = me.v[3];
+            } else {
+                RowCoef nb;
+                row_coef<PRE>(g, ni[q], nj[q], nk[q], nb);
+                v = nb.v[6 - q];
+            }
+            s += v * y[n + me.off[q]];
+        }
+        r[n] = s;
+    }
+}
+
 // createBMatrix_omp (b_matrix.hpp:47-93): b = -(1/dt) div u*, times jp on the CSR path
 template <int PRE>
 __global__ void __launch_bounds__(kThreads)
@@ -2551,6 +2593,14 @@ void launch_apply(const Geo &g, hipStream_t s, int pre, const double *x, double 
         hipLaunchKernelGGL(k_apply<1>, dim3(grid_blocks(g)), dim3(kThreads), 0, s, g, x, r);
     else
         hipLaunchKernelGGL(k_apply<0>, dim3(grid_blocks(g)), dim3(kThreads), 0, s, g, x, r);
+}
+
+void launch_apply_t(const Geo &g, hipStream_t s, int pre, const double *y, double *r)
+{
+    if (pre)
+        hipLaunchKernelGGL(k_apply_t<1>, dim3(grid_blocks(g)), dim3(kThreads), 0, s, g, y, r);
+    else
+        hipLaunchKernelGGL(k_apply_t<0>, dim3(grid_blocks(g)), dim3(kThreads), 0, s, g, y, r);
 }
 
 void launch_bicg_init(const Geo &g, hipStream_t s, int pre, const double *b, const double *x,

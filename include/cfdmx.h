@@ -260,6 +260,13 @@ CFDMX_API int cfdmx_cfl(cfdmx_ctx *ctx, double *cfl);
 /* Pressure operator on host vectors of N doubles: SPE_matrix::multiply_omp
  * (SPE_sparseMatrix.hpp:92-106) or CSR_matrix::multiply_omp (CSR_sparseMatrix.hpp:231-245). */
 CFDMX_API int cfdmx_apply_operator(cfdmx_ctx *ctx, const double *x, double *r);
+/* Its transpose, r = A^T y, on host vectors of N doubles.  No counterpart in the reference:
+ * the adjoint of the pressure solve needs it wherever A is not symmetric, i.e. the SPE
+ * operator on the SML grid (stretched widths and the y-slot swap, SPE_sparseMatrix.hpp:
+ * 79,83) and the row-scaled CSR operator.  Column n sums, in a row's slot order (-x, -y,
+ * -z, diag, +z, +y, +x), the entries the neighbouring rows hold for it.  Like
+ * cfdmx_apply_operator it overwrites two solver work vectors that every solve sets up. */
+CFDMX_API int cfdmx_apply_operator_transpose(cfdmx_ctx *ctx, const double *y, double *r);
 
 /* n whole time steps (the loop body of runSeqOmp, RUN_CPU.cpp:257-436, without file
  * output).  info (may be NULL) receives the last step's report. */

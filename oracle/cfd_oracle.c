@@ -818,6 +818,81 @@ void orc_apply_A(const orc_grid *g, int precond, int z_periodic, const double *x
             }
 }
 
+/* The seven slot values (-x, -y, -z, diag, +z, +y, +x) of row (i, j, k) of the operator
+ * orc_apply_A applies, each formed by the expression it uses there (a slot without a
+ * neighbour gets its value anyway; callers skip it). */
+static void row_vals(const orc_grid *g, int precond, int z_periodic, int i, int j, int k,
+                     double v[7])
+{
+    if (precond == ORC_PRECOND_JACOBI) {
+        const double jp = 1.0 / jacobi_diag(g, z_periodic, i, j, k);
+        const double ijk = jacobi_diag(g, z_periodic, i, j, k);
+        v[0] = -cxm(g, i) * jp;
+        v[1] = -cym(g, j) * jp;
+        v[2] = -czm(g, k) * jp;
+        v[3] = ijk * jp;
+        v[4] = -czp(g, k) * jp;
+        v[5] = -cyp(g, j) * jp;
+        v[6] = -cxp(g, i) * jp;
+    } else {
+        v[0] = -cxm(g, i);
+        v[1] = -cyp(g, j); /* the y-slot swap, as in orc_spe_build */
+        v[2] = -czm(g, k);
+        v[3] = spe_diag(g, z_periodic, i, j, k);
+        v[4] = -czp(g, k);
+        v[5] = -cym(g, j);
+        v[6] = -cxp(g, i);
+    }
+}
+
+/* The transpose of orc_apply_A: r = A^T y (no reference counterpart).  The adjoint of the
+ * pressure solve needs it wherever A is not symmetric: the SPE operator on the SML grid
+ * (stretched widths, and the y-slot swap) and the row-scaled CSR operator.  Every coupling
+ * is mutual in position: row n has a neighbour in slot q exactly when that row has n in
+ * slot 6 - q (also across a periodic z wrap).  So column n takes, in a row's slot order,
+ * slot 6 - q of the row in its slot q, starting from +0.0 and skipping absent slots. */
+void orc_apply_AT(const orc_grid *g, int precond, int z_periodic, const double *y, double *r)
+{
+    const int nxc = g->nxc, nyc = g->nyc, nzc = g->nzc;
+    const long sy = nzc, sx = (long)nyc * nzc;
+    double v[7];
+    for (int i = 0; i < nxc; ++i)
+        for (int j = 0; j < nyc; ++j)
+            for (int k = 0; k < nzc; ++k) {
+                const long n = ((long)i * nyc + j) * nzc + k;
+                const int kfirst = (k == 0), klast = (k == nzc - 1);
+                const int km = kfirst ? nzc - 1 : k - 1, kp = klast ? 0 : k + 1;
+                double s = 0.0;
+                if (i != 0) {
+                    row_vals(g, precond, z_periodic, i - 1, j, k, v);
+                    s += v[6] * y[n - sx];
+                }
+                if (j != 0) {
+                    row_vals(g, precond, z_periodic, i, j - 1, k, v);
+                    s += v[5] * y[n - sy];
+                }
+                if (!kfirst || z_periodic) {
+                    row_vals(g, precond, z_periodic, i, j, km, v);
+                    s += v[4] * y[n + (km - k)];
+                }
+                row_vals(g, precond, z_periodic, i, j, k, v);
+                s += v[3] * y[n];
+                if (!klast || z_periodic) {
+                    row_vals(g, precond, z_periodic, i, j, kp, v);
+                    s += v[2] * y[n + (kp - k)];
+                }
+                if (j != nyc - 1) {
+                    row_vals(g, precond, z_periodic, i, j + 1, k, v);
+                    s += v[1] * y[n + sy];
+                }
+                if (i != nxc - 1) {
+                    row_vals(g, precond, z_periodic, i + 1, j, k, v);
+                    s += v[0] * y[n + sx];
+                }
+                r[n] = s;
+            }
+}
+
 /* ------------------------------------------------------------------------ */
 /* BiCGSTAB: solver::bicgstabRe2 bicgstabRe2.hpp:17-180                      */
 /* ------------------------------------------------------------------------ */

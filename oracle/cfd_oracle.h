@@ -106,6 +106,7 @@ void orc_bc_copy(const orc_grid *g, const double *u0, const double *v0, const do
 void orc_rhs(const orc_grid *g, double dt, int precond, int z_periodic,
              const double *us, const double *vs, const double *ws, double *b);
 void orc_apply_A(const orc_grid *g, int precond, int z_periodic, const double *x, double *r);
+void orc_apply_AT(const orc_grid *g, int precond, int z_periodic, const double *y, double *r);
 void orc_spe_build(const orc_grid *g, double *val);
 void orc_spe_apply_vals(const orc_grid *g, const double *val, const double *x, double *r);
 

@@ -124,6 +124,7 @@ class Oracle:
         L.orc_bc_copy.argtypes = [G] + [DP] * 6
         L.orc_rhs.argtypes = [G, C.c_double, C.c_int, C.c_int] + [DP] * 4
         L.orc_apply_A.argtypes = [G, C.c_int, C.c_int, DP, DP]
+        L.orc_apply_AT.argtypes = [G, C.c_int, C.c_int, DP, DP]
         L.orc_spe_build.argtypes = [G, DP]
         L.orc_spe_apply_vals.argtypes = [G, DP, DP, DP]
         L.orc_bicg_new.restype = C.c_void_p
@@ -219,6 +220,12 @@ class Oracle:
         r = np.zeros(g.N)
         self.lib.orc_apply_A(C.byref(g.c), precond, z_periodic, ptr(np.ascontiguousarray(x)),
                              ptr(r))
+        return r
+
+    def apply_AT(self, g, y, precond=PRECOND_NONE, z_periodic=0):
+        r = np.zeros(g.N)
+        self.lib.orc_apply_AT(C.byref(g.c), precond, z_periodic, ptr(np.ascontiguousarray(y)),
+                              ptr(r))
         return r
 
     def spe_vals(self, g):

@@ -112,6 +112,34 @@ def test_operator_rhs_bicgstab_bitwise(precond, zper):
     gpu.close()
 
 
+# A^T y against the oracle's transpose, which test_operator_transpose.py pins to A itself;
+# the SML cases are where A is not symmetric.  The dot-product identity is checked on the
+# library's own A and A^T.
+@pytest.mark.parametrize("kind,n,L,precond,zper",
+                         [(k, n, L, p, z)
+                          for k, n, L in ((0, (24, 20, 6), (20.0, 10.0, 0.5)),
+                                          (1, (112, 104, 4), (17.0, 12.0, 3.0)))
+                          for p in (0, 1) for z in (0, 1)]
+                         + [(1, (512, 256, 64), (20.0, 10.0, 0.5), 0, 1),
+                            (1, (512, 256, 64), (20.0, 10.0, 0.5), 1, 0)])
+def test_operator_transpose_bitwise(kind, n, L, precond, zper):
+    print(f"transpose: grid {kind} {n} precond {precond} zper {zper}", flush=True)
+    cfg = pkg.default_config(nx=n[0], ny=n[1], nz=n[2], lx=L[0], ly=L[1], lz=L[2],
+                             grid_kind=kind, precond=precond, z_periodic=zper)
+    gpu = pkg.FlowSolver(cfg)
+    o = op.Oracle()
+    g = o.grid(*n, *L, kind=kind)
+    rng = np.random.default_rng(13)
+    x, y = rng.standard_normal(g.N), rng.standard_normal(g.N)
+    aty = gpu.apply_operator_transpose(y)
+    assert np.array_equal(aty, o.apply_AT(g, y, precond, zper))
+    ax = gpu.apply_operator(x)
+    lhs, rhs = float(np.dot(y, ax)), float(np.dot(aty, x))
+    scale = float(np.dot(np.abs(y), np.abs(ax))) + float(np.dot(np.abs(aty), np.abs(x)))
+    assert abs(lhs - rhs) <= 1e-13 * scale
+    gpu.close()
+
+
 def _same(a, b):
     return a == b or (np.isnan(a) and np.isnan(b))
 
