@@ -1302,9 +1302,11 @@ static int solve_bicgstab_impl(cfdmx_ctx *ctx, int *iters, double *res)
 // BiCGSTAB preconditioned by a multigrid V-cycle (FP64 sums), in float for mixed
 // precision or in double for the FP64 multigrid mode, solves A e = r to the relative
 // tolerance max(inner_tol, 0.3 tol ||b|| / ||r||) and x += e in FP64.  The iteration
-// limit bounds the inner iterations summed over the outer passes.  A pass that raises
-// the FP64 residual (a stalled inner Krylov process) is taken back and ends the solve
-// with the previous x.
+// limit bounds the inner iterations summed over the outer passes.  Each pass is applied
+// only after the residual of x + e, formed without storing it, shows that it does not
+// raise the FP64 residual; a pass that does (a stalled inner Krylov process: on an
+// inconsistent system e can grow along the constant null vector until x + e has lost
+// x's low bits) is dropped, and the solve ends with x as it was, bit for bit.
 static constexpr int kMaxOuter = 40;
 // inner_tol = 0 takes the mode's default, the fastest measured at 512x256x64 (tol 1e-10,
 // extrapolated starts): 1e-5 in both, where one pass usually reaches tol.  With the
@@ -1336,6 +1338,9 @@ static int solve_pb_inner(cfdmx_ctx *ctx, double rr, double tol, int maxit, int 
     ps.done = (ps.norm0 > 0.0 && maxit > 0) ? 0 : 1;
     *ctx->pbs_h = ps;
     HIPCHK(hipMemcpyAsync(ctx->pbs, ctx->pbs_h, sizeof ps, hipMemcpyHostToDevice, s));
+    // e starts at 0: the first iteration's update writes it whole; a solve that ends
+    // before its first iteration leaves e = 0 here
+    if (ps.done) launch_fill(s, (int)g.N, 0.0, e);
     std::vector<MGLevelT<T>> &L = mg_levels<T>(ctx);
     for (int it = 0; !ctx->pbs_h->done; ++it) {
         PROF(CFDMX_K_PB, launch_pb_update_p<T>(g, s, it == 0, r, v, p, ctx->pbs));
@@ -1346,8 +1351,8 @@ static int solve_pb_inner(cfdmx_ctx *ctx, double rr, double tol, int maxit, int 
         PROF(CFDMX_K_MG, rc = run_vcycle<T>(ctx, sv, sh));
         if (rc) return rc;
         PROF(CFDMX_K_PB, launch_pb_spmv<T>(g, s, L[0], 2, sh, t, sv, ctx->part, ctx->pbs));
-        PROF(CFDMX_K_PB, launch_pb_update_xr<T>(g, s, ctx->part, ph, sh, sv, t, r0, e, r,
-                                                ctx->pb_part, ctx->pbs));
+        PROF(CFDMX_K_PB, launch_pb_update_xr<T>(g, s, it == 0, ctx->part, ph, sh, sv, t, r0, e,
+                                                r, ctx->pb_part, ctx->pbs));
         launch_pb_end(g, s, ctx->pb_part, ctx->pbs);
         LAUNCHCHK();
         HIPCHK(hipMemcpyAsync(ctx->pbs_h, ctx->pbs, sizeof(PBState), hipMemcpyDeviceToHost, s));
@@ -1367,48 +1372,44 @@ static int solve_refine(cfdmx_ctx *ctx, const double *b, double *x, int *iters, 
     const cfdmx_config &c = ctx->cfg;
     T *const *pv = reinterpret_cast<T *const *>(ctx->pb);
     double *e = static_cast<double *>(ctx->ecorr);
-    int total = 0, outer = 0;
-    double res64 = 0.0, res_prev = 0.0;
-    for (;;) {
-        if (outer > 0) PROF(CFDMX_K_BICG_INIT, launch_mixed_update(g, s, 1.0, x, e));
-        PROF(CFDMX_K_BICG_INIT, launch_mixed_resid<T>(g, s, 0, b, x, pv[0], pv[1], e, ctx->part));
+    // the FP64 residual of x, or of x + e (trial), into pv[0], pv[1]; returns
+    // ||r|| / ||b|| and the inner solve's start (T(r), T(r)) in rr
+    auto residual = [&](const double *ex, double &rr, double &rel) -> int {
+        PROF(CFDMX_K_BICG_INIT, launch_mixed_resid<T>(g, s, 0, b, x, ex, pv[0], pv[1], ctx->part));
         launch_mixed_fold(g, s, ctx->part, ctx->mix);
         LAUNCHCHK();
         HIPCHK(hipMemcpyAsync(ctx->mix_h, ctx->mix, 3 * sizeof(double), hipMemcpyDeviceToHost,
                               s));
         HIPCHK(hipStreamSynchronize(s));
         prof_harvest(ctx);
-        const double bnorm = std::sqrt(ctx->mix_h[0]);
-        res64 = std::sqrt(ctx->mix_h[1]) / bnorm;
-        if (outer > 0 && !(res64 <= res_prev)) {
-            // take the pass back: x - e is within a rounding of the previous x, so its
-            // residual is formed again and reported, not the previous pass's
-            PROF(CFDMX_K_BICG_INIT, launch_mixed_update(g, s, -1.0, x, e));
-            PROF(CFDMX_K_BICG_INIT,
-                 launch_mixed_resid<T>(g, s, 0, b, x, pv[0], pv[1], e, ctx->part));
-            launch_mixed_fold(g, s, ctx->part, ctx->mix);
-            LAUNCHCHK();
-            HIPCHK(hipMemcpyAsync(ctx->mix_h, ctx->mix, 3 * sizeof(double),
-                                  hipMemcpyDeviceToHost, s));
-            HIPCHK(hipStreamSynchronize(s));
-            prof_harvest(ctx);
-            res64 = std::sqrt(ctx->mix_h[1]) / std::sqrt(ctx->mix_h[0]);
-            break;
-        }
-        res_prev = res64;
+        rel = std::sqrt(ctx->mix_h[1]) / std::sqrt(ctx->mix_h[0]);
+        rr = ctx->mix_h[2];
+        return CFDMX_OK;
+    };
+    int total = 0, outer = 0;
+    double res64 = 0.0, rr = 0.0;
+    int rc = residual(nullptr, rr, res64);
+    if (rc) return rc;
+    const double itol = c.inner_tol > 0.0
+                            ? c.inner_tol
+                            : (std::is_same<T, float>::value ? kInnerTolMixed : kInnerTolFp64);
+    for (;;) {
         if (!(res64 >= c.tol)) break; // converged (or NaN: nothing left to refine)
         if (total >= c.maxit || outer >= kMaxOuter) break;
         int it = 0;
         double ires = 0.0;
-        const double itol = c.inner_tol > 0.0
-                                ? c.inner_tol
-                                : (std::is_same<T, float>::value ? kInnerTolMixed : kInnerTolFp64);
-        int rc = solve_pb_inner<T>(ctx, ctx->mix_h[2], std::fmax(itol, 0.3 * c.tol / res64),
-                                   c.maxit - total, &it, &ires);
+        rc = solve_pb_inner<T>(ctx, rr, std::fmax(itol, 0.3 * c.tol / res64), c.maxit - total,
+                               &it, &ires);
         if (rc) return rc;
         total += it;
         ++outer;
         if (!std::isfinite(ires)) break; // diverged: keep x, report res64
+        double rr_new = 0.0, res_new = 0.0;
+        if ((rc = residual(e, rr_new, res_new))) return rc;
+        if (!(res_new <= res64)) break; // the pass raises the residual: x stays as it was
+        PROF(CFDMX_K_BICG_INIT, launch_mixed_update(g, s, x, e));
+        res64 = res_new;
+        rr = rr_new;
     }
     ctx->mix_outer = outer;
     if (iters) *iters = total;

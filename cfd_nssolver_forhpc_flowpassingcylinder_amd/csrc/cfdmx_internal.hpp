@@ -134,17 +134,18 @@ template <typename T>
 void launch_pb_update_s(const Geo &g, hipStream_t s, const double *part, const T *r, const T *v,
                         T *sv, PBState *st);
 template <typename T>
-void launch_pb_update_xr(const Geo &g, hipStream_t s, const double *part_in, const T *ph,
-                         const T *sh, const T *sv, const T *t, const T *r0, double *e, T *r,
-                         double *part_out, PBState *st);
+void launch_pb_update_xr(const Geo &g, hipStream_t s, int first, const double *part_in,
+                         const T *ph, const T *sh, const T *sv, const T *t, const T *r0,
+                         double *e, T *r, double *part_out, PBState *st);
 void launch_pb_end(const Geo &g, hipStream_t s, const double *part, PBState *st);
 
-// iterative refinement: the FP64 residual into r, r0 (e = 0; the correction e is FP64 in
-// both modes), x += sign*e, fold of the three sums
+// iterative refinement: the FP64 residual of x, or of x + e when e is given, into r, r0
+// (the correction e, FP64 in both modes, starts at 0 through the inner solve's first
+// update), x += e, fold of the three sums
 template <typename T>
 void launch_mixed_resid(const Geo &g, hipStream_t s, int pre, const double *b, const double *x,
-                        T *r, T *r0, double *e, double *part);
-void launch_mixed_update(const Geo &g, hipStream_t s, double sign, double *x, const double *e);
+                        const double *e, T *r, T *r0, double *part);
+void launch_mixed_update(const Geo &g, hipStream_t s, double *x, const double *e);
 void launch_mixed_fold(const Geo &g, hipStream_t s, const double *part, double *out);
 
 void launch_bicg_fin0(const Geo &g, hipStream_t s, double *part, KState *st);

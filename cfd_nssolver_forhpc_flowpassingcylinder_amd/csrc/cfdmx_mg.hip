@@ -777,8 +777,10 @@ __global__ void k_pb_update_s(Geo g, const T *r, const T *v, T *s, const PBState
 // and its increments are FP64 in both modes: in FP32 its own rounding (6e-8 of e),
 // multiplied by the operator's condition number (~1e5 here), would leave a true residual
 // near 1e-2 of the pass's start, whatever the inner tolerance; in FP64 the pass's floor is
-// the FP32 recursion's drift from the true residual, ~1e-6
-template <typename T>
+// the FP32 recursion's drift from the true residual, ~1e-6.  FIRST (the solve's first
+// iteration, e = 0): e is written as +0.0 plus the increment without being read, which is
+// the sum the zeroed vector would give, -0.0 turning to +0.0 included.
+template <typename T, bool FIRST>
 __global__ void __launch_bounds__(kThreads)
 k_pb_update_xr(Geo g, const T *ph, const T *sh, const T *s, const T *t, const T *r0, double *e,
                T *r, double *part_out, const PBState *st)
@@ -792,7 +794,9 @@ k_pb_update_xr(Geo g, const T *ph, const T *sh, const T *s, const T *t, const T 
     for (int c = 0; c < kCells; ++c) {
         const int n = B * kPerBlock + c * kThreads + threadIdx.x;
         if (n < g.N) {
-            e[n] += ad * (double)ph[n] + omd * (double)sh[n];
+            const double inc = ad * (double)ph[n] + omd * (double)sh[n];
+            if constexpr (FIRST) e[n] = 0.0 + inc;
+            else e[n] += inc;
             const T rn = s[n] - om * t[n];
             r[n] = rn;
             acc[0] += (double)rn * (double)rn;
@@ -950,13 +954,17 @@ void launch_pb_update_s(const Geo &g, hipStream_t s, const double *part, const T
 }
 
 template <typename T>
-void launch_pb_update_xr(const Geo &g, hipStream_t s, const double *part_in, const T *ph,
-                         const T *sh, const T *sv, const T *t, const T *r0, double *e, T *r,
-                         double *part_out, PBState *st)
+void launch_pb_update_xr(const Geo &g, hipStream_t s, int first, const double *part_in,
+                         const T *ph, const T *sh, const T *sv, const T *t, const T *r0,
+                         double *e, T *r, double *part_out, PBState *st)
 {
     hipLaunchKernelGGL(k_pb_omega, dim3(1), dim3(kThreads), 0, s, g, part_in, st);
-    hipLaunchKernelGGL(k_pb_update_xr<T>, dim3(lin_blocks(g)), dim3(kThreads), 0, s, g, ph, sh,
-                       sv, t, r0, e, r, part_out, (const PBState *)st);
+    if (first)
+        hipLaunchKernelGGL((k_pb_update_xr<T, true>), dim3(lin_blocks(g)), dim3(kThreads), 0, s,
+                           g, ph, sh, sv, t, r0, e, r, part_out, (const PBState *)st);
+    else
+        hipLaunchKernelGGL((k_pb_update_xr<T, false>), dim3(lin_blocks(g)), dim3(kThreads), 0, s,
+                           g, ph, sh, sv, t, r0, e, r, part_out, (const PBState *)st);
 }
 
 void launch_pb_end(const Geo &g, hipStream_t s, const double *part, PBState *st)
@@ -972,9 +980,9 @@ void launch_pb_end(const Geo &g, hipStream_t s, const double *part, PBState *st)
                                         T *, const PBState *);                                \
     template void launch_pb_update_s<T>(const Geo &, hipStream_t, const double *, const T *,  \
                                         const T *, T *, PBState *);                           \
-    template void launch_pb_update_xr<T>(const Geo &, hipStream_t, const double *, const T *, \
+    template void launch_pb_update_xr<T>(const Geo &, hipStream_t, int, const double *,       \
                                          const T *, const T *, const T *, const T *,          \
-                                         double *, T *, double *, PBState *);
+                                         const T *, double *, T *, double *, PBState *);
 CFDMX_MG_INSTANCES(float)
 CFDMX_MG_INSTANCES(double)
 #undef CFDMX_MG_INSTANCES

@@ -77,8 +77,9 @@ typedef struct cfdmx_config {
                          0 (the default) takes the mode's fastest measured value,
                          1e-5 in both modes.  The FP64 refinement stops
                          at tol, or unconverged (cfdmx_step_info.unconverged) after 40
-                         passes, at maxit inner iterations in all, on a pass that raises
-                         the residual (taken back) or on an inner divergence */
+                         passes, at maxit inner iterations in all, on a pass that would
+                         raise the residual (dropped: x stays as it was) or on an inner
+                         divergence */
     /* launch shaping */
     int xcd_remap; /* map consecutive logical blocks to one XCD (0 = hardware order) */
     int mg_graphs; /* multigrid modes: replay each V-cycle from a captured HIP graph
@@ -120,7 +121,7 @@ typedef struct cfdmx_step_info {
     double maxdiff[3];
     /* cumulative pressure solves that ended with err not below tol: the iteration
      * limit, and in the multigrid modes also the 40-pass limit of the refinement, a
-     * refinement pass that raised the residual (taken back) or an inner solve that
+     * refinement pass that would raise the residual (dropped) or an inner solve that
      * diverged; err then holds the residual reached */
     int unconverged;
 } cfdmx_step_info;
