@@ -41,6 +41,7 @@ struct cfdmx_ctx {
     // three folded sums of the FP64 residual
     void *pb[8] = {};
     void *ecorr = nullptr;
+    double *xback = nullptr; // x before the last refinement pass, to drop a rising pass
     std::vector<MGLevelT<float>> mg32;
     std::vector<MGLevelT<double>> mg64;
     PBState *pbs = nullptr, *pbs_h = nullptr;
@@ -420,6 +421,7 @@ template <typename T> static int setup_multigrid(cfdmx_ctx *ctx)
         double *z = nullptr; // the correction is FP64 in both modes (k_pb_update_xr)
         ALLOC(z, N);
         ctx->ecorr = z;
+        ALLOC(ctx->xback, N);
     }
     std::vector<MGLevelT<T>> &lev = mg_levels<T>(ctx);
     lev.clear();
@@ -1302,11 +1304,10 @@ static int solve_bicgstab_impl(cfdmx_ctx *ctx, int *iters, double *res)
 // BiCGSTAB preconditioned by a multigrid V-cycle (FP64 sums), in float for mixed
 // precision or in double for the FP64 multigrid mode, solves A e = r to the relative
 // tolerance max(inner_tol, 0.3 tol ||b|| / ||r||) and x += e in FP64.  The iteration
-// limit bounds the inner iterations summed over the outer passes.  Each pass is applied
-// only after the residual of x + e, formed without storing it, shows that it does not
-// raise the FP64 residual; a pass that does (a stalled inner Krylov process: on an
-// inconsistent system e can grow along the constant null vector until x + e has lost
-// x's low bits) is dropped, and the solve ends with x as it was, bit for bit.
+// limit bounds the inner iterations summed over the outer passes.  A pass that raises
+// the FP64 residual (a stalled inner Krylov process: on an inconsistent system e can grow
+// along the constant null vector until x + e has lost x's low bits) is dropped: x is
+// restored from the copy the update kept, bit for bit, and the solve ends there.
 static constexpr int kMaxOuter = 40;
 // inner_tol = 0 takes the mode's default, the fastest measured at 512x256x64 (tol 1e-10,
 // extrapolated starts): 1e-5 in both, where one pass usually reaches tol.  With the
@@ -1372,10 +1373,10 @@ static int solve_refine(cfdmx_ctx *ctx, const double *b, double *x, int *iters, 
     const cfdmx_config &c = ctx->cfg;
     T *const *pv = reinterpret_cast<T *const *>(ctx->pb);
     double *e = static_cast<double *>(ctx->ecorr);
-    // the FP64 residual of x, or of x + e (trial), into pv[0], pv[1]; returns
-    // ||r|| / ||b|| and the inner solve's start (T(r), T(r)) in rr
-    auto residual = [&](const double *ex, double &rr, double &rel) -> int {
-        PROF(CFDMX_K_BICG_INIT, launch_mixed_resid<T>(g, s, 0, b, x, ex, pv[0], pv[1], ctx->part));
+    // the FP64 residual of x into pv[0], pv[1]; ||r|| / ||b||, and the inner solve's
+    // start (T(r), T(r)) in rr
+    auto residual = [&](double &rr, double &rel) -> int {
+        PROF(CFDMX_K_BICG_INIT, launch_mixed_resid<T>(g, s, 0, b, x, pv[0], pv[1], ctx->part));
         launch_mixed_fold(g, s, ctx->part, ctx->mix);
         LAUNCHCHK();
         HIPCHK(hipMemcpyAsync(ctx->mix_h, ctx->mix, 3 * sizeof(double), hipMemcpyDeviceToHost,
@@ -1388,7 +1389,7 @@ static int solve_refine(cfdmx_ctx *ctx, const double *b, double *x, int *iters, 
     };
     int total = 0, outer = 0;
     double res64 = 0.0, rr = 0.0;
-    int rc = residual(nullptr, rr, res64);
+    int rc = residual(rr, res64);
     if (rc) return rc;
     const double itol = c.inner_tol > 0.0
                             ? c.inner_tol
@@ -1404,10 +1405,15 @@ static int solve_refine(cfdmx_ctx *ctx, const double *b, double *x, int *iters, 
         total += it;
         ++outer;
         if (!std::isfinite(ires)) break; // diverged: keep x, report res64
+        PROF(CFDMX_K_BICG_INIT, launch_mixed_update(g, s, x, e, ctx->xback));
         double rr_new = 0.0, res_new = 0.0;
-        if ((rc = residual(e, rr_new, res_new))) return rc;
-        if (!(res_new <= res64)) break; // the pass raises the residual: x stays as it was
-        PROF(CFDMX_K_BICG_INIT, launch_mixed_update(g, s, x, e));
+        if ((rc = residual(rr_new, res_new))) return rc;
+        if (!(res_new <= res64)) {
+            // the pass raises the residual: x as it was, whose residual is res64
+            launch_copy(s, (int)g.N, ctx->xback, x);
+            LAUNCHCHK();
+            break;
+        }
         res64 = res_new;
         rr = rr_new;
     }

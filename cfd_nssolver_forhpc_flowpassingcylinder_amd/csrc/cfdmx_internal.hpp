@@ -139,13 +139,13 @@ void launch_pb_update_xr(const Geo &g, hipStream_t s, int first, const double *p
                          double *e, T *r, double *part_out, PBState *st);
 void launch_pb_end(const Geo &g, hipStream_t s, const double *part, PBState *st);
 
-// iterative refinement: the FP64 residual of x, or of x + e when e is given, into r, r0
-// (the correction e, FP64 in both modes, starts at 0 through the inner solve's first
-// update), x += e, fold of the three sums
+// iterative refinement: the FP64 residual into r, r0 (the correction e, FP64 in both
+// modes, starts at 0 through the inner solve's first update), x += e with x kept in xb,
+// fold of the three sums
 template <typename T>
 void launch_mixed_resid(const Geo &g, hipStream_t s, int pre, const double *b, const double *x,
-                        const double *e, T *r, T *r0, double *part);
-void launch_mixed_update(const Geo &g, hipStream_t s, double *x, const double *e);
+                        T *r, T *r0, double *part);
+void launch_mixed_update(const Geo &g, hipStream_t s, double *x, const double *e, double *xb);
 void launch_mixed_fold(const Geo &g, hipStream_t s, const double *part, double *out);
 
 void launch_bicg_fin0(const Geo &g, hipStream_t s, double *part, KState *st);

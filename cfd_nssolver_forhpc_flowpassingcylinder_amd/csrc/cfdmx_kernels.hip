@@ -1525,15 +1525,12 @@ k_march_C(Geo g, T *__restrict__ r, const T *__restrict__ ap, T *__restrict__ q,
 // an inner BiCGSTAB in T (float; double for the same algorithm in FP64).  The outer
 // residual r = b - A x is formed in FP64 on the same operator rows as k_bicg_init; the
 // inner solve A e = r starts from e = 0 with r0 = p = T(r).  Its first x/r update writes
-// e as 0 + its increment, so e needs no pass of its own here.  TRIAL: the residual of
-// x + e, each value x[m] + e[m] formed where it is read (the sum x += e then stores), so
-// that a pass is only applied once it is known not to raise the residual.  Partials:
-// (b,b), (r,r) in FP64 and (T(r), T(r)).
-template <typename T, bool TRIAL>
+// e as 0 + its increment, so e needs no pass of its own here.  Partials: (b,b), (r,r) in
+// FP64 and (T(r), T(r)).
+template <typename T>
 __global__ void __launch_bounds__(kThreads)
 k_mixed_resid(Geo g, int pre, const double *__restrict__ b, const double *__restrict__ x,
-              const double *__restrict__ e, T *__restrict__ r32, T *__restrict__ r0_32,
-              double *__restrict__ part)
+              T *__restrict__ r32, T *__restrict__ r0_32, double *__restrict__ part)
 {
     const int L = dealt_block();
     if (L >= g.nblocks) return;
@@ -1544,7 +1541,7 @@ k_mixed_resid(Geo g, int pre, const double *__restrict__ b, const double *__rest
         RowCoef rc;
         if (pre) row_coef<1>(g, i, j, k, rc);
         else row_coef<0>(g, i, j, k, rc);
-        const double ax = row_apply(rc, n, [&](int m) { return TRIAL ? x[m] + e[m] : x[m]; });
+        const double ax = row_apply(rc, n, [&](int m) { return x[m]; });
         const double bn = b[n];
         const double rr = bn - ax;
         const T rf = (T)rr;
@@ -1588,11 +1585,10 @@ template <> struct Vec4T<double> {
     }
 };
 
-template <typename T, bool TRIAL>
+template <typename T>
 __global__ void __launch_bounds__(kThreads)
 k_mixed_resid4(Geo g, int pre, const double *__restrict__ b, const double *__restrict__ x,
-               const double *__restrict__ e, T *__restrict__ r32, T *__restrict__ r0_32,
-               double *__restrict__ part)
+               T *__restrict__ r32, T *__restrict__ r0_32, double *__restrict__ part)
 {
     const int L = dealt_block();
     if (L >= g.nblocks) return;
@@ -1603,25 +1599,14 @@ k_mixed_resid4(Geo g, int pre, const double *__restrict__ b, const double *__res
         decode(g, n0, i, j, k0);
         double xc[4], xxm[4], xxp[4], xym[4], xyp[4], bv[4];
         const bool hxm = i != 0, hxp = i != g.nxc - 1, hym = j != 0, hyp = j != g.nyc - 1;
-        // x, or x + e (TRIAL), at four consecutive cells from m
-        auto xs = [&](int m, double (&v)[4]) {
-            load4(x + m, v);
-            if constexpr (TRIAL) {
-                double t[4];
-                load4(e + m, t);
-#pragma unroll
-                for (int q = 0; q < 4; ++q) v[q] = v[q] + t[q];
-            }
-        };
-        auto x1 = [&](int m) { return TRIAL ? x[m] + e[m] : x[m]; };
-        xs(n0, xc);
-        xs(hxm ? n0 - g.cyz : n0, xxm);
-        xs(hxp ? n0 + g.cyz : n0, xxp);
-        xs(hym ? n0 - g.nzc : n0, xym);
-        xs(hyp ? n0 + g.nzc : n0, xyp);
+        load4(x + n0, xc);
+        load4(x + (hxm ? n0 - g.cyz : n0), xxm);
+        load4(x + (hxp ? n0 + g.cyz : n0), xxp);
+        load4(x + (hym ? n0 - g.nzc : n0), xym);
+        load4(x + (hyp ? n0 + g.nzc : n0), xyp);
         load4(b + n0, bv);
-        const double zlo = x1(k0 > 0 ? n0 - 1 : (g.zper ? n0 + g.nzc - 1 : n0));
-        const double zhi = x1(k0 + 3 < g.nzc - 1 ? n0 + 4 : (g.zper ? n0 + 4 - g.nzc : n0 + 3));
+        const double zlo = x[k0 > 0 ? n0 - 1 : (g.zper ? n0 + g.nzc - 1 : n0)];
+        const double zhi = x[k0 + 3 < g.nzc - 1 ? n0 + 4 : (g.zper ? n0 + 4 - g.nzc : n0 + 3)];
         T rv[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -1645,12 +1630,16 @@ k_mixed_resid4(Geo g, int pre, const double *__restrict__ b, const double *__res
     block_sums<3>(acc, part, g.nblocks, L);
 }
 
-// x += e (the correction, accumulated in FP64 by the inner solve), once its trial
-// residual (k_mixed_resid4<T, true>) has shown that the pass does not raise the residual
-__global__ void k_mixed_update(int n, double *__restrict__ x, const double *__restrict__ e)
+// x += e (the correction, accumulated in FP64 by the inner solve), keeping x in xb so
+// that a pass found to raise the residual can be dropped with x restored bit for bit
+__global__ void k_mixed_update(int n, double *__restrict__ x, const double *__restrict__ e,
+                               double *__restrict__ xb)
 {
-    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x)
-        x[q] += e[q];
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        const double xo = x[q];
+        xb[q] = xo;
+        x[q] = xo + e[q];
+    }
 }
 
 // the start of a multigrid pressure solve from the last two solutions: x <- 2 x - xp,
@@ -2737,35 +2726,23 @@ void launch_bicg_check(const Geo &g, hipStream_t s, double *const part[3], KStat
 
 template <typename T>
 void launch_mixed_resid(const Geo &g, hipStream_t s, int pre, const double *b, const double *x,
-                        const double *e, T *r, T *r0, double *part)
+                        T *r, T *r0, double *part)
 {
-    const dim3 grid(grid_blocks(g)), block(kThreads);
-    if (g.nzc % 4 == 0 && kPerBlock == 4 * kThreads) {
-        if (e)
-            hipLaunchKernelGGL((k_mixed_resid4<T, true>), grid, block, 0, s, g, pre, b, x, e, r,
-                               r0, part);
-        else
-            hipLaunchKernelGGL((k_mixed_resid4<T, false>), grid, block, 0, s, g, pre, b, x, e, r,
-                               r0, part);
-    } else {
-        if (e)
-            hipLaunchKernelGGL((k_mixed_resid<T, true>), grid, block, 0, s, g, pre, b, x, e, r, r0,
-                               part);
-        else
-            hipLaunchKernelGGL((k_mixed_resid<T, false>), grid, block, 0, s, g, pre, b, x, e, r,
-                               r0, part);
-    }
+    if (g.nzc % 4 == 0 && kPerBlock == 4 * kThreads)
+        hipLaunchKernelGGL(k_mixed_resid4<T>, dim3(grid_blocks(g)), dim3(kThreads), 0, s, g, pre,
+                           b, x, r, r0, part);
+    else
+        hipLaunchKernelGGL(k_mixed_resid<T>, dim3(grid_blocks(g)), dim3(kThreads), 0, s, g, pre,
+                           b, x, r, r0, part);
 }
-void launch_mixed_update(const Geo &g, hipStream_t s, double *x, const double *e)
+void launch_mixed_update(const Geo &g, hipStream_t s, double *x, const double *e, double *xb)
 {
-    hipLaunchKernelGGL(k_mixed_update, dim3(2048), dim3(kThreads), 0, s, g.N, x, e);
+    hipLaunchKernelGGL(k_mixed_update, dim3(2048), dim3(kThreads), 0, s, g.N, x, e, xb);
 }
 template void launch_mixed_resid<float>(const Geo &, hipStream_t, int, const double *,
-                                        const double *, const double *, float *, float *,
-                                        double *);
+                                        const double *, float *, float *, double *);
 template void launch_mixed_resid<double>(const Geo &, hipStream_t, int, const double *,
-                                         const double *, const double *, double *, double *,
-                                         double *);
+                                         const double *, double *, double *, double *);
 void launch_extrapolate(const Geo &g, hipStream_t s, double *x, double *xp)
 {
     hipLaunchKernelGGL(k_extrapolate, dim3(2048), dim3(kThreads), 0, s, g.N, x, xp);
