@@ -131,3 +131,24 @@ def test_symmetric_on_the_uniform_grid_and_not_on_sml(o):
     y = rng.standard_normal(g.N)
     d = np.abs(o.apply_AT(g, y, 0, 0) - o.apply_A(g, y, 0, 0))
     assert d.max() > 1e-3 * np.abs(o.apply_A(g, y, 0, 0)).max()
+
+
+@pytest.mark.parametrize("nz", [1, 2, 3])
+@pytest.mark.parametrize("precond", [0, 1])
+def test_short_periodic_z(o, nz, precond):
+    """With periodic z and nz <= 2 a row couples to one cell through both z slots (to itself
+    at nz = 1), so the probing above cannot separate the entries; the dot-product identity
+    still pins the transpose there, on the uniform and (nz >= 2) the SML grid.  With one z
+    cell the SML grid's ghost widths (init_gC_dx, structureGrid.hpp:433-486) come out
+    zero, so its operator is undefined there, A included."""
+    rng = np.random.default_rng(17 + nz)
+    cases = [(op.GRID_UNIFORM, (9, 7, nz), (20.0, 10.0, 0.5))]
+    if nz >= 2:
+        cases.append((op.GRID_SML, (112, 104, nz), (17.0, 12.0, 3.0)))
+    for kind, n, L in cases:
+        g = o.grid(*n, *L, kind=kind)
+        x, y = rng.standard_normal(g.N), rng.standard_normal(g.N)
+        ax, aty = o.apply_A(g, x, precond, 1), o.apply_AT(g, y, precond, 1)
+        lhs, rhs = float(np.dot(y, ax)), float(np.dot(aty, x))
+        scale = float(np.dot(np.abs(y), np.abs(ax))) + float(np.dot(np.abs(aty), np.abs(x)))
+        assert abs(lhs - rhs) <= 1e-13 * scale
